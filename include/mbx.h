@@ -1,0 +1,253 @@
+/*
+ * mbx.h -- C ABI of the MI355X-native fused bitmap-index query path.
+ *
+ * Plain pointers and sizes only.  Every call returns an int status (MBX_OK = 0)
+ * and never exits the process; mbx_last_error() gives the detail.  Device
+ * pointers are gfx950 global memory obtained from mbx_alloc (or any hipMalloc
+ * allocation meeting the alignment rules below).
+ *
+ * Bitmap layout is the reference's (primitive.cuh:94-98): uint32 words, bit b
+ * of word k is row 32k+b, tail bits zero.  The kernels read two reference
+ * words at a time (one 64-row word per lane), so every bitmap pointer must be
+ * 8-byte aligned and readable up to ceil(N/64)*8 bytes; bitmaps made by this
+ * library are 256-byte aligned and padded with zero words.
+ *
+ * Ranges are half-open [lo, hi).  Values are unsigned little-endian of
+ * 1, 2, 4 or 8 bytes.  Sums are int64 (the reference sums into uint32 and
+ * wraps; the low 32 bits of every sum here equal that wrapped value).
+ */
+#ifndef MBX_H
+#define MBX_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Limits of a program, as primitive.cuh:121 (MAXBIN_PERCOL, MAXCOLS,
+ * MAXNINSTR). */
+#define MBX_MAXBIN_PERCOL 4
+#define MBX_MAXCOLS 4
+#define MBX_MAXNINSTR 6
+#define MBX_MAXKEYS 3
+#define MBX_MAXBOUNDS 8
+
+enum mbx_status {
+  MBX_OK = 0,
+  MBX_ERR_INVALID = 1,     /* bad argument */
+  MBX_ERR_OOM = 2,         /* device pool or host memory exhausted */
+  MBX_ERR_HIP = 3,         /* a HIP runtime call failed */
+  MBX_ERR_ALIGN = 4,       /* a bitmap pointer is not 8-byte aligned */
+  MBX_ERR_CAPACITY = 5,    /* a fixed capacity was exceeded */
+  MBX_ERR_NODEVICE = 6,    /* no usable GPU */
+  MBX_ERR_ADMISSION = 7    /* serving: request refused by memory admission */
+};
+
+typedef struct mbx_ctx mbx_ctx;   /* device, stream, memory pool */
+
+/* ---------------------------------------------------------------- context */
+/* Replaces the implicit CUDA context and merle/dumb_pool.cuh:39-95.
+ * pool_bytes = 0 lets the pool grow in 256 MiB arenas; otherwise it is one
+ * arena of exactly pool_bytes (rounded up to 256 B) and exhaustion is
+ * MBX_ERR_OOM. */
+int mbx_init(int device, size_t pool_bytes, mbx_ctx **out);
+int mbx_destroy(mbx_ctx *ctx);
+const char *mbx_strerror(int status);
+const char *mbx_last_error(void);     /* per host thread */
+int mbx_device_count(int *n);
+int mbx_sync(mbx_ctx *ctx);
+/* The HIP stream all launches of this context go to (hipStream_t). */
+void *mbx_stream(mbx_ctx *ctx);
+int mbx_set_stream(mbx_ctx *ctx, void *hip_stream);  /* NULL = own stream */
+
+/* ----------------------------------------------------------------- memory */
+/* Pool-backed, 256-byte aligned.  Replaces __cum (bmpidx_create.cu:4-9) and
+ * dumb_pool_t::alloc/free (merle/dumb_pool.cuh:57-92). */
+int mbx_alloc(mbx_ctx *ctx, size_t bytes, void **dptr);
+int mbx_free(mbx_ctx *ctx, void *dptr);
+int mbx_pool_stats(mbx_ctx *ctx, size_t *used, size_t *capacity,
+                   size_t *peak, size_t *nlive);
+int mbx_memcpy_h2d(mbx_ctx *ctx, void *dst, const void *src, size_t bytes);
+int mbx_memcpy_d2h(mbx_ctx *ctx, void *dst, const void *src, size_t bytes);
+int mbx_memcpy_d2d(mbx_ctx *ctx, void *dst, const void *src, size_t bytes);
+/* Fills with a kernel (no hipMemset), so it is safe inside graph capture. */
+int mbx_fill_u32(mbx_ctx *ctx, void *dst, uint32_t value, size_t count);
+
+/* A host-backed pool with the same allocator, for tests without a GPU and
+ * for admission planning.  Addresses are host memory. */
+typedef struct mbx_hostpool mbx_hostpool;
+int mbx_hostpool_create(size_t capacity, mbx_hostpool **out);
+int mbx_hostpool_alloc(mbx_hostpool *p, size_t bytes, void **ptr);
+int mbx_hostpool_free(mbx_hostpool *p, void *ptr);
+int mbx_hostpool_stats(mbx_hostpool *p, size_t *used, size_t *capacity,
+                       size_t *peak, size_t *nlive, size_t *largest_free);
+int mbx_hostpool_destroy(mbx_hostpool *p);
+
+/* ------------------------------------------------------- bitmap creation */
+/* A value source: a fact column (fk == NULL) or a dimension attribute read
+ * through a foreign-key column of the fact table. */
+typedef struct mbx_colref {
+  const void *data;       /* device: fact column or dimension attribute */
+  const uint32_t *fk;     /* device: foreign keys, or NULL */
+  uint32_t dimsz;         /* dimension rows; fk >= dimsz matches nothing */
+  uint32_t elem_bytes;    /* 1, 2, 4 or 8 */
+} mbx_colref;
+
+/* ceil(N/32): the reference's words per bitmap. */
+size_t mbx_bitmap_words(uint32_t nrows);
+/* Words between consecutive bitmaps of one slab: ceil(N/32) rounded up to 64
+ * words (256 B). */
+size_t mbx_bitmap_stride_words(uint32_t nrows);
+
+/* create_bin (bmpidx_create.cu:14-56): one pass over the column writes nbins
+ * bitmaps, bit j set iff mins[j] <= value < maxs[j]; a foreign key >= dimsz
+ * reads as INT64_MAX (:26-27).  One slab is allocated from the pool;
+ * out[j] = out[0] + j*stride; free only out[0].  *single_bytes (optional)
+ * gets ceil(N/32)*4 as the reference returns.  Host arrays mins/maxs/out. */
+int mbx_create_bin(mbx_ctx *ctx, const mbx_colref *src, uint32_t nrows,
+                   const uint64_t *mins, const uint64_t *maxs, size_t nbins,
+                   uint32_t **out, size_t *single_bytes);
+/* Same, into caller memory: nbins bitmaps at slab + j*stride_words. */
+int mbx_create_bin_into(mbx_ctx *ctx, const mbx_colref *src, uint32_t nrows,
+                        const uint64_t *mins, const uint64_t *maxs,
+                        size_t nbins, uint32_t *slab, size_t stride_words);
+/* create_select / create_join (bmpidx_create.cu:58-136): one bitmap;
+ * *out = NULL and MBX_ERR_OOM on allocation failure. */
+int mbx_create_select(mbx_ctx *ctx, const void *src, uint32_t elem_bytes,
+                      uint32_t nrows, uint32_t lo, uint32_t hi,
+                      uint32_t **out);
+int mbx_create_join(mbx_ctx *ctx, const uint32_t *fk, const void *dimattr,
+                    uint32_t elem_bytes, uint32_t nrows, uint32_t dimsz,
+                    uint32_t lo, uint32_t hi, uint32_t **out);
+
+/* --------------------------------------------------------------- program */
+/* vprg (primitive.cuh:135-198).  Opcodes as vprg::ops. */
+enum mbx_opcode { MBX_END = 0, MBX_AND, MBX_OR, MBX_NOT, MBX_ANDM, MBX_ORM };
+typedef struct mbx_col {
+  const uint32_t *middle[MBX_MAXBIN_PERCOL]; /* bins inside the range */
+  const uint32_t *leftmost, *rightmost;      /* partly covered edge bins */
+} mbx_col;
+typedef struct mbx_instr {
+  uint8_t opcode, dstreg, src1, src2;        /* src1 also names a column */
+} mbx_instr;
+typedef struct mbx_vprg {
+  mbx_col cols[MBX_MAXCOLS];
+  uint32_t factsz;
+  mbx_instr instrs[MBX_MAXNINSTR];
+} mbx_vprg;
+
+/* The row operation: a data descriptor standing in for the reference's
+ * per-query device functor op_t (primitive.cuh:10-15, ssb/lambdas.cuh:19-152).
+ *  - preds[i] is the exact predicate behind vprg column i; an uncertain row
+ *    is kept iff the program's instruction list, run over these exact
+ *    predicate booleans, leaves r0 true.
+ *  - group key = sum_i stride_i * (f_i(v_i) - base_i), f = identity, v/div or
+ *    #{bounds <= v}; rows whose key falls outside [0, nr_grp) are dropped.
+ *  - measure = 1, a, a*b or a-b, in int64. */
+enum mbx_keykind { MBX_KEY_IDENT = 0, MBX_KEY_DIV = 1, MBX_KEY_BUCKET = 2 };
+enum mbx_measure { MBX_MEAS_COUNT = 0, MBX_MEAS_COL = 1, MBX_MEAS_MUL = 2,
+                   MBX_MEAS_SUB = 3 };
+typedef struct mbx_pred {
+  mbx_colref src;
+  uint64_t lo, hi;
+} mbx_pred;
+typedef struct mbx_keypart {
+  mbx_colref src;
+  uint32_t kind;
+  uint32_t div;
+  int64_t base;
+  int64_t stride;
+  uint32_t nbounds;
+  uint32_t pad_;
+  uint64_t bounds[MBX_MAXBOUNDS];
+} mbx_keypart;
+typedef struct mbx_rowop {
+  mbx_pred preds[MBX_MAXCOLS];
+  uint32_t npreds;
+  uint32_t nkeys;
+  mbx_keypart keys[MBX_MAXKEYS];
+  uint32_t measure;
+  uint32_t pad_;
+  mbx_colref ma, mb;
+} mbx_rowop;
+
+/* Execution plans of one query. */
+enum mbx_flags {
+  MBX_PLAN_FUSED = 0,        /* vprg_grpby (primitive.cuh:340-445) */
+  MBX_PLAN_MATERIALISE = 1,  /* _stg1/_stg2/_stg3 (primitive_abla.cuh:262-453) */
+  MBX_PLAN_SCAN = 2,         /* grpby on every row (primitive.cuh:16-48) */
+  MBX_PLAN_MASK = 3,
+  MBX_ACCUMULATE = 16        /* add into d_out instead of zeroing it first */
+};
+
+/* Program + group-by in one call.  d_out[nr_grp] int64 on the device.
+ * The fused plan never materialises a bitmap. */
+int mbx_prog_grpby(mbx_ctx *ctx, const mbx_vprg *prg, const mbx_rowop *op,
+                   uint32_t nr_grp, int64_t *d_out, uint32_t flags);
+
+/* The materialise stages, separately callable (primitive_abla.cuh:262-394).
+ * mbx_prog_eval writes ceil(N/64)*2 words each of possible/uncertain
+ * (d_unc may be NULL for a program without edge bins); it returns nothing
+ * else.  mbx_compact appends possible row ids (MSB = uncertain) to d_rows and
+ * counts them in *d_count (device uint32, zeroed by the call).
+ * mbx_rows_grpby reads the count on the device (no host round trip). */
+int mbx_prog_eval(mbx_ctx *ctx, const mbx_vprg *prg, uint32_t *d_poss,
+                  uint32_t *d_unc);
+int mbx_compact(mbx_ctx *ctx, const uint32_t *d_poss, const uint32_t *d_unc,
+                uint32_t nrows, uint32_t *d_rows, uint32_t *d_count);
+int mbx_rows_grpby(mbx_ctx *ctx, const uint32_t *d_rows,
+                   const uint32_t *d_count, const mbx_vprg *prg,
+                   const mbx_rowop *op, uint32_t nr_grp, int64_t *d_out,
+                   uint32_t flags);
+
+/* Algorithmic bytes of a fused run (compulsory 128-byte lines of every
+ * bitmap, column and dimension the plan must touch, plus 8*nr_grp), counted
+ * on the device by a separate accounting kernel. */
+int mbx_alg_bytes(mbx_ctx *ctx, const mbx_vprg *prg, const mbx_rowop *op,
+                  uint32_t nr_grp, uint64_t *bytes, uint64_t *selected);
+
+/* ------------------------------------------------------------ batching */
+/* One query of a batch. */
+typedef struct mbx_query {
+  mbx_vprg prg;
+  mbx_rowop op;
+  uint32_t nr_grp;
+  uint32_t pad_;
+  int64_t *d_out;          /* device, nr_grp int64 */
+} mbx_query;
+
+/* Launch shape of the persistent batched kernel (chosen by mbx_autotune). */
+typedef struct mbx_tune {
+  uint32_t tile_words;     /* 64-row words per work item (multiple of 256) */
+  uint32_t wg_per_cu;      /* resident workgroups per CU for the grid */
+  uint32_t lds_groups;     /* group-table capacity in LDS (int64 slots) */
+  uint32_t pad_;
+} mbx_tune;
+
+typedef struct mbx_batch mbx_batch;
+/* Builds device descriptors and captures the whole batch (output zeroing,
+ * work-queue reset, one persistent launch) into a hipGraph. */
+int mbx_batch_create(mbx_ctx *ctx, const mbx_query *qs, uint32_t nq,
+                     const mbx_tune *tune, mbx_batch **out);
+int mbx_batch_run(mbx_ctx *ctx, mbx_batch *b);        /* graph launch */
+int mbx_batch_run_eager(mbx_ctx *ctx, mbx_batch *b);  /* same, no graph */
+int mbx_batch_destroy(mbx_ctx *ctx, mbx_batch *b);
+/* Times candidate launch shapes on the batch and returns the fastest. */
+int mbx_autotune(mbx_ctx *ctx, const mbx_query *qs, uint32_t nq,
+                 mbx_tune *best, float *best_ms);
+int mbx_default_tune(mbx_tune *t);
+
+/* --------------------------------------------------------------- timing */
+typedef struct mbx_timer mbx_timer;
+int mbx_timer_create(mbx_ctx *ctx, mbx_timer **t);
+int mbx_timer_start(mbx_ctx *ctx, mbx_timer *t);
+int mbx_timer_stop(mbx_ctx *ctx, mbx_timer *t, float *ms);  /* syncs */
+int mbx_timer_destroy(mbx_timer *t);
+/* Plain device copy of `bytes` (for byte-counter calibration). */
+int mbx_copy_kernel(mbx_ctx *ctx, void *dst, const void *src, size_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,166 @@
+"""ctypes mirror of include/mbx.h (the C ABI of libmbx.so).
+
+The structs below must match mbx.h byte for byte; ``check_layout()`` compares
+their sizes with what the library reports (mbx_abi_sizes).
+"""
+import ctypes as C
+import os
+
+LIB_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
+LIB_PATH = os.path.join(LIB_DIR, "libmbx.so")
+
+MAXBIN_PERCOL, MAXCOLS, MAXNINSTR, MAXKEYS, MAXBOUNDS = 4, 4, 6, 3, 8
+
+# status codes
+OK, ERR_INVALID, ERR_OOM, ERR_HIP, ERR_ALIGN, ERR_CAPACITY, ERR_NODEVICE, \
+    ERR_ADMISSION = range(8)
+# opcodes (vprg::ops, primitive.cuh:169-173)
+END, AND, OR, NOT, ANDM, ORM = range(6)
+KEY_IDENT, KEY_DIV, KEY_BUCKET = range(3)
+MEAS_COUNT, MEAS_COL, MEAS_MUL, MEAS_SUB = range(4)
+PLAN_FUSED, PLAN_MATERIALISE, PLAN_SCAN = 0, 1, 2
+ACCUMULATE = 16
+
+
+class ColRef(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("fk", C.c_void_p),
+                ("dimsz", C.c_uint32), ("elem_bytes", C.c_uint32)]
+
+
+class Col(C.Structure):
+    _fields_ = [("middle", C.c_void_p * MAXBIN_PERCOL),
+                ("leftmost", C.c_void_p), ("rightmost", C.c_void_p)]
+
+
+class Instr(C.Structure):
+    _fields_ = [("opcode", C.c_uint8), ("dstreg", C.c_uint8),
+                ("src1", C.c_uint8), ("src2", C.c_uint8)]
+
+
+class Vprg(C.Structure):
+    _fields_ = [("cols", Col * MAXCOLS), ("factsz", C.c_uint32),
+                ("instrs", Instr * MAXNINSTR)]
+
+
+class Pred(C.Structure):
+    _fields_ = [("src", ColRef), ("lo", C.c_uint64), ("hi", C.c_uint64)]
+
+
+class KeyPart(C.Structure):
+    _fields_ = [("src", ColRef), ("kind", C.c_uint32), ("div", C.c_uint32),
+                ("base", C.c_int64), ("stride", C.c_int64),
+                ("nbounds", C.c_uint32), ("pad_", C.c_uint32),
+                ("bounds", C.c_uint64 * MAXBOUNDS)]
+
+
+class RowOp(C.Structure):
+    _fields_ = [("preds", Pred * MAXCOLS), ("npreds", C.c_uint32),
+                ("nkeys", C.c_uint32), ("keys", KeyPart * MAXKEYS),
+                ("measure", C.c_uint32), ("pad_", C.c_uint32),
+                ("ma", ColRef), ("mb", ColRef)]
+
+
+class Query(C.Structure):
+    _fields_ = [("prg", Vprg), ("op", RowOp), ("nr_grp", C.c_uint32),
+                ("pad_", C.c_uint32), ("d_out", C.c_void_p)]
+
+
+class Tune(C.Structure):
+    _fields_ = [("tile_words", C.c_uint32), ("wg_per_cu", C.c_uint32),
+                ("lds_groups", C.c_uint32), ("pad_", C.c_uint32)]
+
+
+ABI_STRUCTS = [ColRef, Col, Instr, Vprg, Pred, KeyPart, RowOp, Query, Tune]
+
+_vp, _sz, _u32, _u64, _i32 = C.c_void_p, C.c_size_t, C.c_uint32, C.c_uint64, C.c_int
+_P = C.POINTER
+
+# name -> (restype, argtypes)
+PROTOTYPES = {
+    "mbx_init": (_i32, [_i32, _sz, _P(_vp)]),
+    "mbx_destroy": (_i32, [_vp]),
+    "mbx_strerror": (C.c_char_p, [_i32]),
+    "mbx_last_error": (C.c_char_p, []),
+    "mbx_device_count": (_i32, [_P(_i32)]),
+    "mbx_sync": (_i32, [_vp]),
+    "mbx_stream": (_vp, [_vp]),
+    "mbx_set_stream": (_i32, [_vp, _vp]),
+    "mbx_alloc": (_i32, [_vp, _sz, _P(_vp)]),
+    "mbx_free": (_i32, [_vp, _vp]),
+    "mbx_pool_stats": (_i32, [_vp, _P(_sz), _P(_sz), _P(_sz), _P(_sz)]),
+    "mbx_memcpy_h2d": (_i32, [_vp, _vp, _vp, _sz]),
+    "mbx_memcpy_d2h": (_i32, [_vp, _vp, _vp, _sz]),
+    "mbx_memcpy_d2d": (_i32, [_vp, _vp, _vp, _sz]),
+    "mbx_fill_u32": (_i32, [_vp, _vp, _u32, _sz]),
+    "mbx_hostpool_create": (_i32, [_sz, _P(_vp)]),
+    "mbx_hostpool_alloc": (_i32, [_vp, _sz, _P(_vp)]),
+    "mbx_hostpool_free": (_i32, [_vp, _vp]),
+    "mbx_hostpool_stats": (_i32, [_vp, _P(_sz), _P(_sz), _P(_sz), _P(_sz), _P(_sz)]),
+    "mbx_hostpool_destroy": (_i32, [_vp]),
+    "mbx_bitmap_words": (_sz, [_u32]),
+    "mbx_bitmap_stride_words": (_sz, [_u32]),
+    "mbx_create_bin": (_i32, [_vp, _P(ColRef), _u32, _P(_u64), _P(_u64), _sz,
+                              _P(_vp), _P(_sz)]),
+    "mbx_create_bin_into": (_i32, [_vp, _P(ColRef), _u32, _P(_u64), _P(_u64),
+                                   _sz, _vp, _sz]),
+    "mbx_create_select": (_i32, [_vp, _vp, _u32, _u32, _u32, _u32, _P(_vp)]),
+    "mbx_create_join": (_i32, [_vp, _vp, _vp, _u32, _u32, _u32, _u32, _u32,
+                               _P(_vp)]),
+    "mbx_prog_grpby": (_i32, [_vp, _P(Vprg), _P(RowOp), _u32, _vp, _u32]),
+    "mbx_prog_eval": (_i32, [_vp, _P(Vprg), _vp, _vp]),
+    "mbx_compact": (_i32, [_vp, _vp, _vp, _u32, _vp, _vp]),
+    "mbx_rows_grpby": (_i32, [_vp, _vp, _vp, _P(Vprg), _P(RowOp), _u32, _vp,
+                              _u32]),
+    "mbx_alg_bytes": (_i32, [_vp, _P(Vprg), _P(RowOp), _u32, _P(_u64),
+                             _P(_u64)]),
+    "mbx_timer_create": (_i32, [_vp, _P(_vp)]),
+    "mbx_timer_start": (_i32, [_vp, _vp]),
+    "mbx_timer_stop": (_i32, [_vp, _vp, _P(C.c_float)]),
+    "mbx_timer_destroy": (_i32, [_vp]),
+    "mbx_copy_kernel": (_i32, [_vp, _vp, _vp, _sz]),
+    "mbx_abi_sizes": (_i32, [_P(_sz), _i32]),
+    # host-only planning helpers (include/mbx_bmpidx.hpp)
+    "mbx_col_from_bin": (_i32, [_sz, _P(_u64), _P(_u64), _u64, _u64, _P(_u64)]),
+    "mbx_col_from_bin2": (_i32, [_sz, _P(_u64), _P(_u64), _sz, _P(_u64),
+                                 _P(_u64), _u64, _u64, _P(_u64)]),
+    "mbx_recipe_plan": (_i32, [_i32, _i32, _u32, _u32, _P(_u32), _P(_u32),
+                               _P(_i32), _P(_i32)]),
+    # generators and column files
+    "mbx_rand_at": (_u64, [_u64, _u64]),
+    "mbx_zipf_table": (_i32, [_u32, C.c_double, _P(_u64)]),
+    "mbx_gen_uniform_u32_host": (_i32, [_vp, _sz, _u64, _u32, _u32]),
+    "mbx_gen_zipf_u32_host": (_i32, [_vp, _sz, _u64, _u32, C.c_double]),
+    "mbx_gen_uniform_u32": (_i32, [_vp, _vp, _sz, _u64, _u32, _u32]),
+    "mbx_gen_zipf_u32": (_i32, [_vp, _vp, _sz, _u64, _u32, C.c_double]),
+    "mbx_gen_map_u32": (_i32, [_vp, _vp, _vp, _sz, _u32, _u32]),
+    "mbx_colfile_write": (_i32, [C.c_char_p, _vp, _sz, _u32, _i32]),
+    "mbx_colfile_read": (_i32, [C.c_char_p, _P(_vp), _P(_sz), _P(_u32)]),
+    "mbx_colfile_free": (None, [_vp]),
+    "mbx_colfile_load": (_i32, [_vp, C.c_char_p, _P(_vp), _P(_sz), _P(_u32)]),
+}
+
+
+def bind(lib):
+    """Attach prototypes; returns the names that the library does not export."""
+    missing = []
+    for name, (res, args) in PROTOTYPES.items():
+        try:
+            f = getattr(lib, name)
+        except AttributeError:
+            missing.append(name)
+            continue
+        f.restype = res
+        f.argtypes = args
+    return missing
+
+
+def check_layout(lib):
+    n = len(ABI_STRUCTS)
+    out = (C.c_size_t * n)()
+    m = lib.mbx_abi_sizes(out, n)
+    if m != n:
+        raise RuntimeError(f"ABI struct count mismatch: lib {m}, python {n}")
+    bad = [(s.__name__, C.sizeof(s), out[i]) for i, s in enumerate(ABI_STRUCTS)
+           if C.sizeof(s) != out[i]]
+    if bad:
+        raise RuntimeError(f"ABI layout mismatch (name, python, C): {bad}")

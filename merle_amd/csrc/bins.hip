@@ -1,0 +1,359 @@
+// bins.hip -- bitmap creation for gfx950.
+//
+// Replaces bmpcreate (primitive.cuh:88-118) and its host wrappers create_bin /
+// create_select / create_join (bmpidx_create.cu:14-136).
+//
+// One wave owns a block of 1024 consecutive rows (16 groups of 64).  Lane l
+// reads row 64g+l of every group (coalesced 4-byte loads), so a wave64
+// ballot of a per-row predicate is exactly one 64-row bitmap word: the two
+// reference uint32 words of those rows, in little-endian order.  The wave
+// then holds 16 words = 128 bytes per bin and each bin's line is written by
+// one lane with full-line 16-byte stores.
+//
+// Consecutive bins (maxs[j] == mins[j+1], the reference's bin tables,
+// ssb/bmpidx-create.cu:8-30) take the radix path: each row gets a bin index,
+// the wave ballots the index bits once (<= 7 ballots for 64 bins), and lane j
+// rebuilds bin j's word as AND over bits of (bit set ? ballot : ~ballot).
+// The cost per 64 rows is independent of the bin count up to 64 bins,
+// instead of one ballot per bin.  Any other bin set takes the generic path:
+// one ballot per bin, as the reference does.
+#include <cstring>
+
+#include "devfuncs.hpp"
+
+namespace mbx {
+
+enum BinMode : uint32_t {
+  kGeneric = 0,   // arbitrary [mins[j], maxs[j]) per bin
+  kUnit = 1,      // consecutive bins of width 1: idx = v - b0
+  kEqual = 2,     // consecutive bins of equal width
+  kSorted = 3     // consecutive bins of any widths (binary search in LDS)
+};
+
+constexpr uint32_t kMaxSortedBounds = 2048;
+constexpr uint32_t kNone = 0xffffffffu;
+
+struct BinArgs {
+  DevSrc src;
+  uint32_t nrows;
+  uint32_t nblocks;      // 1024-row blocks to write
+  uint32_t nb;           // bins
+  uint32_t mode;
+  uint64_t b0, width;    // kUnit / kEqual
+  double inv_width;      // kEqual
+  uint32_t search_top;   // kSorted: largest power of two <= nb+1
+  uint32_t pad;
+  const uint64_t *mins;  // device, nb (kGeneric) / bounds nb+1 (kSorted)
+  const uint64_t *maxs;  // device, nb (kGeneric)
+  uint32_t *out;
+  size_t stride_words;
+};
+
+// Value of row `row` as create_bin's fetch (bmpidx_create.cu:17-35): a
+// foreign key past the dimension reads as INT64_MAX.
+__device__ __forceinline__ uint64_t bin_value(const DevSrc &s, size_t row) {
+  if (s.fk == nullptr) return ld_elem(s.data, s.eb, row, true);
+  const uint32_t k = ld_stream(s.fk + row);
+  if (k >= s.dimsz) return (uint64_t)INT64_MAX;
+  return ld_elem(s.data, s.eb, k, false);
+}
+
+__device__ __forceinline__ uint32_t bin_index(const BinArgs &a, uint64_t v,
+                                              const uint64_t *s_bounds) {
+  if (a.mode == kUnit) {
+    if (v < a.b0) return kNone;
+    const uint64_t d = v - a.b0;
+    return d < a.nb ? (uint32_t)d : kNone;
+  }
+  if (a.mode == kEqual) {
+    if (v < a.b0) return kNone;
+    const uint64_t d = v - a.b0;
+    if (d >= (uint64_t)a.nb * a.width) return kNone;
+    uint64_t q = (uint64_t)((double)d * a.inv_width);
+    if (q * a.width > d) --q;
+    else if ((q + 1) * a.width <= d) ++q;
+    return (uint32_t)q;
+  }
+  // kSorted: c = #{bounds <= v}; bin = c - 1 when 1 <= c <= nb
+  uint32_t pos = 0;
+  for (uint32_t step = a.search_top; step > 0; step >>= 1) {
+    const uint32_t np = pos + step;
+    if (np <= a.nb + 1 && s_bounds[np - 1] <= v) pos = np;
+  }
+  return (pos >= 1 && pos <= a.nb) ? pos - 1 : kNone;
+}
+
+typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
+
+// One lane writes a full 128-byte line (16 words of 64 rows).
+__device__ __forceinline__ void store_line(uint32_t *dst, const uint64_t *w) {
+  u64x2 *d = reinterpret_cast<u64x2 *>(dst);
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    u64x2 x;
+    x.x = w[2 * i];
+    x.y = w[2 * i + 1];
+    __builtin_nontemporal_store(x, d + i);
+  }
+}
+
+template <bool RADIX>
+__global__ __launch_bounds__(256) void bins_kernel(BinArgs a) {
+  __shared__ uint64_t s_bounds[RADIX ? kMaxSortedBounds : 1];
+  if (RADIX && a.mode == kSorted) {
+    for (uint32_t i = threadIdx.x; i <= a.nb; i += blockDim.x)
+      s_bounds[i] = a.mins[i];
+    __syncthreads();
+  }
+  const uint32_t lane = lane_id();
+  const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const uint32_t nwaves = gridDim.x * (blockDim.x >> 6);
+  const uint32_t nchunks = (a.nb + 63) / 64;
+
+  for (uint32_t blk = wave; blk < a.nblocks; blk += nwaves) {
+    const size_t row0 = (size_t)blk * 1024;
+    uint64_t v[16];
+    uint32_t ok = 0;
+#pragma unroll
+    for (int g = 0; g < 16; ++g) {
+      const size_t row = row0 + 64 * g + lane;
+      v[g] = 0;
+      if (row < a.nrows) {
+        v[g] = bin_value(a.src, row);
+        ok |= 1u << g;
+      }
+    }
+    uint32_t idx[16];
+    if (RADIX) {
+#pragma unroll
+      for (int g = 0; g < 16; ++g)
+        idx[g] = (ok >> g & 1u) ? bin_index(a, v[g], s_bounds) : kNone;
+    }
+    for (uint32_t c = 0; c < nchunks; ++c) {
+      const uint32_t cbase = c * 64;
+      const uint32_t inchunk = min(64u, a.nb - cbase);
+      uint64_t w[16];
+      if (RADIX) {
+        const uint32_t nbits = inchunk <= 1 ? 0 : 32 - __clz(inchunk - 1);
+#pragma unroll
+        for (int g = 0; g < 16; ++g) {
+          const uint32_t rel = idx[g] - cbase;   // kNone wraps out of range
+          const bool inb = rel < inchunk;
+          uint64_t acc = __ballot(inb);
+          for (uint32_t k = 0; k < nbits; ++k) {
+            const uint64_t bk = __ballot(inb && ((rel >> k) & 1u));
+            const uint64_t flip = ((lane >> k) & 1u) ? 0ull : ~0ull;
+            acc &= bk ^ flip;
+          }
+          w[g] = acc;
+        }
+      } else {
+#pragma unroll
+        for (int g = 0; g < 16; ++g) w[g] = 0;
+        for (uint32_t j = 0; j < inchunk; ++j) {
+          const uint64_t lo = a.mins[cbase + j], hi = a.maxs[cbase + j];
+#pragma unroll
+          for (int g = 0; g < 16; ++g) {
+            const bool in = ((ok >> g) & 1u) && v[g] >= lo && v[g] < hi;
+            const uint64_t m = __ballot(in);
+            if (lane == j) w[g] = m;
+          }
+        }
+      }
+      if (lane < inchunk)
+        store_line(a.out + (size_t)(cbase + lane) * a.stride_words +
+                       (size_t)blk * 32,
+                   w);
+    }
+  }
+}
+
+// Classifies the bin set and launches the kernel.  d_tmp (nb+1 or 2nb
+// uint64, device) is used for bounds the kernel must read from memory.
+static int launch_bins(mbx_ctx *ctx, const mbx_colref *src, uint32_t nrows,
+                       const uint64_t *mins, const uint64_t *maxs, size_t nb,
+                       uint32_t *slab, size_t stride_words) {
+  int st = check_src(*src, "create_bin");
+  if (st) return st;
+  if (src->fk && reinterpret_cast<uintptr_t>(src->fk) % 4)
+    return fail(MBX_ERR_ALIGN, "create_bin: misaligned fk");
+  if (nb == 0 || nb > 0x7fffffffu)
+    return fail(MBX_ERR_INVALID, "create_bin: bad bin count");
+  if (stride_words % 64 != 0 || stride_words < mbx_bitmap_words(nrows))
+    return fail(MBX_ERR_INVALID,
+                "create_bin: stride_words must be a multiple of 64 and "
+                ">= ceil(N/32)");
+  if (reinterpret_cast<uintptr_t>(slab) % 16)
+    return fail(MBX_ERR_ALIGN, "create_bin: output not 16-byte aligned");
+
+  BinArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.src.data = src->data;
+  a.src.fk = src->fk;
+  a.src.dimsz = src->dimsz;
+  a.src.eb = src->elem_bytes;
+  a.nrows = nrows;
+  a.nblocks = (uint32_t)(stride_words / 32);
+  a.nb = (uint32_t)nb;
+  a.out = slab;
+  a.stride_words = stride_words;
+
+  bool consecutive = true, equal = true;
+  for (size_t j = 0; j < nb; ++j) {
+    if (!(mins[j] < maxs[j])) consecutive = false;
+    if (j + 1 < nb && maxs[j] != mins[j + 1]) consecutive = false;
+    if (maxs[j] - mins[j] != maxs[0] - mins[0]) equal = false;
+  }
+  const uint64_t w0 = maxs[0] - mins[0];
+  size_t tmp_words = 0;
+  if (consecutive && equal && w0 == 1) {
+    a.mode = kUnit;
+    a.b0 = mins[0];
+  } else if (consecutive && equal && w0 != 0 && w0 < (1ull << 40) &&
+             (double)nb * (double)w0 < 4.0e15) {
+    a.mode = kEqual;
+    a.b0 = mins[0];
+    a.width = w0;
+    a.inv_width = 1.0 / (double)w0;
+  } else if (consecutive && nb + 1 <= kMaxSortedBounds) {
+    a.mode = kSorted;
+    tmp_words = nb + 1;
+    uint32_t top = 1;
+    while (top * 2 <= nb + 1) top *= 2;
+    a.search_top = top;
+  } else {
+    a.mode = kGeneric;
+    tmp_words = 2 * nb;
+  }
+
+  void *tmp = nullptr;
+  if (tmp_words) {
+    st = ctx->pool->alloc(tmp_words * 8, &tmp);
+    if (st) return st;
+    std::vector<uint64_t> h(tmp_words);
+    if (a.mode == kSorted) {
+      for (size_t j = 0; j < nb; ++j) h[j] = mins[j];
+      h[nb] = maxs[nb - 1];
+      a.mins = static_cast<uint64_t *>(tmp);
+    } else {
+      for (size_t j = 0; j < nb; ++j) h[j] = mins[j], h[nb + j] = maxs[j];
+      a.mins = static_cast<uint64_t *>(tmp);
+      a.maxs = static_cast<uint64_t *>(tmp) + nb;
+    }
+    hipError_t e = hipMemcpyAsync(tmp, h.data(), tmp_words * 8,
+                                  hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) {
+      ctx->pool->free(tmp);
+      return hip_fail(e, "create_bin: bounds upload");
+    }
+  }
+  size_t wgs = ceil_div(a.nblocks, 4);
+  const size_t cap = (size_t)ctx->num_cus * 16;
+  if (wgs > cap) wgs = cap;
+  if (wgs == 0) wgs = 1;
+  if (a.mode == kGeneric)
+    hipLaunchKernelGGL(bins_kernel<false>, dim3((unsigned)wgs), dim3(256), 0,
+                       ctx->stream, a);
+  else
+    hipLaunchKernelGGL(bins_kernel<true>, dim3((unsigned)wgs), dim3(256), 0,
+                       ctx->stream, a);
+  hipError_t e = hipGetLastError();
+  if (tmp) {
+    // the bounds must outlive the kernel: wait before returning them
+    hipError_t e2 = hipStreamSynchronize(ctx->stream);
+    ctx->pool->free(tmp);
+    if (e == hipSuccess) e = e2;
+  }
+  if (e != hipSuccess) return hip_fail(e, "create_bin: launch");
+  return MBX_OK;
+}
+
+}  // namespace mbx
+
+using namespace mbx;
+
+extern "C" {
+
+size_t mbx_bitmap_words(uint32_t nrows) { return ceil_div(nrows, 32); }
+
+size_t mbx_bitmap_stride_words(uint32_t nrows) {
+  return ceil_div(ceil_div(nrows, 32), 64) * 64;
+}
+
+int mbx_create_bin_into(mbx_ctx *ctx, const mbx_colref *src, uint32_t nrows,
+                        const uint64_t *mins, const uint64_t *maxs,
+                        size_t nbins, uint32_t *slab, size_t stride_words) {
+  if (!ctx || !src || !mins || !maxs || !slab)
+    return fail(MBX_ERR_INVALID, "create_bin_into: null argument");
+  if (nrows == 0) return fail(MBX_ERR_INVALID, "create_bin_into: 0 rows");
+  return launch_bins(ctx, src, nrows, mins, maxs, nbins, slab, stride_words);
+}
+
+int mbx_create_bin(mbx_ctx *ctx, const mbx_colref *src, uint32_t nrows,
+                   const uint64_t *mins, const uint64_t *maxs, size_t nbins,
+                   uint32_t **out, size_t *single_bytes) {
+  if (!ctx || !src || !mins || !maxs || !out)
+    return fail(MBX_ERR_INVALID, "create_bin: null argument");
+  if (nrows == 0 || nbins == 0)
+    return fail(MBX_ERR_INVALID, "create_bin: 0 rows or 0 bins");
+  const size_t stride = mbx_bitmap_stride_words(nrows);
+  void *slab = nullptr;
+  int st = ctx->pool->alloc(stride * 4 * nbins, &slab);
+  if (st) return st;
+  st = launch_bins(ctx, src, nrows, mins, maxs, nbins,
+                   static_cast<uint32_t *>(slab), stride);
+  if (st) {
+    ctx->pool->free(slab);
+    return st;
+  }
+  for (size_t j = 0; j < nbins; ++j)
+    out[j] = static_cast<uint32_t *>(slab) + j * stride;
+  if (single_bytes) *single_bytes = mbx_bitmap_words(nrows) * 4;
+  return MBX_OK;
+}
+
+int mbx_create_join(mbx_ctx *ctx, const uint32_t *fk, const void *dimattr,
+                    uint32_t elem_bytes, uint32_t nrows, uint32_t dimsz,
+                    uint32_t lo, uint32_t hi, uint32_t **out) {
+  if (!out) return fail(MBX_ERR_INVALID, "create_join: null out");
+  *out = nullptr;
+  mbx_colref src;
+  src.data = dimattr;
+  src.fk = fk;
+  src.dimsz = dimsz;
+  src.elem_bytes = elem_bytes;
+  uint64_t mn = lo, mx = hi;
+  if (mn >= mx) {
+    // an empty range still yields an all-zero bitmap, as the reference's
+    // predicate `val >= min && val < max` does
+    mx = mn;
+  }
+  uint32_t *res[1] = {nullptr};
+  if (mn == mx) {
+    const size_t stride = mbx_bitmap_stride_words(nrows);
+    void *slab = nullptr;
+    int st = ctx ? ctx->pool->alloc(stride * 4, &slab)
+                 : fail(MBX_ERR_INVALID, "null ctx");
+    if (st) return st;
+    st = launch_fill_u32(ctx->stream, slab, 0, stride);
+    if (st) {
+      ctx->pool->free(slab);
+      return st;
+    }
+    *out = static_cast<uint32_t *>(slab);
+    return MBX_OK;
+  }
+  int st = mbx_create_bin(ctx, &src, nrows, &mn, &mx, 1, res, nullptr);
+  if (st) return st;
+  *out = res[0];
+  return MBX_OK;
+}
+
+int mbx_create_select(mbx_ctx *ctx, const void *srcp, uint32_t elem_bytes,
+                      uint32_t nrows, uint32_t lo, uint32_t hi,
+                      uint32_t **out) {
+  return mbx_create_join(ctx, nullptr, srcp, elem_bytes, nrows, 0, lo, hi,
+                         out);
+}
+
+}  // extern "C"

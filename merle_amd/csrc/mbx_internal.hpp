@@ -1,0 +1,142 @@
+// mbx_internal.hpp -- shared host-side declarations of libmbx (not installed).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/mbx.h"
+
+namespace mbx {
+
+// ------------------------------------------------------------------ errors
+// Per-thread detail message behind mbx_last_error().
+void set_error(const std::string &msg);
+int fail(int status, const std::string &msg);
+int hip_fail(hipError_t e, const char *what);
+
+#define MBX_HIP(call)                                                          \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess) return ::mbx::hip_fail(e_, #call);                   \
+  } while (0)
+
+// ------------------------------------------------------------------- pool
+// First-fit allocator with coalescing over one or more arenas, 256-byte
+// granules.  The bookkeeping is host-only; the arena memory is hipMalloc'd
+// (device pool) or aligned_alloc'd (host pool used by CPU tests).  Replaces
+// merle/dumb_pool.cuh:39-95, which is a bump allocator with 48 live slots.
+class Pool {
+ public:
+  static constexpr size_t kAlign = 256;
+  static constexpr size_t kArena = size_t(256) << 20;
+  // fixed_capacity == 0: grow in kArena arenas; else one arena of that size.
+  Pool(bool device, size_t fixed_capacity);
+  ~Pool();
+  Pool(const Pool &) = delete;
+  Pool &operator=(const Pool &) = delete;
+
+  int alloc(size_t bytes, void **out);
+  int free(void *p);
+  void stats(size_t *used, size_t *capacity, size_t *peak, size_t *nlive,
+             size_t *largest_free) const;
+  int init_status() const { return init_status_; }
+
+ private:
+  struct Arena {
+    char *base;
+    size_t size;
+  };
+  int add_arena(size_t size);
+  void *raw_alloc(size_t size);
+  void raw_free(void *p);
+
+  bool device_;
+  bool fixed_;
+  int init_status_ = MBX_OK;
+  std::vector<Arena> arenas_;
+  std::map<char *, size_t> free_;   // start -> size (coalesced)
+  std::map<char *, size_t> live_;   // start -> size
+  size_t used_ = 0, peak_ = 0, capacity_ = 0;
+  mutable std::mutex mu_;
+};
+
+}  // namespace mbx
+
+struct mbx_ctx {
+  int device = 0;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  int num_cus = 256;
+  size_t lds_per_cu = 160 * 1024;
+  mbx::Pool *pool = nullptr;
+  // scratch reused by the plans (one query at a time per context)
+  uint32_t *d_counter = nullptr;   // 4 uint32 work/counter slots
+};
+
+struct mbx_timer {
+  hipEvent_t a = nullptr, b = nullptr;
+};
+
+namespace mbx {
+
+// ------------------------------------------------- device-side descriptors
+// Kernel-argument forms of mbx_vprg / mbx_rowop: pointers are typed as
+// 64-row words, middle-bin counts and instruction counts are precomputed, so
+// the kernels branch only on wave-uniform values.
+struct DevCol {
+  const uint64_t *m[MBX_MAXBIN_PERCOL];
+  const uint64_t *l, *r;
+  uint32_t nm;      // leading non-null middle bins
+  uint32_t has_edge;
+};
+struct DevPrg {
+  DevCol cols[MBX_MAXCOLS];
+  uint8_t ins[MBX_MAXNINSTR][4];
+  uint32_t nins;
+  uint32_t nrows;
+  uint32_t nwords;  // ceil(nrows / 64)
+  uint32_t chk;     // any edge bin -> three-valued evaluation
+};
+struct DevSrc {
+  const void *data;
+  const uint32_t *fk;
+  uint32_t dimsz;
+  uint32_t eb;
+};
+struct DevKey {
+  DevSrc src;
+  uint32_t kind, div;
+  int64_t base, stride;
+  uint32_t nb, pad;
+  uint64_t bounds[MBX_MAXBOUNDS];
+};
+struct DevOp {
+  DevSrc pred[MBX_MAXCOLS];
+  uint64_t lo[MBX_MAXCOLS], hi[MBX_MAXCOLS];
+  uint32_t npreds, nkeys, measure, pad;
+  DevKey keys[MBX_MAXKEYS];
+  DevSrc ma, mb;
+};
+
+// Validation + conversion; returns MBX_OK or an error (with message).
+int make_dev_prg(const mbx_vprg *p, DevPrg *out);
+int make_dev_op(const mbx_rowop *op, uint32_t npreds_hint, DevOp *out);
+int check_src(const mbx_colref &c, const char *what);
+
+inline size_t ceil_div(size_t a, size_t b) { return (a + b - 1) / b; }
+
+// ------------------------------------------------------- launch helpers
+int launch_fill_u32(hipStream_t s, void *dst, uint32_t v, size_t count);
+int launch_fill_u64(hipStream_t s, void *dst, uint64_t v, size_t count);
+
+// Group-table capacity kept in LDS by the query kernels (int64 slots).
+constexpr uint32_t kLdsGroups = 4096;
+// Row-list capacity of one compaction round (uint32 entries).
+constexpr uint32_t kListCap = 2048;
+
+}  // namespace mbx

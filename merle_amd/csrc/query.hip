@@ -1,0 +1,513 @@
+// query.hip -- single-query plans on gfx950:
+//   fused        vprg_grpby            (primitive.cuh:340-445)
+//   materialise  _stg1 / _stg2 / _stg3 (primitive_abla.cuh:262-453)
+//   scan         grpby                 (primitive.cuh:16-48)
+// plus the algorithmic-byte accounting kernel behind mbx_alg_bytes.
+#include <cstring>
+#include <vector>
+
+#include "devquery.hpp"
+#include "query_launch.hpp"
+
+namespace mbx {
+
+struct QArgs {
+  DevPrg prg;
+  DevOp op;
+  unsigned long long *out;
+  uint32_t nr_grp;
+  uint32_t ntiles;
+  const Acct *acct;
+};
+
+// ------------------------------------------------------------ fused plan
+// Persistent: each workgroup walks tiles blockIdx.x, +gridDim.x, ... and
+// keeps its group table in LDS until the end, so the HBM output sees at most
+// one atomic per (workgroup, non-empty group) -- never more than the rows the
+// workgroup aggregated.
+template <bool CHK, bool LDSG, bool ACCT, int WPL>
+__global__ __launch_bounds__(256) void fused_kernel(QArgs a) {
+  __shared__ uint32_t s_list[kListCap];
+  __shared__ uint32_t s_wave[4];
+  extern __shared__ unsigned long long s_grp[];
+  GroupSink sink;
+  sink.lds = LDSG ? s_grp : nullptr;
+  sink.out = a.out;
+  if (LDSG)
+    for (uint32_t i = threadIdx.x; i < a.nr_grp; i += 256) s_grp[i] = 0;
+  __syncthreads();
+  for (uint32_t t = blockIdx.x; t < a.ntiles; t += gridDim.x)
+    run_tile<CHK, ACCT, WPL>(a.prg, a.op, a.nr_grp, t, sink, s_list, s_wave,
+                             a.acct);
+  if (LDSG) {
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < a.nr_grp; i += 256) {
+      const unsigned long long v = s_grp[i];
+      if (v) atomicAdd(a.out + i, v);
+    }
+  }
+}
+
+// ------------------------------------------------------------- scan plan
+// Every row, all predicates checked (op(row, chk=true)), no bitmaps.
+template <bool LDSG>
+__global__ __launch_bounds__(256) void scan_kernel(QArgs a) {
+  extern __shared__ unsigned long long s_grp[];
+  GroupSink sink;
+  sink.lds = LDSG ? s_grp : nullptr;
+  sink.out = a.out;
+  if (LDSG)
+    for (uint32_t i = threadIdx.x; i < a.nr_grp; i += 256) s_grp[i] = 0;
+  __syncthreads();
+  const uint32_t n = a.prg.nrows;
+  for (uint32_t row = blockIdx.x * 256 + threadIdx.x; row < n;
+       row += gridDim.x * 256) {
+    if (!row_passes<false>(a.prg, a.op, row, nullptr)) continue;
+    uint32_t key;
+    int64_t val;
+    if (row_keyval<false>(a.op, row, a.nr_grp, key, val, nullptr))
+      sink_add(sink, key, val);
+  }
+  if (LDSG) {
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < a.nr_grp; i += 256) {
+      const unsigned long long v = s_grp[i];
+      if (v) atomicAdd(a.out + i, v);
+    }
+  }
+}
+
+// ------------------------------------------------------- materialise plan
+// Stage 1: program -> possible/uncertain bitmaps in HBM.
+template <bool CHK>
+__global__ __launch_bounds__(256) void stage1_kernel(DevPrg P,
+                                                     uint64_t *__restrict__ po,
+                                                     uint64_t *__restrict__ uo) {
+  for (uint32_t w = blockIdx.x * 256 + threadIdx.x; w < P.nwords;
+       w += gridDim.x * 256) {
+    uint64_t p, u;
+    eval_word<CHK, false>(P, w, p, u, nullptr);
+    po[w] = p;
+    if (CHK) uo[w] = u;
+  }
+}
+
+// Stage 2: popcount -> block scan -> one atomicAdd per workgroup for its
+// segment of the global row list (primitive_abla.cuh:290-354).
+template <bool CHK, int WPL>
+__global__ __launch_bounds__(256) void stage2_kernel(
+    const uint64_t *__restrict__ po, const uint64_t *__restrict__ uo,
+    uint32_t nwords, uint32_t *__restrict__ rows, uint32_t *count) {
+  __shared__ uint32_t s_wave[4];
+  __shared__ uint32_t s_base;
+  const uint32_t tid = threadIdx.x;
+  uint64_t p[WPL], u[WPL];
+  uint32_t cnt = 0;
+#pragma unroll
+  for (int k = 0; k < WPL; ++k) {
+    const uint32_t w = (blockIdx.x * WPL + k) * 256 + tid;
+    p[k] = w < nwords ? po[w] : 0;
+    u[k] = (CHK && w < nwords) ? uo[w] : 0;
+    cnt += __popcll(p[k]);
+  }
+  uint32_t total;
+  uint32_t off = block_excl_scan(cnt, s_wave, &total);
+  if (tid == 0) s_base = total ? atomicAdd(count, total) : 0;
+  __syncthreads();
+  off += s_base;
+#pragma unroll
+  for (int k = 0; k < WPL; ++k) {
+    uint64_t m = p[k];
+    const uint32_t row0 = ((blockIdx.x * WPL + k) * 256 + tid) * 64;
+    while (m) {
+      const uint32_t b = (uint32_t)__builtin_ctzll(m);
+      m &= m - 1;
+      uint32_t e = row0 + b;
+      if (CHK && ((u[k] >> b) & 1ull)) e |= 0x80000000u;
+      rows[off++] = e;
+    }
+  }
+}
+
+// Stage 3: op + group-by over the list.  The list length is read on the
+// device, so no host round trip sits between stages 2 and 3 (the reference
+// copies it to the host, primitive_abla.cuh:437-444).
+template <bool CHK, bool LDSG>
+__global__ __launch_bounds__(256) void stage3_kernel(
+    QArgs a, const uint32_t *__restrict__ rows, const uint32_t *count) {
+  extern __shared__ unsigned long long s_grp[];
+  GroupSink sink;
+  sink.lds = LDSG ? s_grp : nullptr;
+  sink.out = a.out;
+  if (LDSG)
+    for (uint32_t i = threadIdx.x; i < a.nr_grp; i += 256) s_grp[i] = 0;
+  __syncthreads();
+  const uint32_t n = *count;
+  for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j < n;
+       j += gridDim.x * 256)
+    process_entry<CHK, false>(a.prg, a.op, a.nr_grp, rows[j], sink, nullptr);
+  if (LDSG) {
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < a.nr_grp; i += 256) {
+      const unsigned long long v = s_grp[i];
+      if (v) atomicAdd(a.out + i, v);
+    }
+  }
+}
+
+// Popcount of accounting bitsets: lines[e] = set bits of entry e.
+__global__ void acct_count_kernel(const uint32_t *__restrict__ bits,
+                                  const uint32_t *__restrict__ offs,
+                                  uint32_t nent,
+                                  unsigned long long *__restrict__ lines) {
+  for (uint32_t e = blockIdx.x; e < nent; e += gridDim.x) {
+    unsigned long long c = 0;
+    for (uint32_t i = offs[e] + threadIdx.x; i < offs[e + 1]; i += blockDim.x)
+      c += __popc(bits[i]);
+    atomicAdd(lines + e, c);
+  }
+}
+
+// ---------------------------------------------------------------- launches
+static constexpr int kWPL = 2;   // 64-row words per thread per tile
+
+int prepare(mbx_ctx *ctx, const mbx_vprg *prg, const mbx_rowop *op,
+            uint32_t nr_grp, int64_t *d_out, QueryPlan *q) {
+  if (!ctx) return fail(MBX_ERR_INVALID, "null ctx");
+  if (!d_out) return fail(MBX_ERR_INVALID, "null d_out");
+  if (nr_grp == 0) return fail(MBX_ERR_INVALID, "nr_grp == 0");
+  if (reinterpret_cast<uintptr_t>(d_out) % 8)
+    return fail(MBX_ERR_ALIGN, "d_out not 8-byte aligned");
+  int st = make_dev_prg(prg, &q->prg);
+  if (st) return st;
+  st = make_dev_op(op, 0, &q->op);
+  if (st) return st;
+  if (q->prg.chk && q->op.npreds == 0)
+    return fail(MBX_ERR_INVALID,
+                "program has edge bins but the row op has no predicates to "
+                "re-check uncertain rows");
+  for (uint32_t i = 0; i < q->prg.nins; ++i) {
+    const uint32_t o = q->prg.ins[i][0], s = q->prg.ins[i][2];
+    if (q->prg.chk && (o == MBX_ANDM || o == MBX_ORM) && s >= q->op.npreds)
+      return fail(MBX_ERR_INVALID,
+                  "instruction reads a column without an exact predicate");
+  }
+  q->nr_grp = nr_grp;
+  q->out = reinterpret_cast<unsigned long long *>(d_out);
+  q->lds = nr_grp <= kLdsGroups;
+  return MBX_OK;
+}
+
+static unsigned persistent_grid(mbx_ctx *ctx, uint32_t work, unsigned per_cu) {
+  uint64_t g = (uint64_t)ctx->num_cus * per_cu;
+  if (g > work) g = work;
+  return g ? (unsigned)g : 1u;
+}
+
+static size_t grp_lds(const QueryPlan &q) {
+  return q.lds ? (size_t)q.nr_grp * 8 : 0;
+}
+
+int launch_fused(mbx_ctx *ctx, const QueryPlan &q, const Acct *acct) {
+  QArgs a;
+  a.prg = q.prg;
+  a.op = q.op;
+  a.out = q.out;
+  a.nr_grp = q.nr_grp;
+  a.ntiles = (uint32_t)ceil_div(q.prg.nwords, 256 * kWPL);
+  a.acct = acct;
+  const unsigned grid = persistent_grid(ctx, a.ntiles, 4);
+  const size_t lds = grp_lds(q);
+  const bool chk = q.prg.chk != 0;
+#define MBX_FUSED(C, L, A)                                                     \
+  hipLaunchKernelGGL((fused_kernel<C, L, A, kWPL>), dim3(grid), dim3(256),     \
+                     lds, ctx->stream, a)
+  if (acct) {
+    if (chk) MBX_FUSED(true, false, true);
+    else MBX_FUSED(false, false, true);
+  } else if (chk) {
+    if (q.lds) MBX_FUSED(true, true, false);
+    else MBX_FUSED(true, false, false);
+  } else {
+    if (q.lds) MBX_FUSED(false, true, false);
+    else MBX_FUSED(false, false, false);
+  }
+#undef MBX_FUSED
+  MBX_HIP(hipGetLastError());
+  return MBX_OK;
+}
+
+int launch_scan(mbx_ctx *ctx, const QueryPlan &q) {
+  QArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.prg = q.prg;
+  a.op = q.op;
+  a.out = q.out;
+  a.nr_grp = q.nr_grp;
+  const unsigned grid =
+      persistent_grid(ctx, (uint32_t)ceil_div(q.prg.nrows, 256), 8);
+  if (q.lds)
+    hipLaunchKernelGGL(scan_kernel<true>, dim3(grid), dim3(256), grp_lds(q),
+                       ctx->stream, a);
+  else
+    hipLaunchKernelGGL(scan_kernel<false>, dim3(grid), dim3(256), 0,
+                       ctx->stream, a);
+  MBX_HIP(hipGetLastError());
+  return MBX_OK;
+}
+
+int launch_stage1(mbx_ctx *ctx, const DevPrg &P, uint64_t *po, uint64_t *uo) {
+  const unsigned grid =
+      persistent_grid(ctx, (uint32_t)ceil_div(P.nwords, 256), 16);
+  if (P.chk)
+    hipLaunchKernelGGL(stage1_kernel<true>, dim3(grid), dim3(256), 0,
+                       ctx->stream, P, po, uo);
+  else
+    hipLaunchKernelGGL(stage1_kernel<false>, dim3(grid), dim3(256), 0,
+                       ctx->stream, P, po, uo);
+  MBX_HIP(hipGetLastError());
+  return MBX_OK;
+}
+
+int launch_stage2(mbx_ctx *ctx, const uint64_t *po, const uint64_t *uo,
+                  uint32_t nwords, uint32_t *rows, uint32_t *count) {
+  int st = launch_fill_u32(ctx->stream, count, 0, 1);
+  if (st) return st;
+  const unsigned grid = (unsigned)ceil_div(nwords, 256 * kWPL);
+  if (uo)
+    hipLaunchKernelGGL((stage2_kernel<true, kWPL>), dim3(grid ? grid : 1),
+                       dim3(256), 0, ctx->stream, po, uo, nwords, rows, count);
+  else
+    hipLaunchKernelGGL((stage2_kernel<false, kWPL>), dim3(grid ? grid : 1),
+                       dim3(256), 0, ctx->stream, po, uo, nwords, rows, count);
+  MBX_HIP(hipGetLastError());
+  return MBX_OK;
+}
+
+int launch_stage3(mbx_ctx *ctx, const QueryPlan &q, const uint32_t *rows,
+                  const uint32_t *count) {
+  QArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.prg = q.prg;
+  a.op = q.op;
+  a.out = q.out;
+  a.nr_grp = q.nr_grp;
+  const unsigned grid = (unsigned)ctx->num_cus * 4;
+  const size_t lds = grp_lds(q);
+#define MBX_S3(C, L)                                                           \
+  hipLaunchKernelGGL((stage3_kernel<C, L>), dim3(grid), dim3(256), lds,        \
+                     ctx->stream, a, rows, count)
+  if (q.prg.chk) {
+    if (q.lds) MBX_S3(true, true);
+    else MBX_S3(true, false);
+  } else {
+    if (q.lds) MBX_S3(false, true);
+    else MBX_S3(false, false);
+  }
+#undef MBX_S3
+  MBX_HIP(hipGetLastError());
+  return MBX_OK;
+}
+
+// Materialise plan with scratch from the pool (freed after a sync).
+static int run_materialise(mbx_ctx *ctx, const QueryPlan &q) {
+  const size_t nw = q.prg.nwords;
+  void *buf = nullptr;
+  const size_t bytes = nw * 8 * 2 + (size_t)q.prg.nrows * 4 + 256;
+  int st = ctx->pool->alloc(bytes, &buf);
+  if (st) return st;
+  uint64_t *po = static_cast<uint64_t *>(buf);
+  uint64_t *uo = po + nw;
+  uint32_t *count = reinterpret_cast<uint32_t *>(uo + nw);
+  uint32_t *rows = count + 64;
+  st = launch_stage1(ctx, q.prg, po, q.prg.chk ? uo : nullptr);
+  if (!st)
+    st = launch_stage2(ctx, po, q.prg.chk ? uo : nullptr, (uint32_t)nw, rows,
+                       count);
+  if (!st) st = launch_stage3(ctx, q, rows, count);
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  ctx->pool->free(buf);
+  if (!st && e != hipSuccess) st = hip_fail(e, "materialise plan");
+  return st;
+}
+
+}  // namespace mbx
+
+using namespace mbx;
+
+extern "C" {
+
+int mbx_prog_grpby(mbx_ctx *ctx, const mbx_vprg *prg, const mbx_rowop *op,
+                   uint32_t nr_grp, int64_t *d_out, uint32_t flags) {
+  QueryPlan q;
+  int st = prepare(ctx, prg, op, nr_grp, d_out, &q);
+  if (st) return st;
+  if (!(flags & MBX_ACCUMULATE)) {
+    st = launch_fill_u64(ctx->stream, d_out, 0, nr_grp);
+    if (st) return st;
+  }
+  switch (flags & MBX_PLAN_MASK) {
+  case MBX_PLAN_FUSED: return launch_fused(ctx, q, nullptr);
+  case MBX_PLAN_MATERIALISE: return run_materialise(ctx, q);
+  case MBX_PLAN_SCAN: return launch_scan(ctx, q);
+  default: return fail(MBX_ERR_INVALID, "unknown plan");
+  }
+}
+
+int mbx_prog_eval(mbx_ctx *ctx, const mbx_vprg *prg, uint32_t *d_poss,
+                  uint32_t *d_unc) {
+  if (!ctx || !d_poss) return fail(MBX_ERR_INVALID, "prog_eval: null");
+  DevPrg P;
+  int st = make_dev_prg(prg, &P);
+  if (st) return st;
+  if (reinterpret_cast<uintptr_t>(d_poss) % 8 ||
+      reinterpret_cast<uintptr_t>(d_unc) % 8)
+    return fail(MBX_ERR_ALIGN, "prog_eval: outputs not 8-byte aligned");
+  if (P.chk && !d_unc)
+    return fail(MBX_ERR_INVALID, "prog_eval: program needs d_unc");
+  return launch_stage1(ctx, P, reinterpret_cast<uint64_t *>(d_poss),
+                       reinterpret_cast<uint64_t *>(d_unc));
+}
+
+int mbx_compact(mbx_ctx *ctx, const uint32_t *d_poss, const uint32_t *d_unc,
+                uint32_t nrows, uint32_t *d_rows, uint32_t *d_count) {
+  if (!ctx || !d_poss || !d_rows || !d_count)
+    return fail(MBX_ERR_INVALID, "compact: null");
+  if (nrows == 0 || nrows > 0x7fffffffu)
+    return fail(MBX_ERR_INVALID, "compact: nrows must be in [1, 2^31)");
+  if (reinterpret_cast<uintptr_t>(d_poss) % 8 ||
+      reinterpret_cast<uintptr_t>(d_unc) % 8)
+    return fail(MBX_ERR_ALIGN, "compact: bitmaps not 8-byte aligned");
+  return launch_stage2(ctx, reinterpret_cast<const uint64_t *>(d_poss),
+                       reinterpret_cast<const uint64_t *>(d_unc),
+                       (uint32_t)ceil_div(nrows, 64), d_rows, d_count);
+}
+
+int mbx_rows_grpby(mbx_ctx *ctx, const uint32_t *d_rows,
+                   const uint32_t *d_count, const mbx_vprg *prg,
+                   const mbx_rowop *op, uint32_t nr_grp, int64_t *d_out,
+                   uint32_t flags) {
+  if (!d_rows || !d_count) return fail(MBX_ERR_INVALID, "rows_grpby: null");
+  QueryPlan q;
+  int st = prepare(ctx, prg, op, nr_grp, d_out, &q);
+  if (st) return st;
+  if (!(flags & MBX_ACCUMULATE)) {
+    st = launch_fill_u64(ctx->stream, d_out, 0, nr_grp);
+    if (st) return st;
+  }
+  return launch_stage3(ctx, q, d_rows, d_count);
+}
+
+int mbx_alg_bytes(mbx_ctx *ctx, const mbx_vprg *prg, const mbx_rowop *op,
+                  uint32_t nr_grp, uint64_t *bytes, uint64_t *selected) {
+  QueryPlan q;
+  void *dummy_out = nullptr;
+  if (!ctx) return fail(MBX_ERR_INVALID, "null ctx");
+  int st = ctx->pool->alloc(8, &dummy_out);
+  if (st) return st;
+  st = prepare(ctx, prg, op, nr_grp, static_cast<int64_t *>(dummy_out), &q);
+  if (st) {
+    ctx->pool->free(dummy_out);
+    return st;
+  }
+  // distinct arrays the plan can touch
+  Acct h;
+  std::memset(&h, 0, sizeof(h));
+  std::vector<uint32_t> offs(1, 0);
+  auto add = [&](const void *p, uint64_t nbytes) {
+    if (!p || !nbytes) return;
+    const uint64_t b = reinterpret_cast<uint64_t>(p);
+    for (uint32_t i = 0; i < h.n; ++i)
+      if (h.e[i].base == b) return;
+    if (h.n >= (uint32_t)kAcctMax) return;
+    AcctEntry &e = h.e[h.n++];
+    e.base = b;
+    e.nbytes = nbytes;
+    e.off = offs.back();
+    offs.push_back(offs.back() + (uint32_t)ceil_div(ceil_div(nbytes, 128), 32));
+  };
+  const uint64_t N = q.prg.nrows, bmp_bytes = (uint64_t)q.prg.nwords * 8;
+  for (uint32_t i = 0; i < q.prg.nins; ++i) {
+    const uint32_t o = q.prg.ins[i][0];
+    if (o != MBX_ANDM && o != MBX_ORM) continue;
+    const DevCol &c = q.prg.cols[q.prg.ins[i][2]];
+    for (uint32_t k = 0; k < c.nm; ++k) add(c.m[k], bmp_bytes);
+    add(c.l, bmp_bytes);
+    add(c.r, bmp_bytes);
+  }
+  auto add_src = [&](const DevSrc &s) {
+    if (!s.data) return;
+    if (s.fk) {
+      add(s.fk, N * 4);
+      add(s.data, (uint64_t)s.dimsz * s.eb);
+    } else {
+      add(s.data, N * s.eb);
+    }
+  };
+  if (q.prg.chk)
+    for (uint32_t i = 0; i < q.op.npreds; ++i) add_src(q.op.pred[i]);
+  for (uint32_t i = 0; i < q.op.nkeys; ++i) add_src(q.op.keys[i].src);
+  if (q.op.measure != MBX_MEAS_COUNT) add_src(q.op.ma);
+  if (q.op.measure == MBX_MEAS_MUL || q.op.measure == MBX_MEAS_SUB)
+    add_src(q.op.mb);
+
+  const size_t nbits_words = offs.back();
+  const size_t need = sizeof(Acct) + 256 + nbits_words * 4 + 8 +
+                      (offs.size()) * 4 + h.n * 8 + 256;
+  void *buf = nullptr;
+  st = ctx->pool->alloc(need, &buf);
+  if (st) {
+    ctx->pool->free(dummy_out);
+    return st;
+  }
+  char *b = static_cast<char *>(buf);
+  Acct *d_acct = reinterpret_cast<Acct *>(b);
+  char *cur = b + ((sizeof(Acct) + 255) / 256) * 256;
+  uint32_t *d_bits = reinterpret_cast<uint32_t *>(cur);
+  cur += nbits_words * 4;
+  cur = reinterpret_cast<char *>((reinterpret_cast<uintptr_t>(cur) + 7) & ~7ull);
+  unsigned long long *d_sel = reinterpret_cast<unsigned long long *>(cur);
+  cur += 8;
+  unsigned long long *d_lines = reinterpret_cast<unsigned long long *>(cur);
+  cur += h.n * 8;
+  uint32_t *d_offs = reinterpret_cast<uint32_t *>(cur);
+  h.bits = d_bits;
+  h.selected = d_sel;
+  st = launch_fill_u32(ctx->stream, d_bits, 0, nbits_words);
+  if (!st) st = launch_fill_u64(ctx->stream, d_sel, 0, 1 + h.n);
+  if (!st) {
+    hipError_t e = hipMemcpyAsync(d_acct, &h, sizeof(Acct),
+                                  hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(d_offs, offs.data(), offs.size() * 4,
+                         hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) st = hip_fail(e, "alg_bytes upload");
+  }
+  if (!st) st = launch_fused(ctx, q, d_acct);
+  if (!st && h.n) {
+    hipLaunchKernelGGL(acct_count_kernel, dim3(h.n), dim3(256), 0, ctx->stream,
+                       d_bits, d_offs, h.n, d_lines);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) st = hip_fail(e, "alg_bytes count");
+  }
+  std::vector<unsigned long long> lines(h.n + 1, 0);
+  if (!st) {
+    hipError_t e = hipMemcpyAsync(lines.data(), d_sel, (h.n + 1) * 8,
+                                  hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) st = hip_fail(e, "alg_bytes readback");
+  } else {
+    (void)hipStreamSynchronize(ctx->stream);
+  }
+  ctx->pool->free(buf);
+  ctx->pool->free(dummy_out);
+  if (st) return st;
+  uint64_t tot = 0;
+  for (uint32_t i = 0; i < h.n; ++i) tot += lines[1 + i] * 128ull;
+  tot += 8ull * nr_grp;
+  if (bytes) *bytes = tot;
+  if (selected) *selected = lines[0];
+  return MBX_OK;
+}
+
+}  // extern "C"

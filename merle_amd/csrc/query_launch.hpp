@@ -1,0 +1,28 @@
+// query_launch.hpp -- host-side launch interface of the single-query plans.
+#pragma once
+#include "mbx_internal.hpp"
+
+namespace mbx {
+
+struct Acct;
+
+struct QueryPlan {
+  DevPrg prg;
+  DevOp op;
+  unsigned long long *out = nullptr;
+  uint32_t nr_grp = 0;
+  bool lds = false;   // group table fits in LDS
+};
+
+// Validates the query and builds its device descriptors.
+int prepare(mbx_ctx *ctx, const mbx_vprg *prg, const mbx_rowop *op,
+            uint32_t nr_grp, int64_t *d_out, QueryPlan *q);
+int launch_fused(mbx_ctx *ctx, const QueryPlan &q, const Acct *acct);
+int launch_scan(mbx_ctx *ctx, const QueryPlan &q);
+int launch_stage1(mbx_ctx *ctx, const DevPrg &P, uint64_t *po, uint64_t *uo);
+int launch_stage2(mbx_ctx *ctx, const uint64_t *po, const uint64_t *uo,
+                  uint32_t nwords, uint32_t *rows, uint32_t *count);
+int launch_stage3(mbx_ctx *ctx, const QueryPlan &q, const uint32_t *rows,
+                  const uint32_t *count);
+
+}  // namespace mbx
