@@ -247,6 +247,88 @@ int mbx_timer_destroy(mbx_timer *t);
 /* Plain device copy of `bytes` (for byte-counter calibration). */
 int mbx_copy_kernel(mbx_ctx *ctx, void *dst, const void *src, size_t bytes);
 
+
+/* ------------------------------------------------------------ plans */
+/* A plan of one query: the column order of a conjunctive program
+ * (ORM r0 <- c ; ANDM r0 <- c ; ...) and the execution plan (MBX_PLAN_*).
+ * Every plan of a query selects the same rows and gives the same sums. */
+#define MBX_NFEAT 16
+typedef struct mbx_plan {
+  uint8_t order[MBX_MAXCOLS];
+  uint8_t norder;
+  uint8_t reorderable;      /* program is a pure conjunction */
+  uint16_t pad_;
+  uint32_t flags;           /* MBX_PLAN_FUSED or MBX_PLAN_MATERIALISE */
+} mbx_plan;
+typedef struct mbx_mlp mbx_mlp;
+
+/* Fraction of rows in each column's operand (OR of its bins); columns
+ * without bitmaps get 0.  Popcounts run on the device. */
+int mbx_col_density(mbx_ctx *ctx, const mbx_vprg *prg, double dens[MBX_MAXCOLS]);
+int mbx_plan_describe(const mbx_vprg *prg, mbx_plan *out);
+int mbx_plan_apply(const mbx_vprg *in, const mbx_plan *plan, mbx_vprg *out);
+int mbx_plan_features(const mbx_vprg *prg, const double *dens, uint32_t nr_grp,
+                      const mbx_plan *plan, float *feat /* MBX_NFEAT */);
+int mbx_plan_enumerate(const mbx_vprg *prg, mbx_plan *out, int max, int *count);
+/* Static heuristic: ascending density, fused. */
+int mbx_plan_heuristic(const mbx_vprg *prg, const double *dens, mbx_plan *out);
+/* Learned chooser (cost-model MLP over every candidate); mlp == NULL falls
+ * back to the heuristic. */
+int mbx_plan_choose(mbx_ctx *ctx, mbx_mlp *mlp, const mbx_vprg *prg,
+                    const double *dens, uint32_t nr_grp, mbx_plan *best,
+                    float *predicted);
+int mbx_prog_grpby_plan(mbx_ctx *ctx, const mbx_vprg *prg, const mbx_rowop *op,
+                        uint32_t nr_grp, int64_t *d_out, const mbx_plan *plan);
+
+/* ------------------------------------------------------ cost-model MLP */
+/* y = W2 relu(W1 x + b1) + b2 in fp32; parameters flat W1|b1|W2|b2.
+ * Forward, backward and Adam run as HIP kernels on the device. */
+int mbx_mlp_create(mbx_ctx *ctx, int din, int dh, int dout, uint64_t seed,
+                   mbx_mlp **out);
+int mbx_mlp_destroy(mbx_ctx *ctx, mbx_mlp *m);
+int mbx_mlp_nparams(const mbx_mlp *m);
+int mbx_mlp_set_params(mbx_ctx *ctx, mbx_mlp *m, const float *w);
+int mbx_mlp_get_params(mbx_ctx *ctx, mbx_mlp *m, float *w);
+int mbx_mlp_forward(mbx_ctx *ctx, mbx_mlp *m, const float *d_x, int n,
+                    float *d_y);
+/* loss = mean (y-t)^2; grad (host, nparams) may be NULL */
+int mbx_mlp_loss_grad(mbx_ctx *ctx, mbx_mlp *m, const float *d_x,
+                      const float *d_t, int n, double *loss, float *grad);
+int mbx_mlp_train(mbx_ctx *ctx, mbx_mlp *m, const float *d_x, const float *d_t,
+                  int n, int steps, float lr, double *loss);
+
+/* ------------------------------------------------- data: generators, files */
+/* Seeded synthetic columns, bit-identical on host and device: value i is a
+ * pure function of (seed, i) (splitmix64 mix; uniform by multiply-high;
+ * finite Zipf(s) on {0..nvals-1} by an integer inverse-CDF table). */
+uint64_t mbx_rand_at(uint64_t seed, uint64_t i);
+int mbx_zipf_table(uint32_t nvals, double s, uint64_t *thresholds);
+int mbx_gen_uniform_u32_host(uint32_t *out, size_t n, uint64_t seed,
+                             uint32_t lo, uint32_t hi);
+int mbx_gen_zipf_u32_host(uint32_t *out, size_t n, uint64_t seed,
+                          uint32_t nvals, double s);
+int mbx_gen_uniform_u32(mbx_ctx *ctx, uint32_t *d_out, size_t n,
+                        uint64_t seed, uint32_t lo, uint32_t hi);
+int mbx_gen_zipf_u32(mbx_ctx *ctx, uint32_t *d_out, size_t n, uint64_t seed,
+                     uint32_t nvals, double s);
+/* d_out[i] = d_in[i] / div (% mod when mod != 0): nested attributes. */
+int mbx_gen_map_u32(mbx_ctx *ctx, uint32_t *d_out, const uint32_t *d_in,
+                    size_t n, uint32_t div, uint32_t mod);
+
+/* Column files: 4-byte LE header (element BYTES when header_kind = 0, as
+ * ssb-dbgen-lite.c:192-201 writes; BITS when 1, as synth/onegen_zipf.py:73-84
+ * writes) + raw LE values; readers size by the file (ssb/cpuref.c:8-27). */
+int mbx_colfile_write(const char *path, const void *data, size_t n,
+                      uint32_t elem_bytes, int header_kind);
+int mbx_colfile_read(const char *path, void **data, size_t *nbytes,
+                     uint32_t *header);
+void mbx_colfile_free(void *p);
+int mbx_colfile_load(mbx_ctx *ctx, const char *path, void **dptr,
+                     size_t *nbytes, uint32_t *header);
+
+/* sizeof of every ABI struct in declaration order (binding checks). */
+int mbx_abi_sizes(size_t *out, int n);
+
 #ifdef __cplusplus
 }
 #endif

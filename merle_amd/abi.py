@@ -113,6 +113,12 @@ PROTOTYPES = {
                               _u32]),
     "mbx_alg_bytes": (_i32, [_vp, _P(Vprg), _P(RowOp), _u32, _P(_u64),
                              _P(_u64)]),
+    "mbx_batch_create": (_i32, [_vp, _P(Query), _u32, _P(Tune), _P(_vp)]),
+    "mbx_batch_run": (_i32, [_vp, _vp]),
+    "mbx_batch_run_eager": (_i32, [_vp, _vp]),
+    "mbx_batch_destroy": (_i32, [_vp, _vp]),
+    "mbx_autotune": (_i32, [_vp, _P(Query), _u32, _P(Tune), _P(C.c_float)]),
+    "mbx_default_tune": (_i32, [_P(Tune)]),
     "mbx_timer_create": (_i32, [_vp, _P(_vp)]),
     "mbx_timer_start": (_i32, [_vp, _vp]),
     "mbx_timer_stop": (_i32, [_vp, _vp, _P(C.c_float)]),

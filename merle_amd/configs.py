@@ -1,0 +1,219 @@
+"""The benchmark workloads as concrete, seeded synthetic inputs.
+
+config 1  plain method: 12M-row fact table (scale 2 x 6M rows, the
+          batchgen.sh:13 convention), columns p0..p2 ~ finite Zipf(s=1.0) on
+          {0..999}, measure m ~ uniform [1, 100000]; 40 equal-width bins per
+          predicate column; one conjunctive 3-predicate program, SUM(m) into
+          int64 grouped by p2 (25 groups).
+config 2  star schema: 120M-row fact table, int32 foreign keys into date
+          (2,556), customer (600K), supplier (40K) and part (1.4M)
+          dimensions; one bitmap per attribute value projected onto the fact
+          rows; 12 SSB-flight-shaped queries (conjunctions of 2-4
+          predicates, selectivity 1e-2..1e-5, up to 7x250x250 groups).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi, load
+from .host import check
+from .spec import Key, Pred, QuerySpec, Schema, Src
+
+SEED = 0x6d65726c65  # "merle"
+
+
+def col_seed(base, idx):
+    return load().mbx_rand_at(base, 1000 + idx)
+
+
+# ---------------------------------------------------------------- config 1
+C1_ROWS = 12_000_000
+C1_DOMAIN = 1000
+C1_BOUNDS = tuple(range(0, 1001, 25))        # 40 bins of width 25
+C1_COLS = ("p0", "p1", "p2")
+
+
+def config1_spec(misaligned=False):
+    """ORM p0 in [25,75) ; ANDM p1 in [25,75) ; ANDM p2 in [0,25);
+    SUM(m) GROUP BY p2.  The misaligned variant asks p0 in [30,80), whose
+    edge bins need a candidate check."""
+    lo0, hi0 = (30, 80) if misaligned else (25, 75)
+    preds = [Pred(Src("p0"), lo0, hi0), Pred(Src("p1"), 25, 75),
+             Pred(Src("p2"), 0, 25)]
+    ins = [(abi.ORM, 0, 0), (abi.ANDM, 0, 1), (abi.ANDM, 0, 2)]
+    return QuerySpec(preds, ins, [Key(Src("p2"))], 25, abi.MEAS_COL, Src("m"),
+                     None, "config1" + ("-misaligned" if misaligned else ""))
+
+
+def config1_host(n=C1_ROWS, seed=SEED, s=1.0):
+    """Host copy of the config-1 columns (the same values the device
+    generator writes)."""
+    lib = load()
+    out = {}
+    for i, name in enumerate(C1_COLS):
+        a = np.empty(n, np.uint32)
+        check(lib.mbx_gen_zipf_u32_host(a.ctypes.data, n, col_seed(seed, i),
+                                        C1_DOMAIN, s))
+        out[name] = a
+    m = np.empty(n, np.uint32)
+    check(lib.mbx_gen_uniform_u32_host(m.ctypes.data, n, col_seed(seed, 9), 1,
+                                       100001))
+    out["m"] = m
+    return out
+
+
+def config1_device(dev, n=C1_ROWS, seed=SEED, s=1.0):
+    """Device columns of config 1 -> Schema (host copies left empty)."""
+    lib = dev.lib
+    sch = Schema(n)
+    for i, name in enumerate(C1_COLS):
+        a = dev.alloc(4 * n, np.uint32, n)
+        check(lib.mbx_gen_zipf_u32(dev.ctx, a.ptr, n, col_seed(seed, i),
+                                   C1_DOMAIN, s))
+        sch.add(name, None, a)
+    m = dev.alloc(4 * n, np.uint32, n)
+    check(lib.mbx_gen_uniform_u32(dev.ctx, m.ptr, n, col_seed(seed, 9), 1,
+                                  100001))
+    sch.add("m", None, m)
+    dev.sync()
+    return sch
+
+
+# ---------------------------------------------------------------- config 2
+C2_ROWS = 120_000_000
+C2_DIMS = {"date": 2556, "cust": 600_000, "supp": 40_000, "part": 1_400_000}
+# dimension attribute -> (dimension, fact fk column, cardinality)
+C2_ATTRS = {
+    "d_year": ("date", "lo_date", 7),
+    "c_region": ("cust", "lo_cust", 5), "c_nation": ("cust", "lo_cust", 25),
+    "c_city": ("cust", "lo_cust", 250),
+    "s_region": ("supp", "lo_supp", 5), "s_nation": ("supp", "lo_supp", 25),
+    "s_city": ("supp", "lo_supp", 250),
+    "p_mfgr": ("part", "lo_part", 5), "p_category": ("part", "lo_part", 25),
+    "p_brand": ("part", "lo_part", 1000),
+}
+YEAR_DAY0 = (0, 366, 731, 1096, 1461, 1827, 2192)   # 1992..1998 (ssbdemo.h)
+
+
+def _src(attr):
+    return Src(attr, C2_ATTRS[attr][1])
+
+
+def _eq(attr, v):
+    return Pred(_src(attr), v, v + 1)
+
+
+def _key(attr, base, stride, div=1):
+    kind = abi.KEY_DIV if div > 1 else abi.KEY_IDENT
+    return Key(_src(attr), kind, div, base, stride)
+
+
+REV, COST = Src("lo_revenue"), Src("lo_supplycost")
+O, A, M, N_, R = abi.ORM, abi.ANDM, abi.AND, abi.NOT, abi.OR
+
+
+def config2_queries():
+    """12 queries modeled on the SSB flights (ssb/cpuref.c:252-298).  Each
+    predicate i is program column i; every bin is exact (one per value)."""
+    Q = []
+    # flight 1: no group-by, SUM(revenue)
+    Q.append(QuerySpec([_eq("d_year", 1), _eq("c_nation", 7)],
+                       [(O, 0, 0), (A, 0, 1)], [], 1, abi.MEAS_COL, REV,
+                       None, "Q1.1"))
+    Q.append(QuerySpec([_eq("d_year", 2), _eq("c_nation", 7), _eq("s_region", 2)],
+                       [(O, 0, 0), (A, 0, 1), (A, 0, 2)], [], 1, abi.MEAS_COL,
+                       REV, None, "Q1.2"))
+    # flight 2: part x supplier, GROUP BY year, brand
+    Q.append(QuerySpec([_eq("p_category", 12), _eq("s_region", 1)],
+                       [(O, 0, 0), (A, 0, 1)],
+                       [_key("d_year", 0, 40), _key("p_brand", 480, 1)], 280,
+                       abi.MEAS_COL, REV, None, "Q2.1"))
+    Q.append(QuerySpec([Pred(_src("p_brand"), 521, 525), _eq("s_region", 2)],
+                       [(O, 0, 0), (A, 0, 1)],
+                       [_key("d_year", 0, 4), _key("p_brand", 521, 1)], 28,
+                       abi.MEAS_COL, REV, None, "Q2.2"))
+    Q.append(QuerySpec([_eq("p_brand", 777), _eq("s_nation", 3)],
+                       [(O, 0, 0), (A, 0, 1)], [_key("d_year", 0, 1)], 7,
+                       abi.MEAS_COL, REV, None, "Q2.3"))
+    # flight 3: customer x supplier x date; year != 1998 as NOT + AND
+    # (the reference's program for Q3.x, ssb/fixedlayout.cu:93-97)
+    Q.append(QuerySpec([_eq("c_region", 2), _eq("s_nation", 11), _eq("d_year", 6)],
+                       [(O, 0, 0), (A, 0, 1), (O, 1, 2), (N_, 1, 1), (M, 0, 1)],
+                       [_key("c_nation", 10, 70), _key("s_city", 110, 7),
+                        _key("d_year", 0, 1)], 350, abi.MEAS_COL, REV, None,
+                       "Q3.1"))
+    Q.append(QuerySpec([_eq("c_nation", 14), _eq("s_nation", 14), _eq("d_year", 6)],
+                       [(O, 0, 0), (A, 0, 1), (O, 1, 2), (N_, 1, 1), (M, 0, 1)],
+                       [_key("c_city", 140, 70), _key("s_city", 140, 7),
+                        _key("d_year", 0, 1)], 700, abi.MEAS_COL, REV, None,
+                       "Q3.2"))
+    Q.append(QuerySpec([Pred(_src("c_city"), 141, 143),
+                        Pred(_src("s_city"), 141, 143)],
+                       [(O, 0, 0), (A, 0, 1)],
+                       [_key("c_city", 141, 14), _key("s_city", 141, 7),
+                        _key("d_year", 0, 1)], 28, abi.MEAS_COL, REV, None,
+                       "Q3.3"))
+    Q.append(QuerySpec([Pred(_src("c_city"), 141, 143),
+                        Pred(_src("s_city"), 141, 143), _eq("d_year", 5)],
+                       [(O, 0, 0), (A, 0, 1), (A, 0, 2)],
+                       [_key("c_city", 141, 2), _key("s_city", 141, 1)], 4,
+                       abi.MEAS_COL, REV, None, "Q3.4"))
+    # flight 4: SUM(revenue - supplycost)
+    Q.append(QuerySpec([_eq("c_region", 1), _eq("s_nation", 6),
+                        Pred(_src("p_mfgr"), 0, 2)],
+                       [(O, 0, 0), (A, 0, 1), (A, 0, 2)],
+                       [_key("d_year", 0, 5), _key("c_nation", 5, 1)], 35,
+                       abi.MEAS_SUB, REV, COST, "Q4.1"))
+    Q.append(QuerySpec([_eq("c_region", 1), _eq("s_region", 1),
+                        Pred(_src("p_mfgr"), 0, 2), Pred(_src("d_year"), 5, 7)],
+                       [(O, 0, 0), (A, 0, 1), (A, 0, 2), (A, 0, 3)],
+                       [_key("d_year", 0, 62500), _key("c_city", 0, 250),
+                        _key("s_city", 0, 1)], 437_500, abi.MEAS_SUB, REV, COST,
+                       "Q4.2"))
+    Q.append(QuerySpec([Pred(_src("d_year"), 5, 7), _eq("c_region", 1),
+                        _eq("s_nation", 6), _eq("p_category", 3)],
+                       [(O, 0, 0), (A, 0, 1), (A, 0, 2), (A, 0, 3)],
+                       [_key("d_year", 5, 400), _key("s_city", 60, 40),
+                        _key("p_brand", 120, 1)], 800, abi.MEAS_SUB, REV, COST,
+                       "Q4.3"))
+    return Q
+
+
+def config2_device(dev, n=C2_ROWS, seed=SEED):
+    """Device tables of config 2 -> Schema.  Values are uniform; nested
+    attributes follow ssb-dbgen-lite.c:24-27 (city -> nation = city/10 ->
+    region = nation/5) and brand -> category = brand/40 -> mfgr = brand/200;
+    revenue and supply cost follow the ranges of ssb-dbgen-lite.c:132-137."""
+    lib = dev.lib
+    sch = Schema(n)
+
+    def gen(name, count, lo, hi, idx):
+        a = dev.alloc(4 * count, np.uint32, count)
+        check(lib.mbx_gen_uniform_u32(dev.ctx, a.ptr, count, col_seed(seed, idx),
+                                      lo, hi))
+        sch.add(name, None, a)
+        return a
+
+    gen("lo_date", n, 0, C2_DIMS["date"], 20)
+    gen("lo_cust", n, 0, C2_DIMS["cust"], 21)
+    gen("lo_supp", n, 0, C2_DIMS["supp"], 22)
+    gen("lo_part", n, 0, C2_DIMS["part"], 23)
+    gen("lo_revenue", n, 0, 55450, 24)
+    gen("lo_supplycost", n, 0, 131072, 25)
+    # date dimension: year of each day offset
+    days = np.arange(C2_DIMS["date"], dtype=np.int64)
+    year = (np.searchsorted(np.array(YEAR_DAY0[1:]), days, side="right")).astype(np.uint32)
+    sch.add("d_year", year, dev.upload(year))
+    for pre, dim, idx in (("c", "cust", 26), ("s", "supp", 27)):
+        city = gen(f"{pre}_city", C2_DIMS[dim], 0, 250, idx)
+        for name, div in ((f"{pre}_nation", 10), (f"{pre}_region", 50)):
+            a = dev.alloc(4 * C2_DIMS[dim], np.uint32, C2_DIMS[dim])
+            check(lib.mbx_gen_map_u32(dev.ctx, a.ptr, city.ptr, C2_DIMS[dim], div, 0))
+            sch.add(name, None, a)
+    brand = gen("p_brand", C2_DIMS["part"], 0, 1000, 28)
+    for name, div in (("p_category", 40), ("p_mfgr", 200)):
+        a = dev.alloc(4 * C2_DIMS["part"], np.uint32, C2_DIMS["part"])
+        check(lib.mbx_gen_map_u32(dev.ctx, a.ptr, brand.ptr, C2_DIMS["part"], div, 0))
+        sch.add(name, None, a)
+    dev.sync()
+    return sch

@@ -1,0 +1,312 @@
+// batch.hip -- many queries in one persistent launch, captured in a hipGraph.
+//
+// The reference runs one query per launch (ssb/lambdas.cuh:179-184).  Here a
+// batch of queries becomes a work list of (query, tile range) items, ordered
+// query by query.  A persistent grid pulls items from a device counter; for
+// each item a workgroup zeroes its LDS group table, runs the same tile
+// pipeline as the single-query kernel (program -> compaction -> row op) over
+// the item's tiles, and flushes the table to that query's int64 output.
+// Output zeroing, counter reset and the launch are captured once into a
+// hipGraph (fill kernels only: no memset nodes) and replayed per run.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "devquery.hpp"
+#include "query_launch.hpp"
+
+namespace mbx {
+
+static constexpr int kBWPL = 2;                    // words per thread per tile
+static constexpr uint32_t kTileWords = 256 * kBWPL;
+
+struct BatchQ {
+  DevPrg prg;
+  DevOp op;
+  unsigned long long *out;
+  uint32_t nr_grp, lds, chk, ntiles;
+  uint32_t item0, nitems, tiles_per_item, pad;
+};
+
+struct BatchArgs {
+  const BatchQ *qs;
+  uint32_t nq;
+  uint32_t total_items;
+  uint32_t *counter;
+};
+
+__global__ __launch_bounds__(256) void batch_kernel(BatchArgs a) {
+  __shared__ uint32_t s_list[kListCap];
+  __shared__ uint32_t s_wave[4];
+  __shared__ uint32_t s_item;
+  extern __shared__ unsigned long long s_grp[];
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(a.counter, 1u);
+    __syncthreads();
+    const uint32_t item = __builtin_amdgcn_readfirstlane(s_item);
+    if (item >= a.total_items) break;   // every thread sees the same item
+    // the query owning this item: last q with item0 <= item
+    uint32_t lo = 0, hi = a.nq - 1;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1) >> 1;
+      if (a.qs[mid].item0 <= item) lo = mid;
+      else hi = mid - 1;
+    }
+    const BatchQ &q = a.qs[lo];
+    GroupSink sink;
+    sink.lds = q.lds ? s_grp : nullptr;
+    sink.out = q.out;
+    if (q.lds)
+      for (uint32_t i = threadIdx.x; i < q.nr_grp; i += 256) s_grp[i] = 0;
+    __syncthreads();
+    const uint32_t t0 = (item - q.item0) * q.tiles_per_item;
+    const uint32_t t1 = min(t0 + q.tiles_per_item, q.ntiles);
+    for (uint32_t t = t0; t < t1; ++t) {
+      if (q.chk)
+        run_tile<true, false, kBWPL>(q.prg, q.op, q.nr_grp, t, sink, s_list,
+                                     s_wave, nullptr);
+      else
+        run_tile<false, false, kBWPL>(q.prg, q.op, q.nr_grp, t, sink, s_list,
+                                      s_wave, nullptr);
+    }
+    if (q.lds) {
+      __syncthreads();
+      for (uint32_t i = threadIdx.x; i < q.nr_grp; i += 256) {
+        const unsigned long long v = s_grp[i];
+        if (v) atomicAdd(q.out + i, v);
+      }
+    }
+    __syncthreads();   // s_item and the LDS table are reused next round
+  }
+}
+
+__global__ void zero_outputs_kernel(const BatchQ *qs, uint32_t nq) {
+  for (uint32_t q = blockIdx.x; q < nq; q += gridDim.x) {
+    unsigned long long *o = qs[q].out;
+    const uint32_t g = qs[q].nr_grp;
+    for (uint32_t i = threadIdx.x; i < g; i += blockDim.x) o[i] = 0;
+  }
+}
+
+}  // namespace mbx
+
+using namespace mbx;
+
+struct mbx_batch {
+  BatchQ *d_qs = nullptr;     // pool memory: descriptors then the counter
+  uint32_t *d_counter = nullptr;
+  uint32_t nq = 0, total_items = 0;
+  size_t lds_bytes = 0;
+  unsigned grid = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  mbx_tune tune{};
+};
+
+namespace mbx {
+
+static int enqueue_batch(mbx_ctx *ctx, mbx_batch *b) {
+  hipLaunchKernelGGL(zero_outputs_kernel, dim3(std::min(b->nq, 1024u)),
+                     dim3(256), 0, ctx->stream, b->d_qs, b->nq);
+  MBX_HIP(hipGetLastError());
+  int st = launch_fill_u32(ctx->stream, b->d_counter, 0, 1);
+  if (st) return st;
+  BatchArgs a;
+  a.qs = b->d_qs;
+  a.nq = b->nq;
+  a.total_items = b->total_items;
+  a.counter = b->d_counter;
+  hipLaunchKernelGGL(batch_kernel, dim3(b->grid), dim3(256), b->lds_bytes,
+                     ctx->stream, a);
+  MBX_HIP(hipGetLastError());
+  return MBX_OK;
+}
+
+static void destroy_batch(mbx_ctx *ctx, mbx_batch *b) {
+  if (!b) return;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->d_qs && ctx) ctx->pool->free(b->d_qs);
+  delete b;
+}
+
+static int valid_tune(const mbx_tune *t) {
+  if (t->tile_words == 0 || t->tile_words % kTileWords)
+    return fail(MBX_ERR_INVALID, "tune: tile_words must be a multiple of 512");
+  if (t->wg_per_cu == 0 || t->wg_per_cu > 16)
+    return fail(MBX_ERR_INVALID, "tune: wg_per_cu must be in 1..16");
+  if (t->lds_groups > kLdsGroups)
+    return fail(MBX_ERR_INVALID, "tune: lds_groups above 4096");
+  return MBX_OK;
+}
+
+static int build_batch(mbx_ctx *ctx, const mbx_query *qs, uint32_t nq,
+                       const mbx_tune *tune, bool capture, mbx_batch **out) {
+  if (!ctx || !qs || !out || nq == 0)
+    return fail(MBX_ERR_INVALID, "batch: bad argument");
+  *out = nullptr;
+  mbx_tune t;
+  if (tune) t = *tune;
+  else mbx_default_tune(&t);
+  int st = valid_tune(&t);
+  if (st) return st;
+  std::vector<BatchQ> h(nq);
+  uint32_t items = 0, max_lds_grp = 0;
+  for (uint32_t i = 0; i < nq; ++i) {
+    QueryPlan p;
+    st = prepare(ctx, &qs[i].prg, &qs[i].op, qs[i].nr_grp, qs[i].d_out, &p);
+    if (st) {
+      set_error("batch query " + std::to_string(i) + ": " + mbx_last_error());
+      return st;
+    }
+    BatchQ &b = h[i];
+    std::memset(&b, 0, sizeof(b));
+    b.prg = p.prg;
+    b.op = p.op;
+    b.out = p.out;
+    b.nr_grp = p.nr_grp;
+    b.lds = p.nr_grp <= t.lds_groups ? 1u : 0u;
+    if (b.lds) max_lds_grp = std::max(max_lds_grp, p.nr_grp);
+    b.chk = p.prg.chk;
+    b.ntiles = (uint32_t)ceil_div(p.prg.nwords, kTileWords);
+    b.tiles_per_item = t.tile_words / kTileWords;
+    b.nitems = (uint32_t)ceil_div(b.ntiles, b.tiles_per_item);
+    b.item0 = items;
+    items += b.nitems;
+  }
+  auto *b = new mbx_batch;
+  b->nq = nq;
+  b->total_items = items;
+  b->tune = t;
+  b->lds_bytes = (size_t)max_lds_grp * 8;
+  b->grid = (unsigned)std::max<uint64_t>(
+      1, std::min<uint64_t>((uint64_t)ctx->num_cus * t.wg_per_cu, items));
+  void *mem = nullptr;
+  const size_t qbytes = ceil_div(sizeof(BatchQ) * nq, 256) * 256;
+  st = ctx->pool->alloc(qbytes + 256, &mem);
+  if (st) {
+    delete b;
+    return st;
+  }
+  b->d_qs = static_cast<BatchQ *>(mem);
+  b->d_counter = reinterpret_cast<uint32_t *>(static_cast<char *>(mem) + qbytes);
+  hipError_t e = hipMemcpyAsync(b->d_qs, h.data(), sizeof(BatchQ) * nq,
+                                hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    destroy_batch(ctx, b);
+    return hip_fail(e, "batch: descriptor upload");
+  }
+  if (capture) {
+    e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed);
+    if (e != hipSuccess) {
+      destroy_batch(ctx, b);
+      return hip_fail(e, "batch: begin capture");
+    }
+    st = enqueue_batch(ctx, b);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(ctx->stream, &g);
+    b->graph = g;
+    if (st || e != hipSuccess) {
+      destroy_batch(ctx, b);
+      return st ? st : hip_fail(e, "batch: end capture");
+    }
+    e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      destroy_batch(ctx, b);
+      return hip_fail(e, "batch: graph instantiate");
+    }
+  }
+  *out = b;
+  return MBX_OK;
+}
+
+}  // namespace mbx
+
+extern "C" {
+
+int mbx_default_tune(mbx_tune *t) {
+  if (!t) return fail(MBX_ERR_INVALID, "null tune");
+  t->tile_words = 2048;
+  t->wg_per_cu = 4;
+  t->lds_groups = kLdsGroups;
+  t->pad_ = 0;
+  return MBX_OK;
+}
+
+int mbx_batch_create(mbx_ctx *ctx, const mbx_query *qs, uint32_t nq,
+                     const mbx_tune *tune, mbx_batch **out) {
+  return build_batch(ctx, qs, nq, tune, true, out);
+}
+
+int mbx_batch_run(mbx_ctx *ctx, mbx_batch *b) {
+  if (!ctx || !b) return fail(MBX_ERR_INVALID, "batch_run: null");
+  if (!b->exec) return enqueue_batch(ctx, b);
+  MBX_HIP(hipGraphLaunch(b->exec, ctx->stream));
+  return MBX_OK;
+}
+
+int mbx_batch_run_eager(mbx_ctx *ctx, mbx_batch *b) {
+  if (!ctx || !b) return fail(MBX_ERR_INVALID, "batch_run_eager: null");
+  return enqueue_batch(ctx, b);
+}
+
+int mbx_batch_destroy(mbx_ctx *ctx, mbx_batch *b) {
+  if (ctx) (void)hipStreamSynchronize(ctx->stream);
+  destroy_batch(ctx, b);
+  return MBX_OK;
+}
+
+// Project-written autotuner: times every candidate launch shape on the
+// batch itself (eager launches, median of 3 after one warm-up) and keeps the
+// fastest.  Results do not depend on the shape: each candidate computes the
+// same sums.
+int mbx_autotune(mbx_ctx *ctx, const mbx_query *qs, uint32_t nq,
+                 mbx_tune *best, float *best_ms) {
+  if (!ctx || !qs || !best) return fail(MBX_ERR_INVALID, "autotune: null");
+  static const uint32_t tiles[] = {512, 1024, 2048, 4096, 8192};
+  static const uint32_t wgs[] = {2, 4, 8};
+  static const uint32_t ldsg[] = {1024, 4096};
+  hipEvent_t e0, e1;
+  MBX_HIP(hipEventCreate(&e0));
+  MBX_HIP(hipEventCreate(&e1));
+  float best_t = 1e30f;
+  mbx_tune bt;
+  mbx_default_tune(&bt);
+  int st = MBX_OK;
+  for (uint32_t tw : tiles)
+    for (uint32_t wg : wgs)
+      for (uint32_t lg : ldsg) {
+        mbx_tune t{tw, wg, lg, 0};
+        mbx_batch *b = nullptr;
+        st = build_batch(ctx, qs, nq, &t, false, &b);
+        if (st) goto done;
+        float reps[3];
+        for (int r = -1; r < 3 && !st; ++r) {
+          (void)hipEventRecord(e0, ctx->stream);
+          st = enqueue_batch(ctx, b);
+          (void)hipEventRecord(e1, ctx->stream);
+          if (hipEventSynchronize(e1) != hipSuccess)
+            st = fail(MBX_ERR_HIP, "autotune: sync");
+          float ms = 0;
+          (void)hipEventElapsedTime(&ms, e0, e1);
+          if (r >= 0) reps[r] = ms;
+        }
+        destroy_batch(ctx, b);
+        if (st) goto done;
+        std::sort(reps, reps + 3);
+        if (reps[1] < best_t) {
+          best_t = reps[1];
+          bt = t;
+        }
+      }
+done:
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st) return st;
+  *best = bt;
+  if (best_ms) *best_ms = best_t;
+  return MBX_OK;
+}
+
+}  // extern "C"

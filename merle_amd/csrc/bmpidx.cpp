@@ -72,7 +72,9 @@ int recipe_plan(int kind, int i, uint32_t mn, uint32_t mx, BinReq *r) {
   }
   if (kind == 2 && i == 1) {            // candchk column 1 (:284-298)
     if (step == 0) {
-      r[n++] = {mn - 1, mx, 5};
+      // the reference computes min - 1 in uint and so builds an empty bin
+      // when min == 0 (bmpidx_create.cu:289); clamp at 0 instead
+      r[n++] = {mn ? mn - 1 : 0, mx, 5};
     } else {
       r[n++] = {mn + 2 * step, mx + step / 2, 5};
       r[n++] = {mn, mn + step, 0};

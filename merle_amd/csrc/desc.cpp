@@ -1,0 +1,145 @@
+// desc.cpp -- validation and conversion of the ABI query descriptors
+// (mbx_vprg, mbx_rowop) into their kernel-argument forms (DevPrg, DevOp).
+// Host-only code.
+#include <cstring>
+#include <string>
+
+#include "mbx_internal.hpp"
+
+namespace mbx {
+
+int check_src(const mbx_colref &c, const char *what) {
+  if (c.data == nullptr)
+    return fail(MBX_ERR_INVALID, std::string(what) + ": null column");
+  if (c.elem_bytes != 1 && c.elem_bytes != 2 && c.elem_bytes != 4 &&
+      c.elem_bytes != 8)
+    return fail(MBX_ERR_INVALID,
+                std::string(what) + ": elem_bytes must be 1, 2, 4 or 8");
+  if ((reinterpret_cast<uintptr_t>(c.data) % c.elem_bytes) != 0 ||
+      (reinterpret_cast<uintptr_t>(c.fk) % 4) != 0)
+    return fail(MBX_ERR_ALIGN, std::string(what) + ": misaligned column");
+  return MBX_OK;
+}
+
+static DevSrc to_dev(const mbx_colref &c) {
+  DevSrc d;
+  d.data = c.data;
+  d.fk = c.fk;
+  d.dimsz = c.dimsz;
+  d.eb = c.elem_bytes;
+  return d;
+}
+
+static int check_bmp(const uint32_t *p) {
+  if (reinterpret_cast<uintptr_t>(p) % 8 != 0)
+    return fail(MBX_ERR_ALIGN, "bitmap pointer is not 8-byte aligned");
+  return MBX_OK;
+}
+
+// vprg validation follows the reference's limits (primitive.cuh:121,164-198):
+// registers r0..r2, column operands 0..3, at most 6 instructions.
+int make_dev_prg(const mbx_vprg *p, DevPrg *out) {
+  if (!p || !out) return fail(MBX_ERR_INVALID, "vprg: null");
+  if (p->factsz == 0) return fail(MBX_ERR_INVALID, "vprg: factsz == 0");
+  if (p->factsz > 0x7fffffffu)
+    return fail(MBX_ERR_CAPACITY, "vprg: factsz must be < 2^31");
+  std::memset(out, 0, sizeof(*out));
+  bool chk = false;
+  for (int c = 0; c < MBX_MAXCOLS; ++c) {
+    const mbx_col &col = p->cols[c];
+    DevCol &d = out->cols[c];
+    uint32_t nm = 0;
+    while (nm < MBX_MAXBIN_PERCOL && col.middle[nm]) {
+      int st = check_bmp(col.middle[nm]);
+      if (st) return st;
+      d.m[nm] = reinterpret_cast<const uint64_t *>(col.middle[nm]);
+      ++nm;
+    }
+    d.nm = nm;
+    if (col.leftmost) {
+      int st = check_bmp(col.leftmost);
+      if (st) return st;
+    }
+    if (col.rightmost) {
+      int st = check_bmp(col.rightmost);
+      if (st) return st;
+    }
+    d.l = reinterpret_cast<const uint64_t *>(col.leftmost);
+    d.r = reinterpret_cast<const uint64_t *>(col.rightmost);
+    d.has_edge = (col.leftmost || col.rightmost) ? 1u : 0u;
+    chk = chk || d.has_edge;
+  }
+  uint32_t n = 0;
+  for (; n < MBX_MAXNINSTR; ++n) {
+    const mbx_instr &in = p->instrs[n];
+    if (in.opcode == MBX_END) break;
+    if (in.opcode > MBX_ORM)
+      return fail(MBX_ERR_INVALID, "vprg: bad opcode " +
+                                       std::to_string(in.opcode));
+    if (in.dstreg > 2) return fail(MBX_ERR_INVALID, "vprg: dstreg > 2");
+    const bool mem = in.opcode == MBX_ANDM || in.opcode == MBX_ORM;
+    if (mem ? in.src1 >= MBX_MAXCOLS : in.src1 > 2)
+      return fail(MBX_ERR_INVALID, "vprg: src1 out of range");
+    out->ins[n][0] = in.opcode;
+    out->ins[n][1] = in.dstreg;
+    out->ins[n][2] = in.src1;
+    out->ins[n][3] = in.src2;
+  }
+  out->nins = n;
+  out->nrows = p->factsz;
+  out->nwords = (uint32_t)ceil_div(p->factsz, 64);
+  out->chk = chk ? 1u : 0u;
+  return MBX_OK;
+}
+
+int make_dev_op(const mbx_rowop *op, uint32_t npreds_hint, DevOp *out) {
+  if (!op || !out) return fail(MBX_ERR_INVALID, "rowop: null");
+  std::memset(out, 0, sizeof(*out));
+  (void)npreds_hint;
+  if (op->npreds > MBX_MAXCOLS || op->nkeys > MBX_MAXKEYS)
+    return fail(MBX_ERR_INVALID, "rowop: too many predicates or keys");
+  out->npreds = op->npreds;
+  for (uint32_t i = 0; i < op->npreds; ++i) {
+    int st = check_src(op->preds[i].src, "rowop predicate");
+    if (st) return st;
+    out->pred[i] = to_dev(op->preds[i].src);
+    out->lo[i] = op->preds[i].lo;
+    out->hi[i] = op->preds[i].hi;
+  }
+  out->nkeys = op->nkeys;
+  for (uint32_t i = 0; i < op->nkeys; ++i) {
+    const mbx_keypart &k = op->keys[i];
+    int st = check_src(k.src, "rowop key");
+    if (st) return st;
+    if (k.kind > MBX_KEY_BUCKET)
+      return fail(MBX_ERR_INVALID, "rowop key: bad kind");
+    if (k.kind == MBX_KEY_DIV && k.div == 0)
+      return fail(MBX_ERR_INVALID, "rowop key: div == 0");
+    if (k.nbounds > MBX_MAXBOUNDS)
+      return fail(MBX_ERR_INVALID, "rowop key: too many bounds");
+    DevKey &d = out->keys[i];
+    d.src = to_dev(k.src);
+    d.kind = k.kind;
+    d.div = k.div;
+    d.base = k.base;
+    d.stride = k.stride;
+    d.nb = k.nbounds;
+    for (uint32_t j = 0; j < MBX_MAXBOUNDS; ++j) d.bounds[j] = k.bounds[j];
+  }
+  if (op->measure > MBX_MEAS_SUB)
+    return fail(MBX_ERR_INVALID, "rowop: bad measure");
+  out->measure = op->measure;
+  if (op->measure != MBX_MEAS_COUNT) {
+    int st = check_src(op->ma, "rowop measure a");
+    if (st) return st;
+    out->ma = to_dev(op->ma);
+  }
+  if (op->measure == MBX_MEAS_MUL || op->measure == MBX_MEAS_SUB) {
+    int st = check_src(op->mb, "rowop measure b");
+    if (st) return st;
+    out->mb = to_dev(op->mb);
+  }
+  return MBX_OK;
+}
+
+}  // namespace mbx

@@ -233,21 +233,16 @@ __device__ __forceinline__ bool row_keyval(const DevOp &O, uint32_t row,
     k += kp.stride * ((int64_t)f - kp.base);
   }
   if (k < 0 || k >= (int64_t)nr_grp) return false;
-  uint64_t a = 0, b = 0;
-  if (O.measure == MBX_MEAS_COUNT) {
-    val = 1;
-  } else {
-    if (!fetch_a<ACCT>(O.ma, row, &a, acct)) return false;
-    if (O.measure == MBX_MEAS_COL) {
-      val = (int64_t)a;
-    } else {
-      if (!fetch_a<ACCT>(O.mb, row, &b, acct)) return false;
-      val = O.measure == MBX_MEAS_MUL ? (int64_t)(a * b)
-                                      : (int64_t)a - (int64_t)b;
-    }
-  }
   key = (uint32_t)k;
-  return true;
+  // measure: 1, a, a*b or a-b; one exit for every kind
+  uint64_t a = 1, b = 0;
+  bool ok = true;
+  if (O.measure != MBX_MEAS_COUNT) ok = fetch_a<ACCT>(O.ma, row, &a, acct);
+  if (ok && O.measure >= MBX_MEAS_MUL) ok = fetch_a<ACCT>(O.mb, row, &b, acct);
+  val = O.measure == MBX_MEAS_MUL   ? (int64_t)(a * b)
+        : O.measure == MBX_MEAS_SUB ? (int64_t)a - (int64_t)b
+                                    : (int64_t)a;
+  return ok;
 }
 
 // Aggregation target: an LDS table (small group counts) or the output in
@@ -257,8 +252,15 @@ struct GroupSink {
   unsigned long long *out;   // HBM output
 };
 
+// The key indexes memory, so it is bounds-checked right here, behind an
+// empty asm that hides its provenance from the optimizer: gfx950 code built
+// from an earlier shape of row_keyval used a stale register as the key on
+// some measure paths (the live key was overwritten by the measure load),
+// and a wild key must cost a wrong sum in a test, never a memory fault.
 __device__ __forceinline__ void sink_add(const GroupSink &s, uint32_t key,
-                                         int64_t val) {
+                                         int64_t val, uint32_t nr_grp) {
+  asm volatile("" : "+v"(key));
+  if (key >= nr_grp) return;
   if (s.lds) atomicAdd(s.lds + key, (unsigned long long)val);
   else atomicAdd(s.out + key, (unsigned long long)val);
 }
@@ -278,7 +280,7 @@ __device__ __forceinline__ void process_entry(const DevPrg &P, const DevOp &O,
     atomicAdd(acct->selected, 1ull);
     return;
   }
-  sink_add(sink, key, val);
+  sink_add(sink, key, val, nr_grp);
 }
 
 // ------------------------------------------------------- block primitives

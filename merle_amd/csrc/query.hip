@@ -66,7 +66,7 @@ __global__ __launch_bounds__(256) void scan_kernel(QArgs a) {
     uint32_t key;
     int64_t val;
     if (row_keyval<false>(a.op, row, a.nr_grp, key, val, nullptr))
-      sink_add(sink, key, val);
+      sink_add(sink, key, val, a.nr_grp);
   }
   if (LDSG) {
     __syncthreads();
@@ -166,6 +166,30 @@ __global__ void acct_count_kernel(const uint32_t *__restrict__ bits,
       c += __popc(bits[i]);
     atomicAdd(lines + e, c);
   }
+}
+
+
+// Rows in the OR of a column's bins (plan densities).
+__global__ void density_kernel(DevCol c, uint32_t nwords, uint32_t nrows,
+                               unsigned long long *out) {
+  __shared__ unsigned long long s[256];
+  unsigned long long cnt = 0;
+  for (uint32_t w = blockIdx.x * 256 + threadIdx.x; w < nwords;
+       w += gridDim.x * 256) {
+    uint64_t x = 0;
+    for (uint32_t k = 0; k < c.nm; ++k) x |= c.m[k][w];
+    if (c.l) x |= c.l[w];
+    if (c.r) x |= c.r[w];
+    if (w == nwords - 1 && (nrows & 63u)) x &= (1ull << (nrows & 63u)) - 1ull;
+    cnt += __popcll(x);
+  }
+  s[threadIdx.x] = cnt;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) s[threadIdx.x] += s[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && s[0]) atomicAdd(out, s[0]);
 }
 
 // ---------------------------------------------------------------- launches
@@ -507,6 +531,34 @@ int mbx_alg_bytes(mbx_ctx *ctx, const mbx_vprg *prg, const mbx_rowop *op,
   tot += 8ull * nr_grp;
   if (bytes) *bytes = tot;
   if (selected) *selected = lines[0];
+  return MBX_OK;
+}
+
+int mbx_col_density(mbx_ctx *ctx, const mbx_vprg *prg, double dens[MBX_MAXCOLS]) {
+  if (!ctx || !dens) return fail(MBX_ERR_INVALID, "col_density: null");
+  DevPrg P;
+  int st = make_dev_prg(prg, &P);
+  if (st) return st;
+  void *buf = nullptr;
+  st = ctx->pool->alloc(8 * MBX_MAXCOLS, &buf);
+  if (st) return st;
+  unsigned long long *d = static_cast<unsigned long long *>(buf);
+  st = launch_fill_u64(ctx->stream, d, 0, MBX_MAXCOLS);
+  const unsigned grid = persistent_grid(ctx, (uint32_t)ceil_div(P.nwords, 256), 4);
+  for (int c = 0; c < MBX_MAXCOLS && !st; ++c) {
+    if (!P.cols[c].nm && !P.cols[c].has_edge) continue;
+    hipLaunchKernelGGL(density_kernel, dim3(grid), dim3(256), 0, ctx->stream,
+                       P.cols[c], P.nwords, P.nrows, d + c);
+  }
+  unsigned long long h[MBX_MAXCOLS] = {0, 0, 0, 0};
+  hipError_t e = hipGetLastError();
+  if (!st && e == hipSuccess)
+    e = hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  ctx->pool->free(buf);
+  if (st) return st;
+  if (e != hipSuccess) return hip_fail(e, "col_density");
+  for (int c = 0; c < MBX_MAXCOLS; ++c) dens[c] = (double)h[c] / P.nrows;
   return MBX_OK;
 }
 

@@ -1,0 +1,181 @@
+"""GPU parity of the core path through the C ABI: bitmap creation, the fused
+program + group-by, the materialise and scan plans, against the oracle."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from merle_amd import abi, configs, spec as S
+from merle_amd.host import check, stride_words
+from oracle import cpuref
+
+import datagen
+import gpuq
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [1, 31, 32, 33, 63, 64, 65, 1000, 1023, 1024, 1025, 4099, 65537,
+         1 << 20 | 17]
+
+
+@pytest.mark.parametrize("n", SIZES)
+@pytest.mark.parametrize("dt", [np.uint8, np.uint16, np.uint32, np.uint64])
+def test_bins_all_modes(dev, n, dt):
+    """Byte-identical bitmaps vs ref_create_bin for every bin mode: width-1,
+    equal-width, sorted (uneven) and arbitrary (overlapping) bins."""
+    rng = np.random.default_rng(n + np.dtype(dt).itemsize)
+    v = rng.integers(0, 300, n).astype(dt)
+    d = dev.upload(v)
+    ref = abi.ColRef(d.ptr, None, 0, v.dtype.itemsize)
+    uneven = np.unique(rng.integers(0, 300, 70))
+    sets = [(list(range(10, 110)), list(range(11, 111))),          # unit, 100 bins
+            (list(range(0, 300, 30)), list(range(30, 330, 30))),   # equal
+            (list(uneven[:-1]), list(uneven[1:])),                 # sorted
+            (list(rng.integers(0, 150, 7)), list(rng.integers(100, 300, 7)))]
+    for mins, maxs in sets:
+        nb = len(mins)
+        outs = (C.c_void_p * nb)()
+        check(dev.lib.mbx_create_bin(dev.ctx, C.byref(ref), n,
+                                     (C.c_uint64 * nb)(*map(int, mins)),
+                                     (C.c_uint64 * nb)(*map(int, maxs)), nb,
+                                     outs, None))
+        got = gpuq.bitmaps_host(dev, list(outs), n)
+        want = cpuref.create_bin(v, mins, maxs)
+        np.testing.assert_array_equal(got, want)
+        check(dev.lib.mbx_free(dev.ctx, outs[0]))
+    d.free()
+
+
+@pytest.mark.parametrize("n", [1, 77, 4099, 300001])
+def test_bins_fk_projection(dev, n):
+    """FK projection (create_bin with fk, bmpidx_create.cu:25-34) incl.
+    fk >= dimsz -> INT64_MAX, and create_join / create_select."""
+    rng = np.random.default_rng(n)
+    dim = rng.integers(0, 50, 1000).astype(np.uint16)
+    fk = rng.integers(0, 1100, n).astype(np.uint32)
+    dd, df = dev.upload(dim), dev.upload(fk)
+    ref = abi.ColRef(dd.ptr, df.ptr, 1000, 2)
+    mins = list(range(50)) + [1 << 62]
+    maxs = list(range(1, 51)) + [(1 << 64) - 1]
+    nb = len(mins)
+    outs = (C.c_void_p * nb)()
+    check(dev.lib.mbx_create_bin(dev.ctx, C.byref(ref), n,
+                                 (C.c_uint64 * nb)(*mins),
+                                 (C.c_uint64 * nb)(*maxs), nb, outs, None))
+    got = gpuq.bitmaps_host(dev, list(outs), n)
+    np.testing.assert_array_equal(got, cpuref.create_bin(dim, mins, maxs, fk, 1000))
+    out = C.c_void_p()
+    check(dev.lib.mbx_create_join(dev.ctx, df.ptr, dd.ptr, 2, n, 1000, 7, 31,
+                                  C.byref(out)))
+    got = gpuq.bitmaps_host(dev, [out.value], n)[0]
+    np.testing.assert_array_equal(got, cpuref.create_join(dim, 7, 31, fk, 1000))
+    check(dev.lib.mbx_create_select(dev.ctx, df.ptr, 4, n, 100, 900, C.byref(out)))
+    got = gpuq.bitmaps_host(dev, [out.value], n)[0]
+    np.testing.assert_array_equal(got, cpuref.create_join(fk, 100, 900))
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_random_queries_all_plans(dev, seed):
+    """Random programs (AND/OR/NOT/ANDM/ORM, exact and edge bins), group
+    counts from 1 to >4096 (LDS table and HBM atomics): fused ==
+    materialise == scan == oracle, bit-exact int64."""
+    rng = np.random.default_rng(1000 + seed)
+    n = int(rng.choice([1, 63, 64, 65, 5000, 70001, 300007]))
+    schema = gpuq.upload_schema(dev, datagen.small_schema(n, seed))
+    for _ in range(6):
+        spec = datagen.random_spec(rng, schema, extra_groups=int(rng.integers(0, 3)))
+        want, _ = cpuref.query(spec, schema)
+        cols, allocs = gpuq.plan_cols(dev, schema, spec, rng)
+        for plan in (abi.PLAN_FUSED, abi.PLAN_MATERIALISE, abi.PLAN_SCAN):
+            got = gpuq.run_query(dev, schema, spec, cols, plan)
+            np.testing.assert_array_equal(got, want, err_msg=f"plan {plan} {spec}")
+        gpuq.free_all(dev, allocs)
+    for c in schema.cols.values():
+        c.dev.free()
+
+
+def test_dense_and_empty(dev):
+    """All rows selected (many list rounds per tile) and none selected."""
+    n = 1 << 21
+    rng = np.random.default_rng(7)
+    schema = gpuq.upload_schema(dev, datagen.small_schema(n, 7))
+    for lo, hi in ((0, 256), (300, 400)):
+        spec = S.QuerySpec([S.Pred(S.Src("a8"), lo, hi)], [(abi.ORM, 0, 0)],
+                           [S.Key(S.Src("b32"))], 50, abi.MEAS_COL, S.Src("m32"))
+        want, _ = cpuref.query(spec, schema)
+        cols, allocs = gpuq.plan_cols(dev, schema, spec, rng, binned_frac=0)
+        for plan in (abi.PLAN_FUSED, abi.PLAN_MATERIALISE, abi.PLAN_SCAN):
+            np.testing.assert_array_equal(gpuq.run_query(dev, schema, spec, cols, plan), want)
+        gpuq.free_all(dev, allocs)
+
+
+def test_errors(dev):
+    """Misaligned bitmaps and bad programs fail with a status, not a fault."""
+    n = 1000
+    v = dev.upload(np.arange(n, dtype=np.uint32))
+    prg = abi.Vprg()
+    prg.factsz = n
+    prg.cols[0].middle[0] = v.ptr + 4
+    prg.instrs[0].opcode = abi.ORM
+    op = abi.RowOp()
+    out = dev.alloc(8)
+    assert dev.lib.mbx_prog_grpby(dev.ctx, C.byref(prg), C.byref(op), 1,
+                                  out.ptr, 0) == abi.ERR_ALIGN
+    prg.cols[0].middle[0] = v.ptr
+    prg.instrs[0].dstreg = 5
+    assert dev.lib.mbx_prog_grpby(dev.ctx, C.byref(prg), C.byref(op), 1,
+                                  out.ptr, 0) == abi.ERR_INVALID
+    prg.instrs[0].dstreg = 0
+    prg.factsz = 1 << 31
+    assert dev.lib.mbx_prog_grpby(dev.ctx, C.byref(prg), C.byref(op), 1,
+                                  out.ptr, 0) == abi.ERR_CAPACITY
+    v.free()
+    out.free()
+
+
+@pytest.fixture(scope="module")
+def config1(dev):
+    n = configs.C1_ROWS
+    sch = configs.config1_device(dev, n)
+    host = configs.config1_host(n)
+    for k, a in host.items():
+        sch[k].host = a
+    yield sch
+    for c in sch.cols.values():
+        c.dev.free()
+
+
+def test_config1_generator_matches_host(dev, config1):
+    for k in ("p0", "p1", "p2", "m"):
+        np.testing.assert_array_equal(config1[k].dev.host(), config1[k].host)
+
+
+def test_config1_full(dev, config1):
+    """Config 1 at full size: 40-bin bitmaps byte-identical to the oracle,
+    the fused query (aligned and misaligned) bit-exact vs the raw-column
+    oracle, invariants sum(groups) and count."""
+    b = list(configs.C1_BOUNDS)
+    bins = {}
+    for c in configs.C1_COLS:
+        bins[c] = gpuq.create_bin(dev, config1, S.Src(c), b[:-1], b[1:])
+    got = gpuq.bitmaps_host(dev, bins["p1"], config1.nrows)
+    np.testing.assert_array_equal(got, cpuref.create_bin(config1["p1"].host, b[:-1], b[1:]))
+    for mis in (False, True):
+        spec = configs.config1_spec(mis)
+        want, nsel = cpuref.query(spec, config1)
+        cols = []
+        for p in spec.preds:
+            mid, l, r = S.from_bin(b, p.lo, p.hi)
+            ptrs = bins[p.src.col]
+            cols.append(S.BitmapCol([ptrs[i] for i in mid],
+                                    None if l is None else ptrs[l],
+                                    None if r is None else ptrs[r]))
+        for plan in (abi.PLAN_FUSED, abi.PLAN_MATERIALISE, abi.PLAN_SCAN):
+            got = gpuq.run_query(dev, config1, spec, cols, plan)
+            np.testing.assert_array_equal(got, want, err_msg=f"{spec.name} plan {plan}")
+        # COUNT version == number of selected rows
+        cnt = S.QuerySpec(spec.preds, spec.instrs, [], 1, abi.MEAS_COUNT)
+        assert gpuq.run_query(dev, config1, cnt, cols)[0] == nsel
+        assert 0.005 < nsel / config1.nrows < 0.02
+    for c in configs.C1_COLS:
+        check(dev.lib.mbx_free(dev.ctx, bins[c][0]))

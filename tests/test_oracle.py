@@ -304,11 +304,24 @@ def test_mlp_grad_finite_difference():
     yy = h @ W2.T.astype(np.float64) + b2
     np.testing.assert_allclose(y, yy, rtol=1e-5, atol=1e-5)
     assert abs(loss - np.mean((yy - t) ** 2)) < 1e-9 * max(1, loss)
+    # analytic backward in numpy
+    pre = x.astype(np.float64) @ W1.T.astype(np.float64) + b1
+    dy = 2 * (yy - t) / (n * dout)
+    dpre = (dy @ W2.astype(np.float64)) * (pre > 0)
+    for got, want in zip(g, (dpre.T @ x, dpre.sum(0), dy.T @ h, dy.sum(0))):
+        np.testing.assert_allclose(got, want, rtol=1e-4, atol=1e-7)
+    # central differences, away from ReLU kinks (a perturbation of eps can
+    # move a pre-activation across 0 only if it is within eps*|x|_1 of it)
     params = [W1, b1, W2, b2]
     eps = 1e-3
+    margin = eps * (np.abs(x).sum(1).max() + 1) * 4
+    safe_h = np.all(np.abs(pre) > margin, axis=0)
     for pi, P in enumerate(params):
         flat = P.reshape(-1)
-        for idx in rng.choice(flat.size, size=min(12, flat.size), replace=False):
+        cand = np.arange(flat.size)
+        if pi in (0, 1):
+            cand = cand[safe_h[cand // (din if pi == 0 else 1)]]
+        for idx in rng.choice(cand, size=min(12, cand.size), replace=False):
             old = flat[idx]
             flat[idx] = old + eps
             lp, _, _ = cpuref.mlp_loss_grad(x, t, *params)
