@@ -70,7 +70,14 @@ class Tune(C.Structure):
                 ("lds_groups", C.c_uint32), ("pad_", C.c_uint32)]
 
 
-ABI_STRUCTS = [ColRef, Col, Instr, Vprg, Pred, KeyPart, RowOp, Query, Tune]
+class Plan(C.Structure):
+    _fields_ = [("order", C.c_uint8 * MAXCOLS), ("norder", C.c_uint8),
+                ("reorderable", C.c_uint8), ("pad_", C.c_uint16),
+                ("flags", C.c_uint32)]
+
+
+NFEAT = 16
+ABI_STRUCTS = [ColRef, Col, Instr, Vprg, Pred, KeyPart, RowOp, Query, Tune, Plan]
 
 _vp, _sz, _u32, _u64, _i32 = C.c_void_p, C.c_size_t, C.c_uint32, C.c_uint64, C.c_int
 _P = C.POINTER
@@ -125,6 +132,28 @@ PROTOTYPES = {
     "mbx_timer_destroy": (_i32, [_vp]),
     "mbx_copy_kernel": (_i32, [_vp, _vp, _vp, _sz]),
     "mbx_abi_sizes": (_i32, [_P(_sz), _i32]),
+    # plans and the cost-model MLP
+    "mbx_col_density": (_i32, [_vp, _P(Vprg), _P(C.c_double)]),
+    "mbx_plan_describe": (_i32, [_P(Vprg), _P(Plan)]),
+    "mbx_plan_apply": (_i32, [_P(Vprg), _P(Plan), _P(Vprg)]),
+    "mbx_plan_features": (_i32, [_P(Vprg), _P(C.c_double), _u32, _P(Plan),
+                                 _P(C.c_float)]),
+    "mbx_plan_enumerate": (_i32, [_P(Vprg), _P(Plan), _i32, _P(_i32)]),
+    "mbx_plan_heuristic": (_i32, [_P(Vprg), _P(C.c_double), _P(Plan)]),
+    "mbx_plan_choose": (_i32, [_vp, _vp, _P(Vprg), _P(C.c_double), _u32,
+                               _P(Plan), _P(C.c_float)]),
+    "mbx_prog_grpby_plan": (_i32, [_vp, _P(Vprg), _P(RowOp), _u32, _vp,
+                                   _P(Plan)]),
+    "mbx_mlp_create": (_i32, [_vp, _i32, _i32, _i32, _u64, _P(_vp)]),
+    "mbx_mlp_destroy": (_i32, [_vp, _vp]),
+    "mbx_mlp_nparams": (_i32, [_vp]),
+    "mbx_mlp_set_params": (_i32, [_vp, _vp, _vp]),
+    "mbx_mlp_get_params": (_i32, [_vp, _vp, _vp]),
+    "mbx_mlp_forward": (_i32, [_vp, _vp, _vp, _i32, _vp]),
+    "mbx_mlp_loss_grad": (_i32, [_vp, _vp, _vp, _vp, _i32, _P(C.c_double),
+                                 _vp]),
+    "mbx_mlp_train": (_i32, [_vp, _vp, _vp, _vp, _i32, _i32, C.c_float,
+                             _P(C.c_double)]),
     # host-only planning helpers (include/mbx_bmpidx.hpp)
     "mbx_col_from_bin": (_i32, [_sz, _P(_u64), _P(_u64), _u64, _u64, _P(_u64)]),
     "mbx_col_from_bin2": (_i32, [_sz, _P(_u64), _P(_u64), _sz, _P(_u64),

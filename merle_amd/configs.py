@@ -217,3 +217,102 @@ def config2_device(dev, n=C2_ROWS, seed=SEED):
         sch.add(name, None, a)
     dev.sync()
     return sch
+
+
+# ---------------------------------------------------------------- config 3
+def zipf_pmf(n=C1_DOMAIN, s=1.0):
+    w = 1.0 / np.arange(1, n + 1, dtype=np.float64) ** s
+    return w / w.sum()
+
+
+def _range_density(pmf, lo, hi):
+    return float(pmf[lo:hi].sum())
+
+
+def config3_queries(nq=256, seed=SEED + 3, allow_edges=True):
+    """nq random queries over the config-1 tables: AND, OR and ANDNOT
+    programs over 2-4 range predicates on p0..p2 (bin-aligned, or with
+    partly covered edge bins that need a candidate check), 1..10,000 groups
+    from bucketed keys, SUM(m) or COUNT.  Queries are drawn so that their
+    estimated selectivity (independent columns, exact Zipf pmf) is spread
+    log-uniformly over [1e-6, 1e-1]."""
+    rng = np.random.default_rng(seed)
+    pmf = zipf_pmf()
+    bounds = C1_BOUNDS
+    nb = len(bounds) - 1
+    key_choices = [
+        (lambda c: Key(Src(c), abi.KEY_DIV, 25, 0), 40),
+        (lambda c: Key(Src(c), abi.KEY_DIV, 100, 0), 10),
+        (lambda c: Key(Src(c), abi.KEY_BUCKET, 1, 0, 1, (5, 25, 100, 300)), 5),
+        (lambda c: Key(Src("m"), abi.KEY_DIV, 10000, 0), 11),
+    ]
+    pool = []
+    while len(pool) < nq * 40:
+        npred = int(rng.integers(2, 5))
+        preds, dens = [], []
+        for _ in range(npred):
+            col = C1_COLS[rng.integers(3)]
+            j = int(rng.integers(nb))
+            w = int(rng.integers(1, 5))
+            lo, hi = bounds[j], bounds[min(nb, j + w)]
+            if allow_edges and rng.random() < 0.3:   # misaligned range
+                lo = min(hi - 1, lo + int(rng.integers(1, 20)))
+            preds.append(Pred(Src(col), lo, hi))
+            dens.append(_range_density(pmf, lo, hi))
+        shape = rng.integers(3)
+        if shape == 0:        # AND chain
+            ins = [(abi.ORM, 0, 0)] + [(abi.ANDM, 0, i) for i in range(1, npred)]
+            sel = float(np.prod(dens))
+        elif shape == 1:      # (c0 OR c1) AND rest
+            ins = [(abi.ORM, 0, 0), (abi.ORM, 0, 1)] + \
+                  [(abi.ANDM, 0, i) for i in range(2, npred)]
+            sel = (dens[0] + dens[1] - dens[0] * dens[1]) * float(np.prod(dens[2:]))
+        else:                 # c0 AND NOT c1 AND rest
+            ins = [(abi.ORM, 0, 0), (abi.ORM, 1, 1), (abi.NOT, 1, 1),
+                   (abi.AND, 0, 1)] + [(abi.ANDM, 0, i) for i in range(2, npred)]
+            sel = dens[0] * (1 - dens[1]) * float(np.prod(dens[2:]))
+        if not (1e-6 <= sel <= 1e-1):
+            continue
+        keys, total, stride = [], 1, 1
+        for _ in range(int(rng.integers(0, 4))):
+            mk, card = key_choices[rng.integers(len(key_choices))]
+            if total * card > 10_000:
+                break
+            k = mk(C1_COLS[rng.integers(3)])
+            keys.append(Key(k.src, k.kind, k.div, k.base, stride, k.bounds))
+            stride *= card
+            total *= card
+        meas = abi.MEAS_COL if rng.random() < 0.8 else abi.MEAS_COUNT
+        pool.append((np.log10(sel), QuerySpec(
+            preds, ins, keys, total, meas, Src("m") if meas else None, None,
+            f"b{len(pool)}")))
+    # stratify: equal counts per log-selectivity band
+    pool.sort(key=lambda t: t[0])
+    edges = np.linspace(-6, -1, nq + 1)
+    out, i = [], 0
+    for b in range(nq):
+        while i < len(pool) and pool[i][0] < edges[b]:
+            i += 1
+        if i < len(pool):
+            out.append(pool[i][1])
+            i += 1
+    while len(out) < nq:
+        out.append(pool[int(rng.integers(len(pool)))][1])
+    return out
+
+
+def bin_cols(spec, bin_ptrs, bounds=C1_BOUNDS):
+    """BitmapCols of a spec over per-column bin slabs {col: [ptrs]} through
+    col::from_bin (edges when a range is misaligned)."""
+    from .spec import BitmapCol, from_bin
+    cols = []
+    for p in spec.preds:
+        r = from_bin(list(bounds), p.lo, p.hi)
+        if r is None:
+            raise ValueError(f"range [{p.lo},{p.hi}) not coverable by bins")
+        mid, l, rr = r
+        ptrs = bin_ptrs[p.src.col]
+        cols.append(BitmapCol([ptrs[i] for i in mid],
+                              None if l is None else ptrs[l],
+                              None if rr is None else ptrs[rr]))
+    return cols

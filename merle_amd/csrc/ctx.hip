@@ -244,7 +244,8 @@ extern "C" {
 int mbx_abi_sizes(size_t *out, int n) {
   const size_t s[] = {sizeof(mbx_colref), sizeof(mbx_col),   sizeof(mbx_instr),
                       sizeof(mbx_vprg),   sizeof(mbx_pred),  sizeof(mbx_keypart),
-                      sizeof(mbx_rowop),  sizeof(mbx_query), sizeof(mbx_tune)};
+                      sizeof(mbx_rowop),  sizeof(mbx_query), sizeof(mbx_tune),
+                      sizeof(mbx_plan)};
   const int m = (int)(sizeof(s) / sizeof(s[0]));
   for (int i = 0; i < n && i < m; ++i) out[i] = s[i];
   return m;

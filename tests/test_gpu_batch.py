@@ -1,0 +1,127 @@
+"""Batched persistent launch (hipGraph) vs the plain one-query-per-launch
+path, on config-3-style queries over config-1-shaped tables."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from merle_amd import abi, configs, spec as S
+from merle_amd.host import check
+from oracle import cpuref
+
+import gpuq
+
+pytestmark = pytest.mark.gpu
+
+N = 2_000_003   # smaller than config 1, not a multiple of 64
+
+
+@pytest.fixture(scope="module")
+def tables(dev):
+    host = configs.config1_host(N, seed=4242)
+    sch = S.Schema(N)
+    for k, v in host.items():
+        sch.add(k, v, dev.upload(v))
+    b = list(configs.C1_BOUNDS)
+    bins = {c: gpuq.create_bin(dev, sch, S.Src(c), b[:-1], b[1:])
+            for c in configs.C1_COLS}
+    yield sch, bins
+    for c in configs.C1_COLS:
+        check(dev.lib.mbx_free(dev.ctx, bins[c][0]))
+    for c in sch.cols.values():
+        c.dev.free()
+
+
+def make_queries(dev, sch, bins, specs):
+    qs = (abi.Query * len(specs))()
+    outs = []
+    for i, s in enumerate(specs):
+        qs[i].prg = S.to_vprg(configs.bin_cols(s, bins), N, s.instrs)
+        qs[i].op = S.to_rowop(s, sch)
+        qs[i].nr_grp = s.nr_grp
+        out = dev.alloc(8 * s.nr_grp, np.int64, s.nr_grp)
+        qs[i].d_out = out.ptr
+        outs.append(out)
+    return qs, outs
+
+
+def plain_results(dev, qs, specs):
+    res = []
+    for i, s in enumerate(specs):
+        out = dev.alloc(8 * s.nr_grp, np.int64, s.nr_grp)
+        check(dev.lib.mbx_prog_grpby(dev.ctx, C.byref(qs[i].prg), C.byref(qs[i].op),
+                                     s.nr_grp, out.ptr, abi.PLAN_FUSED))
+        dev.sync()
+        res.append(out.host())
+        out.free()
+    return res
+
+
+def test_batch_equals_plain_and_oracle(dev, tables):
+    sch, bins = tables
+    specs = configs.config3_queries(64, seed=11)
+    qs, outs = make_queries(dev, sch, bins, specs)
+    plain = plain_results(dev, qs, specs)
+    for i in range(0, len(specs), 8):           # oracle on a subset
+        want, _ = cpuref.query(specs[i], sch)
+        np.testing.assert_array_equal(plain[i], want, err_msg=specs[i].name)
+    b = C.c_void_p()
+    check(dev.lib.mbx_batch_create(dev.ctx, qs, len(specs), None, C.byref(b)))
+    # graph replays with eager work in between; every replay == plain
+    for rep in range(3):
+        check(dev.lib.mbx_batch_run(dev.ctx, b))
+        dev.sync()
+        for i, o in enumerate(outs):
+            np.testing.assert_array_equal(o.host(), plain[i],
+                                          err_msg=f"replay {rep} {specs[i].name}")
+        gpuq.run_query(dev, sch, specs[0], configs.bin_cols(specs[0], bins))
+    check(dev.lib.mbx_batch_run_eager(dev.ctx, b))
+    dev.sync()
+    for i, o in enumerate(outs):
+        np.testing.assert_array_equal(o.host(), plain[i])
+    check(dev.lib.mbx_batch_destroy(dev.ctx, b))
+    for o in outs:
+        o.free()
+
+
+@pytest.mark.parametrize("tune", [(512, 2, 1024), (8192, 8, 4096), (1024, 1, 0)])
+def test_batch_launch_shapes_agree(dev, tables, tune):
+    """Results do not depend on tile size, grid or LDS group capacity."""
+    sch, bins = tables
+    specs = configs.config3_queries(24, seed=12)
+    qs, outs = make_queries(dev, sch, bins, specs)
+    plain = plain_results(dev, qs, specs)
+    t = abi.Tune(*tune)
+    b = C.c_void_p()
+    check(dev.lib.mbx_batch_create(dev.ctx, qs, len(specs), C.byref(t), C.byref(b)))
+    check(dev.lib.mbx_batch_run(dev.ctx, b))
+    dev.sync()
+    for i, o in enumerate(outs):
+        np.testing.assert_array_equal(o.host(), plain[i])
+    check(dev.lib.mbx_batch_destroy(dev.ctx, b))
+    for o in outs:
+        o.free()
+
+
+def test_autotune(dev, tables):
+    sch, bins = tables
+    specs = configs.config3_queries(16, seed=13)
+    qs, outs = make_queries(dev, sch, bins, specs)
+    best = abi.Tune()
+    ms = C.c_float()
+    check(dev.lib.mbx_autotune(dev.ctx, qs, len(specs), C.byref(best), C.byref(ms)))
+    assert best.tile_words % 512 == 0 and 1 <= best.wg_per_cu <= 16
+    assert ms.value > 0
+    for o in outs:
+        o.free()
+
+
+def test_bad_tune_rejected(dev, tables):
+    sch, bins = tables
+    specs = configs.config3_queries(2, seed=14)
+    qs, outs = make_queries(dev, sch, bins, specs)
+    b = C.c_void_p()
+    t = abi.Tune(100, 4, 4096)
+    assert dev.lib.mbx_batch_create(dev.ctx, qs, 2, C.byref(t), C.byref(b)) == abi.ERR_INVALID
+    for o in outs:
+        o.free()

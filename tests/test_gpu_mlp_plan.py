@@ -1,0 +1,119 @@
+"""The cost-model MLP (HIP forward/backward/Adam) against the CPU oracle, and
+plan identity: every plan of a query gives the same int64 sums."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from merle_amd import abi, configs, spec as S
+from merle_amd.host import check
+from oracle import cpuref
+
+import gpuq
+
+pytestmark = pytest.mark.gpu
+
+DIN, DH, DOUT = 16, 64, 1
+
+
+def _mlp(dev, seed=1):
+    m = C.c_void_p()
+    check(dev.lib.mbx_mlp_create(dev.ctx, DIN, DH, DOUT, seed, C.byref(m)))
+    return m
+
+
+def _split(w):
+    a = DH * DIN
+    return (w[:a].reshape(DH, DIN), w[a:a + DH],
+            w[a + DH:a + DH + DOUT * DH].reshape(DOUT, DH), w[a + DH + DOUT * DH:])
+
+
+def test_mlp_forward_and_grad_vs_oracle(dev):
+    """fp32 kernels vs the oracle's double-accumulated fp32 MLP:
+    outputs and gradients within rtol 1e-4 (atol 1e-6)."""
+    rng = np.random.default_rng(5)
+    n = 1000
+    x = rng.normal(size=(n, DIN)).astype(np.float32)
+    t = rng.normal(size=(n, DOUT)).astype(np.float32)
+    m = _mlp(dev)
+    npar = dev.lib.mbx_mlp_nparams(m)
+    w = (rng.normal(size=npar) * 0.2).astype(np.float32)
+    check(dev.lib.mbx_mlp_set_params(dev.ctx, m, w.ctypes.data))
+    dx, dt = dev.upload(x), dev.upload(t)
+    dy = dev.alloc(4 * n * DOUT, np.float32, n * DOUT)
+    check(dev.lib.mbx_mlp_forward(dev.ctx, m, dx.ptr, n, dy.ptr))
+    dev.sync()
+    loss, g_ref, y_ref = cpuref.mlp_loss_grad(x, t, *_split(w))
+    np.testing.assert_allclose(dy.host().reshape(n, DOUT), y_ref, rtol=1e-4, atol=1e-5)
+    dl = C.c_double()
+    g = np.zeros(npar, np.float32)
+    check(dev.lib.mbx_mlp_loss_grad(dev.ctx, m, dx.ptr, dt.ptr, n, C.byref(dl),
+                                    g.ctypes.data))
+    assert abs(dl.value - loss) < 1e-5 * max(1.0, loss)
+    for got, want in zip(_split(g), g_ref):
+        np.testing.assert_allclose(got.reshape(want.shape), want, rtol=1e-4, atol=1e-6)
+    check(dev.lib.mbx_mlp_destroy(dev.ctx, m))
+
+
+def test_mlp_training_reduces_loss(dev):
+    """Adam on a synthetic regression target reaches a small loss, and two
+    identical runs give identical weights (deterministic reductions)."""
+    rng = np.random.default_rng(6)
+    n = 4000
+    x = rng.uniform(-1, 1, size=(n, DIN)).astype(np.float32)
+    t = (np.sin(x[:, :1] * 2) + x[:, 1:2] * x[:, 2:3]).astype(np.float32)
+    dx, dt = dev.upload(x), dev.upload(t)
+    ws = []
+    for _ in range(2):
+        m = _mlp(dev, seed=3)
+        l0 = C.c_double()
+        check(dev.lib.mbx_mlp_loss_grad(dev.ctx, m, dx.ptr, dt.ptr, n, C.byref(l0), None))
+        l1 = C.c_double()
+        check(dev.lib.mbx_mlp_train(dev.ctx, m, dx.ptr, dt.ptr, n, 400,
+                                    C.c_float(3e-3), C.byref(l1)))
+        assert l1.value < 0.2 * l0.value, (l0.value, l1.value)
+        w = np.zeros(dev.lib.mbx_mlp_nparams(m), np.float32)
+        check(dev.lib.mbx_mlp_get_params(dev.ctx, m, w.ctypes.data))
+        ws.append(w)
+        check(dev.lib.mbx_mlp_destroy(dev.ctx, m))
+    np.testing.assert_array_equal(ws[0], ws[1])
+
+
+def test_every_plan_same_result(dev):
+    """All evaluation orders x {fused, materialise} of conjunctive queries,
+    the heuristic plan and the plan without a chooser: identical sums."""
+    n = 1_000_003
+    host = configs.config1_host(n, seed=99)
+    sch = S.Schema(n)
+    for k, v in host.items():
+        sch.add(k, v, dev.upload(v))
+    b = list(configs.C1_BOUNDS)
+    bins = {c: gpuq.create_bin(dev, sch, S.Src(c), b[:-1], b[1:])
+            for c in configs.C1_COLS}
+    specs = [q for q in configs.config3_queries(40, seed=21)
+             if all(i[0] in (abi.ORM, abi.ANDM) and i[1] == 0 for i in q.instrs)][:6]
+    assert specs
+    for s in specs:
+        prg = S.to_vprg(configs.bin_cols(s, bins), n, s.instrs)
+        op = S.to_rowop(s, sch)
+        want, _ = cpuref.query(s, sch)
+        dens = (C.c_double * 4)()
+        check(dev.lib.mbx_col_density(dev.ctx, C.byref(prg), dens))
+        plans = (abi.Plan * 48)()
+        cnt = C.c_int()
+        check(dev.lib.mbx_plan_enumerate(C.byref(prg), plans, 48, C.byref(cnt)))
+        heur = abi.Plan()
+        check(dev.lib.mbx_plan_heuristic(C.byref(prg), dens, C.byref(heur)))
+        d = sorted(dens[c] for c in range(len(s.preds)))
+        assert [dens[heur.order[k]] for k in range(heur.norder)] == d
+        out = dev.alloc(8 * s.nr_grp, np.int64, s.nr_grp)
+        for p in list(plans[:cnt.value]) + [heur]:
+            check(dev.lib.mbx_prog_grpby_plan(dev.ctx, C.byref(prg), C.byref(op),
+                                              s.nr_grp, out.ptr, C.byref(p)))
+            dev.sync()
+            np.testing.assert_array_equal(out.host(), want)
+        out.free()
+    for c in configs.C1_COLS:
+        check(dev.lib.mbx_free(dev.ctx, bins[c][0]))
+    for c in sch.cols.values():
+        c.dev.free()
