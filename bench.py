@@ -33,26 +33,47 @@ def log(*a):
 
 def build_index(dev, sch, attrs):
     """One bitmap per attribute value, projected through the fact FK
-    (create_bin with fk).  Returns {attr: [device ptrs]} and timing."""
+    (create_bin with fk).  Slabs are allocated first, so the timed region
+    holds only the creation kernels (the reference times cudaMalloc too,
+    ssb/bmpidx-create.cu:64-68).  Returns {attr: [device ptrs]}, ms, bytes."""
     from merle_amd import spec as S
     from merle_amd.configs import C2_ATTRS
-    from merle_amd.host import Timer, check
+    from merle_amd.host import Timer, check, stride_words
     idx, ms, nbytes = {}, 0.0, 0
+    sw = stride_words(sch.nrows)
     t = Timer(dev)
     for a in attrs:
         card = C2_ATTRS[a][2]
         ref = S.colref(sch, S.Src(a, C2_ATTRS[a][1]))
-        outs = (C.c_void_p * card)()
+        slab = dev.alloc(sw * 4 * card)
+        dev.sync()
         t.start()
-        check(dev.lib.mbx_create_bin(dev.ctx, C.byref(ref), sch.nrows,
-                                     (C.c_uint64 * card)(*range(card)),
-                                     (C.c_uint64 * card)(*range(1, card + 1)),
-                                     card, outs, None))
+        check(dev.lib.mbx_create_bin_into(dev.ctx, C.byref(ref), sch.nrows,
+                                          (C.c_uint64 * card)(*range(card)),
+                                          (C.c_uint64 * card)(*range(1, card + 1)),
+                                          card, slab.ptr, sw))
         ms += t.stop()
         nbytes += sch.nrows * 4 + card * sch.nrows // 8
-        idx[a] = list(outs)
+        idx[a] = [slab.ptr + j * sw * 4 for j in range(card)]
     t.close()
     return idx, ms, nbytes
+
+
+def copy_calibration(nbytes, reps):
+    """Plain device copy of nbytes, reps times (for byte-counter scaling)."""
+    from merle_amd.host import Device, Timer, check
+    dev = Device(0)
+    a, b = dev.alloc(nbytes), dev.alloc(nbytes)
+    check(dev.lib.mbx_fill_u32(dev.ctx, a.ptr, 7, nbytes // 4))
+    t = Timer(dev)
+    t.start()
+    for _ in range(reps):
+        check(dev.lib.mbx_copy_kernel(dev.ctx, b.ptr, a.ptr, nbytes))
+    ms = t.stop()
+    print(json.dumps({"copy_bytes": nbytes, "reps": reps,
+                      "GB_per_s": round(2 * nbytes * reps / ms / 1e6, 1)}))
+    dev.close()
+    return 0
 
 
 def query_cols(spec, idx):
@@ -115,7 +136,11 @@ def main():
     ap.add_argument("--rows", type=int, default=None,
                     help="fact rows (default: config 2's 120M)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="run only a plain device copy of this many bytes")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration, max(1, args.steps))
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the "
               "path shards by row range with one int64 group reduction at the "
@@ -135,12 +160,14 @@ def main():
     nbm = sum(len(v) for v in idx.values())
     log(f"index: {nbm} bitmaps in {cms:.1f} ms ({cbytes / cms / 1e6:.0f} GB/s)")
 
+    # outputs of the 12 queries in one buffer, zeroed by one fill per step
+    offs = np.cumsum([0] + [q.nr_grp for q in specs])
+    outbuf = dev.alloc(8 * int(offs[-1]), np.int64, int(offs[-1]))
     qs = []
-    for s in specs:
+    for i, s in enumerate(specs):
         prg = S.to_vprg(query_cols(s, idx), n, s.instrs)
         op = S.to_rowop(s, sch)
-        out = dev.alloc(8 * s.nr_grp, np.int64, s.nr_grp)
-        qs.append((s, prg, op, out))
+        qs.append((s, prg, op, outbuf.ptr + 8 * int(offs[i])))
 
     # algorithmic bytes of each query (device accounting kernel)
     balg, nsel = [], []
@@ -152,14 +179,17 @@ def main():
         nsel.append(k.value)
     log("selectivity:", ", ".join(f"{s.name}={k / n:.1e}" for s, k in zip(specs, nsel)))
 
+    flags = abi.PLAN_FUSED | abi.ACCUMULATE
+
     def step(timers=None):
+        check(dev.lib.mbx_fill_u32(dev.ctx, outbuf.ptr, 0, 2 * int(offs[-1])))
         for i, (s, prg, op, out) in enumerate(qs):
             if timers:
                 timers[i].start()
             check(dev.lib.mbx_prog_grpby(dev.ctx, C.byref(prg), C.byref(op),
-                                         s.nr_grp, out.ptr, abi.PLAN_FUSED))
+                                         s.nr_grp, out, flags))
             if timers:
-                per_q[i].append(timers[i].stop())
+                timers[i].mark_stop()
 
     for _ in range(args.warmup):
         step()
@@ -171,11 +201,15 @@ def main():
     for _ in range(args.steps):
         step()
     total_ms = whole.stop()
-    # per-launch durations, on the same stream, in a second timed pass
+    # per-launch durations of the fused kernel: event pairs around each
+    # launch on the kernels' stream, enqueued back to back (no host sync
+    # inside a step), read after the step
     timers = [Timer(dev) for _ in qs]
     per_q = [[] for _ in qs]
-    for _ in range(max(3, args.steps // 2)):
+    for _ in range(max(3, args.steps)):
         step(timers)
+        for i, t in enumerate(timers):
+            per_q[i].append(t.elapsed())
     qms = [float(np.median(v)) for v in per_q]
     ms_per_step = total_ms / args.steps
     rows_per_s = n * len(qs) / (ms_per_step / 1e3)
@@ -221,12 +255,13 @@ def main():
         result["cpu_baseline"] = cb
         # parity on the sample: the same queries restricted to R rows
         ok = True
-        for (s, prg, op, out), want in zip(qs, cres):
+        for (s, prg, op, out), want, o in zip(qs, cres, offs):
             prg.factsz = R
             check(dev.lib.mbx_prog_grpby(dev.ctx, C.byref(prg), C.byref(op),
-                                         s.nr_grp, out.ptr, abi.PLAN_FUSED))
+                                         s.nr_grp, out, abi.PLAN_FUSED))
             dev.sync()
-            ok &= bool(np.array_equal(out.host(), want))
+            got = outbuf.host()[int(o):int(o) + s.nr_grp]
+            ok &= bool(np.array_equal(got, want))
             prg.factsz = n
         result["parity_on_cpu_sample"] = ok
         log(f"cpu {cb['value']} G rows/s on {cb['cores']} threads; parity {ok}")

@@ -238,11 +238,44 @@ int mbx_autotune(mbx_ctx *ctx, const mbx_query *qs, uint32_t nq,
                  mbx_tune *best, float *best_ms);
 int mbx_default_tune(mbx_tune *t);
 
+
+/* ------------------------------------------------------------- serving */
+/* Replays a request trace (arrival times in seconds, sorted) against
+ * resident data: memory-based admission (each request's int64 output is
+ * allocated from the pool when it fits mem_budget), continuous batching of
+ * admitted requests onto nstreams slots, up to max_batch requests per
+ * persistent batched launch; plain = 1 runs the one-at-a-time baseline (one
+ * fused launch per request, one stream).  reqs[i].d_out is ignored: results
+ * land in h_results (requests' outputs concatenated in request order);
+ * latency_s[i] = result on the host - arrival, or -1 if refused. */
+typedef struct mbx_serve_cfg {
+  uint32_t nstreams;
+  uint32_t max_batch;
+  uint64_t mem_budget;      /* bytes; 0 = unbounded */
+  uint32_t plain;
+  uint32_t pad_;
+  mbx_tune tune;            /* tile_words == 0: default shape */
+} mbx_serve_cfg;
+typedef struct mbx_serve_stats {
+  uint32_t completed, rejected;
+  double wall_s, p50_s, p99_s, mean_s, max_s;
+  uint32_t batches, admission_waits;
+  double mean_batch;
+  uint64_t peak_mem;
+} mbx_serve_stats;
+int mbx_serve_replay(mbx_ctx *ctx, const mbx_query *reqs, uint32_t nq,
+                     const double *arrival_s, const mbx_serve_cfg *cfg,
+                     int64_t *h_results, double *latency_s,
+                     mbx_serve_stats *stats);
+
 /* --------------------------------------------------------------- timing */
 typedef struct mbx_timer mbx_timer;
 int mbx_timer_create(mbx_ctx *ctx, mbx_timer **t);
 int mbx_timer_start(mbx_ctx *ctx, mbx_timer *t);
 int mbx_timer_stop(mbx_ctx *ctx, mbx_timer *t, float *ms);  /* syncs */
+/* Records the end event without waiting; mbx_timer_elapsed waits for it. */
+int mbx_timer_record_stop(mbx_ctx *ctx, mbx_timer *t);
+int mbx_timer_elapsed(mbx_timer *t, float *ms);
 int mbx_timer_destroy(mbx_timer *t);
 /* Plain device copy of `bytes` (for byte-counter calibration). */
 int mbx_copy_kernel(mbx_ctx *ctx, void *dst, const void *src, size_t bytes);

@@ -76,8 +76,24 @@ class Plan(C.Structure):
                 ("flags", C.c_uint32)]
 
 
+class ServeCfg(C.Structure):
+    _fields_ = [("nstreams", C.c_uint32), ("max_batch", C.c_uint32),
+                ("mem_budget", C.c_uint64), ("plain", C.c_uint32),
+                ("pad_", C.c_uint32), ("tune", Tune)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("completed", C.c_uint32), ("rejected", C.c_uint32),
+                ("wall_s", C.c_double), ("p50_s", C.c_double),
+                ("p99_s", C.c_double), ("mean_s", C.c_double),
+                ("max_s", C.c_double), ("batches", C.c_uint32),
+                ("admission_waits", C.c_uint32), ("mean_batch", C.c_double),
+                ("peak_mem", C.c_uint64)]
+
+
 NFEAT = 16
-ABI_STRUCTS = [ColRef, Col, Instr, Vprg, Pred, KeyPart, RowOp, Query, Tune, Plan]
+ABI_STRUCTS = [ColRef, Col, Instr, Vprg, Pred, KeyPart, RowOp, Query, Tune, Plan,
+               ServeCfg, ServeStats]
 
 _vp, _sz, _u32, _u64, _i32 = C.c_void_p, C.c_size_t, C.c_uint32, C.c_uint64, C.c_int
 _P = C.POINTER
@@ -126,10 +142,15 @@ PROTOTYPES = {
     "mbx_batch_destroy": (_i32, [_vp, _vp]),
     "mbx_autotune": (_i32, [_vp, _P(Query), _u32, _P(Tune), _P(C.c_float)]),
     "mbx_default_tune": (_i32, [_P(Tune)]),
+    "mbx_serve_replay": (_i32, [_vp, _P(Query), _u32, _P(C.c_double),
+                                _P(ServeCfg), _vp, _P(C.c_double),
+                                _P(ServeStats)]),
     "mbx_timer_create": (_i32, [_vp, _P(_vp)]),
     "mbx_timer_start": (_i32, [_vp, _vp]),
     "mbx_timer_stop": (_i32, [_vp, _vp, _P(C.c_float)]),
     "mbx_timer_destroy": (_i32, [_vp]),
+    "mbx_timer_record_stop": (_i32, [_vp, _vp]),
+    "mbx_timer_elapsed": (_i32, [_vp, _P(C.c_float)]),
     "mbx_copy_kernel": (_i32, [_vp, _vp, _vp, _sz]),
     "mbx_abi_sizes": (_i32, [_P(_sz), _i32]),
     # plans and the cost-model MLP

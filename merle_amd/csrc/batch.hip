@@ -12,21 +12,11 @@
 #include <cstring>
 #include <vector>
 
+#include "batch.hpp"
 #include "devquery.hpp"
 #include "query_launch.hpp"
 
 namespace mbx {
-
-static constexpr int kBWPL = 2;                    // words per thread per tile
-static constexpr uint32_t kTileWords = 256 * kBWPL;
-
-struct BatchQ {
-  DevPrg prg;
-  DevOp op;
-  unsigned long long *out;
-  uint32_t nr_grp, lds, chk, ntiles;
-  uint32_t item0, nitems, tiles_per_item, pad;
-};
 
 struct BatchArgs {
   const BatchQ *qs;
@@ -105,20 +95,62 @@ struct mbx_batch {
 
 namespace mbx {
 
-static int enqueue_batch(mbx_ctx *ctx, mbx_batch *b) {
-  hipLaunchKernelGGL(zero_outputs_kernel, dim3(std::min(b->nq, 1024u)),
-                     dim3(256), 0, ctx->stream, b->d_qs, b->nq);
+int batch_launch(hipStream_t s, const BatchQ *d_qs, uint32_t nq,
+                 uint32_t total_items, uint32_t *d_counter, size_t lds_bytes,
+                 unsigned grid) {
+  hipLaunchKernelGGL(zero_outputs_kernel, dim3(std::min(nq, 1024u)),
+                     dim3(256), 0, s, d_qs, nq);
   MBX_HIP(hipGetLastError());
-  int st = launch_fill_u32(ctx->stream, b->d_counter, 0, 1);
+  int st = launch_fill_u32(s, d_counter, 0, 1);
   if (st) return st;
   BatchArgs a;
-  a.qs = b->d_qs;
-  a.nq = b->nq;
-  a.total_items = b->total_items;
-  a.counter = b->d_counter;
-  hipLaunchKernelGGL(batch_kernel, dim3(b->grid), dim3(256), b->lds_bytes,
-                     ctx->stream, a);
+  a.qs = d_qs;
+  a.nq = nq;
+  a.total_items = total_items;
+  a.counter = d_counter;
+  hipLaunchKernelGGL(batch_kernel, dim3(grid), dim3(256), lds_bytes, s, a);
   MBX_HIP(hipGetLastError());
+  return MBX_OK;
+}
+
+unsigned batch_grid(mbx_ctx *ctx, const mbx_tune &t, uint32_t total_items) {
+  return (unsigned)std::max<uint64_t>(
+      1, std::min<uint64_t>((uint64_t)ctx->num_cus * t.wg_per_cu, total_items));
+}
+
+static int enqueue_batch(mbx_ctx *ctx, mbx_batch *b) {
+  return batch_launch(ctx->stream, b->d_qs, b->nq, b->total_items,
+                      b->d_counter, b->lds_bytes, b->grid);
+}
+
+int batch_prepare(mbx_ctx *ctx, const mbx_query *qs, uint32_t nq,
+                  const mbx_tune &t, BatchQ *h, uint32_t *total_items,
+                  size_t *lds_bytes) {
+  uint32_t items = 0, max_lds_grp = 0;
+  for (uint32_t i = 0; i < nq; ++i) {
+    QueryPlan p;
+    int st = prepare(ctx, &qs[i].prg, &qs[i].op, qs[i].nr_grp, qs[i].d_out, &p);
+    if (st) {
+      set_error("batch query " + std::to_string(i) + ": " + mbx_last_error());
+      return st;
+    }
+    BatchQ &b = h[i];
+    std::memset(&b, 0, sizeof(b));
+    b.prg = p.prg;
+    b.op = p.op;
+    b.out = p.out;
+    b.nr_grp = p.nr_grp;
+    b.lds = p.nr_grp <= t.lds_groups ? 1u : 0u;
+    if (b.lds) max_lds_grp = std::max(max_lds_grp, p.nr_grp);
+    b.chk = p.prg.chk;
+    b.ntiles = (uint32_t)ceil_div(p.prg.nwords, kTileWords);
+    b.tiles_per_item = t.tile_words / kTileWords;
+    b.nitems = (uint32_t)ceil_div(b.ntiles, b.tiles_per_item);
+    b.item0 = items;
+    items += b.nitems;
+  }
+  *total_items = items;
+  *lds_bytes = (size_t)max_lds_grp * 8;
   return MBX_OK;
 }
 
@@ -151,36 +183,16 @@ static int build_batch(mbx_ctx *ctx, const mbx_query *qs, uint32_t nq,
   int st = valid_tune(&t);
   if (st) return st;
   std::vector<BatchQ> h(nq);
-  uint32_t items = 0, max_lds_grp = 0;
-  for (uint32_t i = 0; i < nq; ++i) {
-    QueryPlan p;
-    st = prepare(ctx, &qs[i].prg, &qs[i].op, qs[i].nr_grp, qs[i].d_out, &p);
-    if (st) {
-      set_error("batch query " + std::to_string(i) + ": " + mbx_last_error());
-      return st;
-    }
-    BatchQ &b = h[i];
-    std::memset(&b, 0, sizeof(b));
-    b.prg = p.prg;
-    b.op = p.op;
-    b.out = p.out;
-    b.nr_grp = p.nr_grp;
-    b.lds = p.nr_grp <= t.lds_groups ? 1u : 0u;
-    if (b.lds) max_lds_grp = std::max(max_lds_grp, p.nr_grp);
-    b.chk = p.prg.chk;
-    b.ntiles = (uint32_t)ceil_div(p.prg.nwords, kTileWords);
-    b.tiles_per_item = t.tile_words / kTileWords;
-    b.nitems = (uint32_t)ceil_div(b.ntiles, b.tiles_per_item);
-    b.item0 = items;
-    items += b.nitems;
-  }
+  uint32_t items = 0;
+  size_t lds_bytes = 0;
+  st = batch_prepare(ctx, qs, nq, t, h.data(), &items, &lds_bytes);
+  if (st) return st;
   auto *b = new mbx_batch;
   b->nq = nq;
   b->total_items = items;
   b->tune = t;
-  b->lds_bytes = (size_t)max_lds_grp * 8;
-  b->grid = (unsigned)std::max<uint64_t>(
-      1, std::min<uint64_t>((uint64_t)ctx->num_cus * t.wg_per_cu, items));
+  b->lds_bytes = lds_bytes;
+  b->grid = batch_grid(ctx, t, items);
   void *mem = nullptr;
   const size_t qbytes = ceil_div(sizeof(BatchQ) * nq, 256) * 256;
   st = ctx->pool->alloc(qbytes + 256, &mem);

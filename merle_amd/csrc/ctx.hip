@@ -213,6 +213,19 @@ int mbx_timer_stop(mbx_ctx *ctx, mbx_timer *t, float *ms) {
   return MBX_OK;
 }
 
+int mbx_timer_record_stop(mbx_ctx *ctx, mbx_timer *t) {
+  if (!ctx || !t) return fail(MBX_ERR_INVALID, "null");
+  MBX_HIP(hipEventRecord(t->b, ctx->stream));
+  return MBX_OK;
+}
+
+int mbx_timer_elapsed(mbx_timer *t, float *ms) {
+  if (!t || !ms) return fail(MBX_ERR_INVALID, "null");
+  MBX_HIP(hipEventSynchronize(t->b));
+  MBX_HIP(hipEventElapsedTime(ms, t->a, t->b));
+  return MBX_OK;
+}
+
 int mbx_timer_destroy(mbx_timer *t) {
   if (!t) return MBX_OK;
   (void)hipEventDestroy(t->a);
@@ -245,7 +258,8 @@ int mbx_abi_sizes(size_t *out, int n) {
   const size_t s[] = {sizeof(mbx_colref), sizeof(mbx_col),   sizeof(mbx_instr),
                       sizeof(mbx_vprg),   sizeof(mbx_pred),  sizeof(mbx_keypart),
                       sizeof(mbx_rowop),  sizeof(mbx_query), sizeof(mbx_tune),
-                      sizeof(mbx_plan)};
+                      sizeof(mbx_plan), sizeof(mbx_serve_cfg),
+                      sizeof(mbx_serve_stats)};
   const int m = (int)(sizeof(s) / sizeof(s[0]));
   for (int i = 0; i < n && i < m; ++i) out[i] = s[i];
   return m;

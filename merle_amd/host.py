@@ -127,6 +127,14 @@ class Timer:
         check(self.dev.lib.mbx_timer_stop(self.dev.ctx, self.t, C.byref(ms)))
         return ms.value
 
+    def mark_stop(self):
+        check(self.dev.lib.mbx_timer_record_stop(self.dev.ctx, self.t))
+
+    def elapsed(self):
+        ms = C.c_float()
+        check(self.dev.lib.mbx_timer_elapsed(self.t, C.byref(ms)))
+        return ms.value
+
     def close(self):
         self.dev.lib.mbx_timer_destroy(self.t)
 

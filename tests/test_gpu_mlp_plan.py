@@ -90,8 +90,10 @@ def test_every_plan_same_result(dev):
     b = list(configs.C1_BOUNDS)
     bins = {c: gpuq.create_bin(dev, sch, S.Src(c), b[:-1], b[1:])
             for c in configs.C1_COLS}
-    specs = [q for q in configs.config3_queries(40, seed=21)
-             if all(i[0] in (abi.ORM, abi.ANDM) and i[1] == 0 for i in q.instrs)][:6]
+    def conj(q):
+        return all(op == (abi.ORM if k == 0 else abi.ANDM) and d == 0
+                   for k, (op, d, _) in enumerate(q.instrs))
+    specs = [q for q in configs.config3_queries(60, seed=21) if conj(q)][:6]
     assert specs
     for s in specs:
         prg = S.to_vprg(configs.bin_cols(s, bins), n, s.instrs)
