@@ -178,7 +178,10 @@ enum mbx_flags {
   MBX_PLAN_MATERIALISE = 1,  /* _stg1/_stg2/_stg3 (primitive_abla.cuh:262-453) */
   MBX_PLAN_SCAN = 2,         /* grpby on every row (primitive.cuh:16-48) */
   MBX_PLAN_MASK = 3,
-  MBX_ACCUMULATE = 16        /* add into d_out instead of zeroing it first */
+  MBX_ACCUMULATE = 16,       /* add into d_out instead of zeroing it first */
+  MBX_PLAN_EAGER = 32        /* fused/materialise: load every bitmap operand
+                                up front instead of skipping ANDM loads for
+                                empty words (same bits, fewer round trips) */
 };
 
 /* Program + group-by in one call.  d_out[nr_grp] int64 on the device.

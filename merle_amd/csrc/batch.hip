@@ -26,10 +26,9 @@ struct BatchArgs {
 };
 
 __global__ __launch_bounds__(256) void batch_kernel(BatchArgs a) {
-  __shared__ uint32_t s_list[kListCap];
-  __shared__ uint32_t s_wave[4];
+  __shared__ uint32_t s_list[4][kWaveList];
   __shared__ uint32_t s_item;
-  extern __shared__ unsigned long long s_grp[];
+  const uint32_t wv = threadIdx.x >> 6;
   for (;;) {
     if (threadIdx.x == 0) s_item = atomicAdd(a.counter, 1u);
     __syncthreads();
@@ -44,25 +43,26 @@ __global__ __launch_bounds__(256) void batch_kernel(BatchArgs a) {
     }
     const BatchQ &q = a.qs[lo];
     GroupSink sink;
-    sink.lds = q.lds ? s_grp : nullptr;
+    sink.lds = q.lds != 0;
     sink.out = q.out;
     if (q.lds)
-      for (uint32_t i = threadIdx.x; i < q.nr_grp; i += 256) s_grp[i] = 0;
+      for (uint32_t i = threadIdx.x; i < q.nr_grp; i += 256) s_grp_dyn[i] = 0;
     __syncthreads();
+    // the item's wave tiles, spread over the workgroup's four waves
     const uint32_t t0 = (item - q.item0) * q.tiles_per_item;
     const uint32_t t1 = min(t0 + q.tiles_per_item, q.ntiles);
-    for (uint32_t t = t0; t < t1; ++t) {
+    for (uint32_t t = t0 + wv; t < t1; t += 4) {
       if (q.chk)
-        run_tile<true, false, kBWPL>(q.prg, q.op, q.nr_grp, t, sink, s_list,
-                                     s_wave, nullptr);
+        run_wave_tile<true, false, kBWPL, false>(q.prg, q.op, q.nr_grp, t, sink,
+                                                 s_list[wv], nullptr);
       else
-        run_tile<false, false, kBWPL>(q.prg, q.op, q.nr_grp, t, sink, s_list,
-                                      s_wave, nullptr);
+        run_wave_tile<false, false, kBWPL, false>(q.prg, q.op, q.nr_grp, t,
+                                                  sink, s_list[wv], nullptr);
     }
     if (q.lds) {
       __syncthreads();
       for (uint32_t i = threadIdx.x; i < q.nr_grp; i += 256) {
-        const unsigned long long v = s_grp[i];
+        const unsigned long long v = s_grp_dyn[i];
         if (v) atomicAdd(q.out + i, v);
       }
     }
@@ -163,7 +163,7 @@ static void destroy_batch(mbx_ctx *ctx, mbx_batch *b) {
 }
 
 static int valid_tune(const mbx_tune *t) {
-  if (t->tile_words == 0 || t->tile_words % kTileWords)
+  if (t->tile_words == 0 || t->tile_words % (4 * kTileWords))
     return fail(MBX_ERR_INVALID, "tune: tile_words must be a multiple of 512");
   if (t->wg_per_cu == 0 || t->wg_per_cu > 16)
     return fail(MBX_ERR_INVALID, "tune: wg_per_cu must be in 1..16");

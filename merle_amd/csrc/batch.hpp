@@ -5,7 +5,7 @@
 namespace mbx {
 
 constexpr int kBWPL = 2;                    // words per thread per tile
-constexpr uint32_t kTileWords = 256 * kBWPL;
+constexpr uint32_t kTileWords = 64 * kBWPL;   // one wave tile
 
 struct BatchQ {
   DevPrg prg;

@@ -93,7 +93,7 @@ __device__ __forceinline__ void store_line(uint32_t *dst, const uint64_t *w) {
     u64x2 x;
     x.x = w[2 * i];
     x.y = w[2 * i + 1];
-    __builtin_nontemporal_store(x, d + i);
+    d[i] = x;   // plain stores: L2 merges the 8 parts into one full-line write
   }
 }
 

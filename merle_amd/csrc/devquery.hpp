@@ -107,13 +107,30 @@ __device__ __forceinline__ uint64_t memop_u(const DevCol &c, uint32_t w,
 // Evaluates the program on 64-row word w.  nochk form: primitive.cuh:201-252;
 // chk form: primitive.cuh:255-332, with the three-valued NOT corrected to
 // P' = ~P_s | U_s, U' = U_s (the reference's P' = ~P_s drops uncertain rows).
-// ANDM skips its loads for lanes whose destination is already empty, the
-// early-out of primitive.cuh:215/292: a 128-byte line of the operand is
-// fetched only if one of its 16 words is still live.
-template <bool CHK, bool ACCT>
+// EAGER = false keeps the ANDM early-out of primitive.cuh:215/292: a lane
+// whose destination is already empty loads nothing, so a 128-byte line of
+// the operand is fetched only if one of its 16 words is live -- at the cost
+// of one memory round trip per instruction.  EAGER = true issues every
+// operand load of the program up front (one round trip for the word) and
+// combines afterwards; the plan picks it when the early-out would skip
+// almost no lines.  Both give the same bits.
+template <bool CHK, bool ACCT, bool EAGER = false>
 __device__ __forceinline__ void eval_word(const DevPrg &P, uint32_t w,
                                           uint64_t &po, uint64_t &uo,
                                           const Acct *acct) {
+  uint64_t pc[MBX_MAXNINSTR], pu[MBX_MAXNINSTR];
+  if (EAGER) {
+#pragma unroll
+    for (uint32_t i = 0; i < MBX_MAXNINSTR; ++i) {
+      pc[i] = 0;
+      pu[i] = 0;
+      if (i < P.nins && (P.ins[i][0] == MBX_ANDM || P.ins[i][0] == MBX_ORM)) {
+        const DevCol &c = P.cols[P.ins[i][2]];
+        pc[i] = memop_c<ACCT>(c, w, acct);
+        if (CHK) pu[i] = memop_u<ACCT>(c, w, acct);
+      }
+    }
+  }
   uint64_t p0 = 0, p1 = 0, p2 = 0, u0 = 0, u1 = 0, u2 = 0;
 #pragma unroll
   for (uint32_t i = 0; i < MBX_MAXNINSTR; ++i) {
@@ -144,22 +161,22 @@ __device__ __forceinline__ void eval_word(const DevPrg &P, uint32_t w,
       }
     } else if (op == MBX_ANDM) {
       const DevCol &c = P.cols[s];
-      if (pd != 0) {
+      if (EAGER || pd != 0) {
         if (CHK) {
-          const uint64_t us = memop_u<ACCT>(c, w, acct);
-          const uint64_t cs = memop_c<ACCT>(c, w, acct);
+          const uint64_t us = EAGER ? pu[i] : memop_u<ACCT>(c, w, acct);
+          const uint64_t cs = EAGER ? pc[i] : memop_c<ACCT>(c, w, acct);
           const uint64_t np = pd & (us | cs);
           rset(p0, p1, p2, d, np);
           rset(u0, u1, u2, d, (rget(u0, u1, u2, d) | us) & np);
         } else {
-          rset(p0, p1, p2, d, pd & memop_c<ACCT>(c, w, acct));
+          rset(p0, p1, p2, d, pd & (EAGER ? pc[i] : memop_c<ACCT>(c, w, acct)));
         }
       }
     } else if (op == MBX_ORM) {
       const DevCol &c = P.cols[s];
-      const uint64_t cs = memop_c<ACCT>(c, w, acct);
+      const uint64_t cs = EAGER ? pc[i] : memop_c<ACCT>(c, w, acct);
       if (CHK) {
-        const uint64_t us = memop_u<ACCT>(c, w, acct);
+        const uint64_t us = EAGER ? pu[i] : memop_u<ACCT>(c, w, acct);
         rset(u0, u1, u2, d, (rget(u0, u1, u2, d) & ~cs) | (us & ~pd));
         rset(p0, p1, p2, d, pd | us | cs);
       } else {
@@ -179,21 +196,43 @@ __device__ __forceinline__ void eval_word(const DevPrg &P, uint32_t w,
 }
 
 // ----------------------------------------------------------- row operation
+// Every value the row op needs is fetched in two rounds: first the foreign
+// keys of all its sources, then all values.  Sources never wait on one
+// another, so a row costs two memory round trips whatever its number of
+// predicates, key parts and measures (the reference's ops fetch one source
+// after the other, ssb/lambdas.cuh:104-151).
+template <bool ACCT>
+__device__ __forceinline__ uint32_t src_index(const DevSrc &s, uint32_t row,
+                                              const Acct *acct) {
+  if (s.fk == nullptr) return row;
+  mark<ACCT>(acct, s.fk + row);
+  return s.fk[row];
+}
+
+template <bool ACCT>
+__device__ __forceinline__ uint64_t src_value(const DevSrc &s, uint32_t idx,
+                                              bool &ok, const Acct *acct) {
+  ok = s.fk == nullptr || idx < s.dimsz;   // fk >= dimsz: no value
+  return ok ? elem_at<ACCT>(s.data, s.eb, idx, acct) : 0;
+}
+
 // Re-check of an uncertain row: the program's instruction list over the
 // exact predicates of its columns (the op's chk branch, ssb/lambdas.cuh:27-33).
 template <bool ACCT>
 __device__ __forceinline__ bool row_passes(const DevPrg &P, const DevOp &O,
-                                        uint32_t row, const Acct *acct) {
-  bool e0 = false, e1 = false, e2 = false, e3 = false;
-  for (uint32_t c = 0; c < O.npreds; ++c) {
-    uint64_t v;
-    const bool ok = fetch_a<ACCT>(O.pred[c], row, &v, acct) && v >= O.lo[c] &&
-                    v < O.hi[c];
-    if (c == 0) e0 = ok;
-    else if (c == 1) e1 = ok;
-    else if (c == 2) e2 = ok;
-    else e3 = ok;
-  }
+                                           uint32_t row, const Acct *acct) {
+  uint32_t ix[MBX_MAXCOLS];
+#pragma unroll
+  for (int c = 0; c < MBX_MAXCOLS; ++c)
+    if (c < (int)O.npreds) ix[c] = src_index<ACCT>(O.pred[c], row, acct);
+  bool e[MBX_MAXCOLS] = {false, false, false, false};
+#pragma unroll
+  for (int c = 0; c < MBX_MAXCOLS; ++c)
+    if (c < (int)O.npreds) {
+      bool ok;
+      const uint64_t v = src_value<ACCT>(O.pred[c], ix[c], ok, acct);
+      e[c] = ok && v >= O.lo[c] && v < O.hi[c];
+    }
   bool r0 = false, r1 = false, r2 = false;
   for (uint32_t i = 0; i < P.nins; ++i) {
     const uint32_t op = P.ins[i][0], d = P.ins[i][1], s = P.ins[i][2];
@@ -203,52 +242,106 @@ __device__ __forceinline__ bool row_passes(const DevPrg &P, const DevOp &O,
       const bool rs = s == 0 ? r0 : (s == 1 ? r1 : r2);
       v = op == MBX_AND ? (rd && rs) : (op == MBX_OR ? (rd || rs) : !rs);
     } else {
-      const bool es = s == 0 ? e0 : (s == 1 ? e1 : (s == 2 ? e2 : e3));
+      const bool es = s == 0 ? e[0] : (s == 1 ? e[1] : (s == 2 ? e[2] : e[3]));
       v = op == MBX_ANDM ? (rd && es) : (rd || es);
     }
-    if (d == 0) r0 = v;
-    else if (d == 1) r1 = v;
-    else r2 = v;
+    r0 = d == 0 ? v : r0;
+    r1 = d == 1 ? v : r1;
+    r2 = d == 2 ? v : r2;
   }
   return r0;
 }
 
-// {value, group} of a selected row (primitive.cuh:10-15).  false drops it.
+// {value, group} of E selected rows at once (primitive.cuh:10-15); ok[e] is
+// false for a dropped row (fk out of range or key outside [0, nr_grp)).
+template <bool ACCT, int E>
+__device__ __forceinline__ void rows_keyval(const DevOp &O, const uint32_t *row,
+                                            const bool *live, uint32_t nr_grp,
+                                            uint32_t *key, int64_t *val,
+                                            bool *ok, const Acct *acct) {
+  constexpr int NS = MBX_MAXKEYS + 2;   // key parts, then measure a, b
+  const bool has_a = O.measure != MBX_MEAS_COUNT;
+  const bool has_b = O.measure == MBX_MEAS_MUL || O.measure == MBX_MEAS_SUB;
+  uint32_t ix[E][NS];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    if (!live[e]) continue;
+#pragma unroll
+    for (int i = 0; i < MBX_MAXKEYS; ++i)
+      if (i < (int)O.nkeys) ix[e][i] = src_index<ACCT>(O.keys[i].src, row[e], acct);
+    if (has_a) ix[e][3] = src_index<ACCT>(O.ma, row[e], acct);
+    if (has_b) ix[e][4] = src_index<ACCT>(O.mb, row[e], acct);
+  }
+  uint64_t v[E][NS];
+  bool vok[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    vok[e] = live[e];
+    if (!live[e]) continue;
+#pragma unroll
+    for (int i = 0; i < MBX_MAXKEYS; ++i)
+      if (i < (int)O.nkeys) {
+        bool o;
+        v[e][i] = src_value<ACCT>(O.keys[i].src, ix[e][i], o, acct);
+        vok[e] = vok[e] && o;
+      }
+    v[e][3] = 1;
+    v[e][4] = 0;
+    if (has_a) {
+      bool o;
+      v[e][3] = src_value<ACCT>(O.ma, ix[e][3], o, acct);
+      vok[e] = vok[e] && o;
+    }
+    if (has_b) {
+      bool o;
+      v[e][4] = src_value<ACCT>(O.mb, ix[e][4], o, acct);
+      vok[e] = vok[e] && o;
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    int64_t k = 0;
+#pragma unroll
+    for (int i = 0; i < MBX_MAXKEYS; ++i)
+      if (i < (int)O.nkeys) {
+        const DevKey &kp = O.keys[i];
+        const uint64_t x = v[e][i];
+        uint64_t f = x;
+        if (kp.kind == MBX_KEY_DIV) {
+          f = x / kp.div;
+        } else if (kp.kind == MBX_KEY_BUCKET) {
+          f = 0;
+          for (uint32_t j = 0; j < kp.nb; ++j) f += kp.bounds[j] <= x;
+        }
+        k += kp.stride * ((int64_t)f - kp.base);
+      }
+    const uint64_t a = v[e][3], b = v[e][4];
+    val[e] = O.measure == MBX_MEAS_MUL   ? (int64_t)(a * b)
+             : O.measure == MBX_MEAS_SUB ? (int64_t)a - (int64_t)b
+                                         : (int64_t)a;
+    key[e] = (uint32_t)k;
+    ok[e] = vok[e] && k >= 0 && k < (int64_t)nr_grp;
+  }
+}
+
 template <bool ACCT>
 __device__ __forceinline__ bool row_keyval(const DevOp &O, uint32_t row,
                                            uint32_t nr_grp, uint32_t &key,
                                            int64_t &val, const Acct *acct) {
-  int64_t k = 0;
-  for (uint32_t i = 0; i < O.nkeys; ++i) {
-    const DevKey &kp = O.keys[i];
-    uint64_t v;
-    if (!fetch_a<ACCT>(kp.src, row, &v, acct)) return false;
-    uint64_t f = v;
-    if (kp.kind == MBX_KEY_DIV) {
-      f = v / kp.div;
-    } else if (kp.kind == MBX_KEY_BUCKET) {
-      f = 0;
-      for (uint32_t j = 0; j < kp.nb; ++j) f += kp.bounds[j] <= v;
-    }
-    k += kp.stride * ((int64_t)f - kp.base);
-  }
-  if (k < 0 || k >= (int64_t)nr_grp) return false;
-  key = (uint32_t)k;
-  // measure: 1, a, a*b or a-b; one exit for every kind
-  uint64_t a = 1, b = 0;
-  bool ok = true;
-  if (O.measure != MBX_MEAS_COUNT) ok = fetch_a<ACCT>(O.ma, row, &a, acct);
-  if (ok && O.measure >= MBX_MEAS_MUL) ok = fetch_a<ACCT>(O.mb, row, &b, acct);
-  val = O.measure == MBX_MEAS_MUL   ? (int64_t)(a * b)
-        : O.measure == MBX_MEAS_SUB ? (int64_t)a - (int64_t)b
-                                    : (int64_t)a;
+  const bool live = true;
+  bool ok;
+  rows_keyval<ACCT, 1>(O, &row, &live, nr_grp, &key, &val, &ok, acct);
   return ok;
 }
 
-// Aggregation target: an LDS table (small group counts) or the output in
-// HBM (global int64 atomics).
+// Aggregation target: the workgroup's LDS table (small group counts) or the
+// output in HBM (global int64 atomics).  The LDS table is the dynamic LDS of
+// the launch, named here so that its atomics stay in the LDS address space
+// (they then never order against the global loads of other rows).
+extern __shared__ unsigned long long s_grp_dyn[];
+
 struct GroupSink {
-  unsigned long long *lds;   // non-null: LDS table of nr_grp slots
+  bool lds;                  // true: LDS table of nr_grp slots
   unsigned long long *out;   // HBM output
 };
 
@@ -261,26 +354,48 @@ __device__ __forceinline__ void sink_add(const GroupSink &s, uint32_t key,
                                          int64_t val, uint32_t nr_grp) {
   asm volatile("" : "+v"(key));
   if (key >= nr_grp) return;
-  if (s.lds) atomicAdd(s.lds + key, (unsigned long long)val);
+  if (s.lds) atomicAdd(&s_grp_dyn[key], (unsigned long long)val);
   else atomicAdd(s.out + key, (unsigned long long)val);
 }
 
-// One list entry: row id, MSB = uncertain (primitive.cuh:406-407).
+// List entries: row id, MSB = uncertain (primitive.cuh:406-407).  E entries
+// per lane share their memory round trips.
+template <bool CHK, bool ACCT, int E>
+__device__ __forceinline__ void process_entries(const DevPrg &P,
+                                                const DevOp &O,
+                                                uint32_t nr_grp,
+                                                const uint32_t *ent,
+                                                const bool *valid,
+                                                const GroupSink &sink,
+                                                const Acct *acct) {
+  uint32_t row[E];
+  bool live[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    row[e] = ent[e] & 0x7fffffffu;
+    live[e] = valid[e];
+    if (CHK && live[e] && (ent[e] >> 31))
+      live[e] = row_passes<ACCT>(P, O, row[e], acct);
+  }
+  uint32_t key[E];
+  int64_t val[E];
+  bool ok[E];
+  rows_keyval<ACCT, E>(O, row, live, nr_grp, key, val, ok, acct);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    if (!ok[e]) continue;
+    if (ACCT) atomicAdd(acct->selected, 1ull);
+    else sink_add(sink, key[e], val[e], nr_grp);
+  }
+}
+
 template <bool CHK, bool ACCT>
 __device__ __forceinline__ void process_entry(const DevPrg &P, const DevOp &O,
                                               uint32_t nr_grp, uint32_t e,
                                               const GroupSink &sink,
                                               const Acct *acct) {
-  const uint32_t row = e & 0x7fffffffu;
-  if (CHK && (e >> 31) && !row_passes<ACCT>(P, O, row, acct)) return;
-  uint32_t key;
-  int64_t val;
-  if (!row_keyval<ACCT>(O, row, nr_grp, key, val, acct)) return;
-  if (ACCT) {
-    atomicAdd(acct->selected, 1ull);
-    return;
-  }
-  sink_add(sink, key, val, nr_grp);
+  const bool valid = true;
+  process_entries<CHK, ACCT, 1>(P, O, nr_grp, &e, &valid, sink, acct);
 }
 
 // ------------------------------------------------------- block primitives
@@ -308,37 +423,51 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t x,
 }
 
 // ------------------------------------------------------------ tile pipeline
-// One tile = 256 threads x WPL 64-row words.  Program -> popcount -> block
-// scan -> LDS row list (rounds of kListCap entries) -> row op -> sink.
-// The reference's method 1/method 2 switch (primitive.cuh:373-397) becomes
-// rounds: a dense tile simply takes more rounds, every round using all lanes.
-template <bool CHK, bool ACCT, int WPL>
-__device__ __forceinline__ void run_tile(const DevPrg &P, const DevOp &O,
-                                         uint32_t nr_grp, uint32_t tile,
-                                         const GroupSink &sink,
-                                         uint32_t *s_list, uint32_t *s_wave,
-                                         const Acct *acct) {
-  const uint32_t tid = threadIdx.x;
+// Orders this wave's LDS writes before its own later LDS reads: the LDS
+// executes a wave's instructions in issue order, so only the compiler must
+// be kept from moving accesses across this point.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One wave tile = 64 lanes x WPL 64-row words: program -> popcount -> wave
+// scan -> the wave's own LDS row list (rounds of kWaveList entries) -> row
+// op -> sink.  Waves never wait for each other (no workgroup barrier), so a
+// CU overlaps the bitmap loads of some waves with the gathers of others.
+// The reference's method 1 / method 2 switch (primitive.cuh:373-397)
+// becomes rounds: a dense tile takes more rounds, each using all lanes.
+constexpr uint32_t kWaveList = kListCap / 4;
+
+template <bool CHK, bool ACCT, int WPL, bool EAGER>
+__device__ __forceinline__ void run_wave_tile(const DevPrg &P, const DevOp &O,
+                                              uint32_t nr_grp, uint32_t wt,
+                                              const GroupSink &sink,
+                                              uint32_t *s_list,
+                                              const Acct *acct) {
+  const uint32_t lane = lane_id();
   uint64_t p[WPL], u[WPL];
   uint32_t cnt = 0;
 #pragma unroll
   for (int k = 0; k < WPL; ++k) {
-    const uint32_t w = (tile * WPL + k) * 256 + tid;
+    const uint32_t w = (wt * WPL + k) * 64 + lane;
     p[k] = 0;
     u[k] = 0;
-    if (w < P.nwords) eval_word<CHK, ACCT>(P, w, p[k], u[k], acct);
+    if (w < P.nwords) eval_word<CHK, ACCT, EAGER>(P, w, p[k], u[k], acct);
     cnt += __popcll(p[k]);
   }
-  uint32_t total;
-  const uint32_t off = block_excl_scan(cnt, s_wave, &total);
-  for (uint32_t base = 0; base < total; base += kListCap) {
-    const uint32_t end = base + kListCap;
+  const uint32_t incl = wave_incl_scan(cnt);
+  const uint32_t total = __shfl(incl, 63, 64);
+  const uint32_t off = incl - cnt;
+  for (uint32_t base = 0; base < total; base += kWaveList) {
+    const uint32_t end = base + kWaveList;
     if (cnt && off < end && off + cnt > base) {
       uint32_t pos = off;
 #pragma unroll
       for (int k = 0; k < WPL; ++k) {
         uint64_t m = p[k];
-        const uint32_t row0 = ((tile * WPL + k) * 256 + tid) * 64;
+        const uint32_t row0 = ((wt * WPL + k) * 64 + lane) * 64;
         while (m && pos < end) {
           const uint32_t b = (uint32_t)__builtin_ctzll(m);
           m &= m - 1;
@@ -351,11 +480,18 @@ __device__ __forceinline__ void run_tile(const DevPrg &P, const DevOp &O,
         }
       }
     }
-    __syncthreads();
-    const uint32_t nl = min(kListCap, total - base);
-    for (uint32_t j = tid; j < nl; j += 256)
-      process_entry<CHK, ACCT>(P, O, nr_grp, s_list[j], sink, acct);
-    __syncthreads();
+    wave_lds_sync();
+    const uint32_t nl = min(kWaveList, total - base);
+    for (uint32_t j = lane; j < nl; j += 128) {
+      uint32_t ent[2];
+      bool valid[2];
+      ent[0] = s_list[j];
+      valid[0] = true;
+      valid[1] = j + 64 < nl;
+      ent[1] = valid[1] ? s_list[j + 64] : 0;
+      process_entries<CHK, ACCT, 2>(P, O, nr_grp, ent, valid, sink, acct);
+    }
+    wave_lds_sync();
   }
 }
 

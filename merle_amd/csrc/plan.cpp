@@ -105,19 +105,19 @@ int mbx_plan_features(const mbx_vprg *p, const double *dens, uint32_t nr_grp,
     chk |= (p->cols[c].leftmost || p->cols[c].rightmost);
   }
   while (nins < MBX_MAXNINSTR && p->instrs[nins].opcode != MBX_END) ++nins;
-  f[8] = nins / 6.f;
+  f[8] = nins / 6.f + 0.5f * chk;
   f[9] = (float)(std::log2((double)nr_grp + 1.0) / 20.0);
   f[10] = (float)(std::log2((double)p->factsz + 1.0) / 32.0);
   f[11] = (plan->flags & MBX_PLAN_MASK) == MBX_PLAN_MATERIALISE ? 1.f : 0.f;
   f[12] = (float)(std::log10(std::max(run, 1e-9)) / 9.0 + 1.0);
   f[13] = nbins / 24.f;
-  f[14] = (float)chk;
+  f[14] = (plan->flags & MBX_PLAN_EAGER) ? 1.f : 0.f;
   f[15] = n ? (float)(lines / n) : 0.f;
   return MBX_OK;
 }
 
-// Every order of a conjunction x {fused, materialise}; other programs get
-// their own order only.
+// Every order of a conjunction x {fused, fused eager, materialise}; other
+// programs get their own order only.
 int mbx_plan_enumerate(const mbx_vprg *p, mbx_plan *out, int max, int *count) {
   if (!p || !out || !count) return fail(MBX_ERR_INVALID, "enumerate: null");
   mbx_plan base;
@@ -127,7 +127,9 @@ int mbx_plan_enumerate(const mbx_vprg *p, mbx_plan *out, int max, int *count) {
   std::vector<uint8_t> perm(base.order, base.order + base.norder);
   std::sort(perm.begin(), perm.end());
   do {
-    for (uint32_t fl : {(uint32_t)MBX_PLAN_FUSED, (uint32_t)MBX_PLAN_MATERIALISE}) {
+    for (uint32_t fl : {(uint32_t)MBX_PLAN_FUSED,
+                        (uint32_t)(MBX_PLAN_FUSED | MBX_PLAN_EAGER),
+                        (uint32_t)MBX_PLAN_MATERIALISE}) {
       mbx_plan q = base;
       std::copy(perm.begin(), perm.end(), q.order);
       q.flags = fl;
@@ -148,6 +150,14 @@ int mbx_plan_heuristic(const mbx_vprg *p, const double *dens, mbx_plan *out) {
     std::stable_sort(out->order, out->order + out->norder,
                      [&](uint8_t a, uint8_t b) { return dens[a] < dens[b]; });
   out->flags = MBX_PLAN_FUSED;
+  // Load every operand up front unless the ANDM early-out would skip
+  // lines: a 1024-row line of the next operand is skipped only when all its
+  // rows are already empty, probability (1-d)^1024 for a first operand of
+  // density d; below 10% the early-out saves too little to pay for a
+  // round trip per instruction.
+  const double d0 = out->reorderable ? dens[out->order[0]] : 1.0;
+  if (std::pow(1.0 - std::min(1.0, std::max(0.0, d0)), 1024.0) < 0.1)
+    out->flags |= MBX_PLAN_EAGER;
   return MBX_OK;
 }
 
@@ -159,9 +169,9 @@ int mbx_plan_choose(mbx_ctx *ctx, mbx_mlp *mlp, const mbx_vprg *p,
                     float *pred) {
   if (!ctx || !p || !dens || !best) return fail(MBX_ERR_INVALID, "choose: null");
   if (!mlp) return mbx_plan_heuristic(p, dens, best);
-  mbx_plan cand[48];
+  mbx_plan cand[80];
   int n = 0;
-  int st = mbx_plan_enumerate(p, cand, 48, &n);
+  int st = mbx_plan_enumerate(p, cand, 80, &n);
   if (st) return st;
   std::vector<float> feat((size_t)n * MBX_NFEAT);
   for (int i = 0; i < n; ++i) {
@@ -201,7 +211,9 @@ int mbx_prog_grpby_plan(mbx_ctx *ctx, const mbx_vprg *prg, const mbx_rowop *op,
   mbx_vprg q;
   int st = mbx_plan_apply(prg, plan, &q);
   if (st) return st;
-  return mbx_prog_grpby(ctx, &q, op, nr_grp, d_out, plan->flags & MBX_PLAN_MASK);
+  return mbx_prog_grpby(ctx, &q, op, nr_grp, d_out,
+                        plan->flags &
+                            (MBX_PLAN_MASK | MBX_PLAN_EAGER | MBX_ACCUMULATE));
 }
 
 }  // extern "C"

@@ -4,6 +4,8 @@
 //   scan         grpby                 (primitive.cuh:16-48)
 // plus the algorithmic-byte accounting kernel behind mbx_alg_bytes.
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "devquery.hpp"
@@ -25,24 +27,23 @@ struct QArgs {
 // keeps its group table in LDS until the end, so the HBM output sees at most
 // one atomic per (workgroup, non-empty group) -- never more than the rows the
 // workgroup aggregated.
-template <bool CHK, bool LDSG, bool ACCT, int WPL>
+template <bool CHK, bool LDSG, bool ACCT, int WPL, bool EAGER>
 __global__ __launch_bounds__(256) void fused_kernel(QArgs a) {
-  __shared__ uint32_t s_list[kListCap];
-  __shared__ uint32_t s_wave[4];
-  extern __shared__ unsigned long long s_grp[];
+  __shared__ uint32_t s_list[4][kWaveList];
   GroupSink sink;
-  sink.lds = LDSG ? s_grp : nullptr;
+  sink.lds = LDSG;
   sink.out = a.out;
   if (LDSG)
-    for (uint32_t i = threadIdx.x; i < a.nr_grp; i += 256) s_grp[i] = 0;
+    for (uint32_t i = threadIdx.x; i < a.nr_grp; i += 256) s_grp_dyn[i] = 0;
   __syncthreads();
-  for (uint32_t t = blockIdx.x; t < a.ntiles; t += gridDim.x)
-    run_tile<CHK, ACCT, WPL>(a.prg, a.op, a.nr_grp, t, sink, s_list, s_wave,
-                             a.acct);
+  const uint32_t wv = threadIdx.x >> 6;
+  for (uint32_t t = blockIdx.x * 4 + wv; t < a.ntiles; t += gridDim.x * 4)
+    run_wave_tile<CHK, ACCT, WPL, EAGER>(a.prg, a.op, a.nr_grp, t, sink,
+                                         s_list[wv], a.acct);
   if (LDSG) {
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < a.nr_grp; i += 256) {
-      const unsigned long long v = s_grp[i];
+      const unsigned long long v = s_grp_dyn[i];
       if (v) atomicAdd(a.out + i, v);
     }
   }
@@ -52,12 +53,11 @@ __global__ __launch_bounds__(256) void fused_kernel(QArgs a) {
 // Every row, all predicates checked (op(row, chk=true)), no bitmaps.
 template <bool LDSG>
 __global__ __launch_bounds__(256) void scan_kernel(QArgs a) {
-  extern __shared__ unsigned long long s_grp[];
   GroupSink sink;
-  sink.lds = LDSG ? s_grp : nullptr;
+  sink.lds = LDSG;
   sink.out = a.out;
   if (LDSG)
-    for (uint32_t i = threadIdx.x; i < a.nr_grp; i += 256) s_grp[i] = 0;
+    for (uint32_t i = threadIdx.x; i < a.nr_grp; i += 256) s_grp_dyn[i] = 0;
   __syncthreads();
   const uint32_t n = a.prg.nrows;
   for (uint32_t row = blockIdx.x * 256 + threadIdx.x; row < n;
@@ -71,7 +71,7 @@ __global__ __launch_bounds__(256) void scan_kernel(QArgs a) {
   if (LDSG) {
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < a.nr_grp; i += 256) {
-      const unsigned long long v = s_grp[i];
+      const unsigned long long v = s_grp_dyn[i];
       if (v) atomicAdd(a.out + i, v);
     }
   }
@@ -86,7 +86,7 @@ __global__ __launch_bounds__(256) void stage1_kernel(DevPrg P,
   for (uint32_t w = blockIdx.x * 256 + threadIdx.x; w < P.nwords;
        w += gridDim.x * 256) {
     uint64_t p, u;
-    eval_word<CHK, false>(P, w, p, u, nullptr);
+    eval_word<CHK, false, true>(P, w, p, u, nullptr);
     po[w] = p;
     if (CHK) uo[w] = u;
   }
@@ -135,12 +135,11 @@ __global__ __launch_bounds__(256) void stage2_kernel(
 template <bool CHK, bool LDSG>
 __global__ __launch_bounds__(256) void stage3_kernel(
     QArgs a, const uint32_t *__restrict__ rows, const uint32_t *count) {
-  extern __shared__ unsigned long long s_grp[];
   GroupSink sink;
-  sink.lds = LDSG ? s_grp : nullptr;
+  sink.lds = LDSG;
   sink.out = a.out;
   if (LDSG)
-    for (uint32_t i = threadIdx.x; i < a.nr_grp; i += 256) s_grp[i] = 0;
+    for (uint32_t i = threadIdx.x; i < a.nr_grp; i += 256) s_grp_dyn[i] = 0;
   __syncthreads();
   const uint32_t n = *count;
   for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j < n;
@@ -149,7 +148,7 @@ __global__ __launch_bounds__(256) void stage3_kernel(
   if (LDSG) {
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < a.nr_grp; i += 256) {
-      const unsigned long long v = s_grp[i];
+      const unsigned long long v = s_grp_dyn[i];
       if (v) atomicAdd(a.out + i, v);
     }
   }
@@ -232,31 +231,66 @@ static size_t grp_lds(const QueryPlan &q) {
   return q.lds ? (size_t)q.nr_grp * 8 : 0;
 }
 
+// Resident workgroups per CU for a kernel at a dynamic LDS size (cached).
+static unsigned occupancy(const void *fn, size_t lds) {
+  static std::mutex mu;
+  static std::map<std::pair<const void *, size_t>, unsigned> cache;
+  std::lock_guard<std::mutex> g(mu);
+  auto key = std::make_pair(fn, lds);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 256, lds) !=
+          hipSuccess ||
+      nb < 1) {
+    (void)hipGetLastError();
+    nb = 2;
+  }
+  cache[key] = (unsigned)nb;
+  return (unsigned)nb;
+}
+
+template <bool C, bool L, bool A, bool E>
+static void fused_launch(mbx_ctx *ctx, const QArgs &a, size_t lds) {
+  const void *fn = reinterpret_cast<const void *>(&fused_kernel<C, L, A, kWPL, E>);
+  const unsigned per_cu = occupancy(fn, lds);
+  const unsigned grid =
+      persistent_grid(ctx, (uint32_t)ceil_div(a.ntiles, 4), per_cu);
+  hipLaunchKernelGGL((fused_kernel<C, L, A, kWPL, E>), dim3(grid), dim3(256),
+                     lds, ctx->stream, a);
+}
+
 int launch_fused(mbx_ctx *ctx, const QueryPlan &q, const Acct *acct) {
   QArgs a;
   a.prg = q.prg;
   a.op = q.op;
   a.out = q.out;
   a.nr_grp = q.nr_grp;
-  a.ntiles = (uint32_t)ceil_div(q.prg.nwords, 256 * kWPL);
+  a.ntiles = (uint32_t)ceil_div(q.prg.nwords, 64 * kWPL);   // wave tiles
   a.acct = acct;
-  const unsigned grid = persistent_grid(ctx, a.ntiles, 4);
-  const size_t lds = grp_lds(q);
+  const size_t lds = acct ? 0 : grp_lds(q);
   const bool chk = q.prg.chk != 0;
-#define MBX_FUSED(C, L, A)                                                     \
-  hipLaunchKernelGGL((fused_kernel<C, L, A, kWPL>), dim3(grid), dim3(256),     \
-                     lds, ctx->stream, a)
+  // accounting runs the early-out form: the compulsory-line model
   if (acct) {
-    if (chk) MBX_FUSED(true, false, true);
-    else MBX_FUSED(false, false, true);
-  } else if (chk) {
-    if (q.lds) MBX_FUSED(true, true, false);
-    else MBX_FUSED(true, false, false);
+    if (chk) fused_launch<true, false, true, false>(ctx, a, lds);
+    else fused_launch<false, false, true, false>(ctx, a, lds);
+  } else if (q.eager) {
+    if (chk) {
+      if (q.lds) fused_launch<true, true, false, true>(ctx, a, lds);
+      else fused_launch<true, false, false, true>(ctx, a, lds);
+    } else {
+      if (q.lds) fused_launch<false, true, false, true>(ctx, a, lds);
+      else fused_launch<false, false, false, true>(ctx, a, lds);
+    }
   } else {
-    if (q.lds) MBX_FUSED(false, true, false);
-    else MBX_FUSED(false, false, false);
+    if (chk) {
+      if (q.lds) fused_launch<true, true, false, false>(ctx, a, lds);
+      else fused_launch<true, false, false, false>(ctx, a, lds);
+    } else {
+      if (q.lds) fused_launch<false, true, false, false>(ctx, a, lds);
+      else fused_launch<false, false, false, false>(ctx, a, lds);
+    }
   }
-#undef MBX_FUSED
   MBX_HIP(hipGetLastError());
   return MBX_OK;
 }
@@ -370,6 +404,7 @@ int mbx_prog_grpby(mbx_ctx *ctx, const mbx_vprg *prg, const mbx_rowop *op,
     st = launch_fill_u64(ctx->stream, d_out, 0, nr_grp);
     if (st) return st;
   }
+  q.eager = (flags & MBX_PLAN_EAGER) != 0;
   switch (flags & MBX_PLAN_MASK) {
   case MBX_PLAN_FUSED: return launch_fused(ctx, q, nullptr);
   case MBX_PLAN_MATERIALISE: return run_materialise(ctx, q);

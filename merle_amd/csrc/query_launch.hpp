@@ -12,6 +12,7 @@ struct QueryPlan {
   unsigned long long *out = nullptr;
   uint32_t nr_grp = 0;
   bool lds = false;   // group table fits in LDS
+  bool eager = false; // all operand loads up front (no ANDM early-out)
 };
 
 // Validates the query and builds its device descriptors.

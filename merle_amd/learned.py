@@ -112,9 +112,9 @@ class Prepared:
         self.op = to_rowop(spec, sch)
         self.dens = (C.c_double * 4)()
         check(dev.lib.mbx_col_density(dev.ctx, C.byref(self.prg), self.dens))
-        cand = (abi.Plan * 48)()
+        cand = (abi.Plan * 80)()
         cnt = C.c_int()
-        check(dev.lib.mbx_plan_enumerate(C.byref(self.prg), cand, 48, C.byref(cnt)))
+        check(dev.lib.mbx_plan_enumerate(C.byref(self.prg), cand, 80, C.byref(cnt)))
         self.cands = list(cand[:cnt.value])
         self.as_written = abi.Plan()
         check(dev.lib.mbx_plan_describe(C.byref(self.prg), C.byref(self.as_written)))

@@ -101,9 +101,9 @@ def test_every_plan_same_result(dev):
         want, _ = cpuref.query(s, sch)
         dens = (C.c_double * 4)()
         check(dev.lib.mbx_col_density(dev.ctx, C.byref(prg), dens))
-        plans = (abi.Plan * 48)()
+        plans = (abi.Plan * 80)()
         cnt = C.c_int()
-        check(dev.lib.mbx_plan_enumerate(C.byref(prg), plans, 48, C.byref(cnt)))
+        check(dev.lib.mbx_plan_enumerate(C.byref(prg), plans, 80, C.byref(cnt)))
         heur = abi.Plan()
         check(dev.lib.mbx_plan_heuristic(C.byref(prg), dens, C.byref(heur)))
         d = sorted(dens[c] for c in range(len(s.preds)))
