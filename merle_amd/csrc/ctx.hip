@@ -111,6 +111,23 @@ int mbx_init(int device, size_t pool_bytes, mbx_ctx **out) {
     return st;
   }
   c->d_counter = static_cast<uint32_t *>(p);
+  const size_t rep_bytes = (size_t)kSpreadRep * kLdsGroups * 8;
+  const size_t done_bytes = (size_t)kSpreadRep * kSpreadDoneStride * 4;
+  st = c->pool->alloc(rep_bytes + done_bytes, &p);
+  if (st == MBX_OK)
+    st = launch_fill_u32(c->stream, p, 0, (rep_bytes + done_bytes) / 4);
+  if (st == MBX_OK) {
+    hipError_t se = hipStreamSynchronize(c->stream);
+    if (se != hipSuccess) st = hip_fail(se, "mbx_init: scratch");
+  }
+  if (st != MBX_OK) {
+    delete c->pool;
+    (void)hipStreamDestroy(c->own_stream);
+    delete c;
+    return st;
+  }
+  c->spread = static_cast<unsigned long long *>(p);
+  c->spread_done = reinterpret_cast<uint32_t *>(static_cast<char *>(p) + rep_bytes);
   *out = c;
   return MBX_OK;
 }

@@ -86,113 +86,166 @@ __device__ __forceinline__ uint64_t ld_bmp(const uint64_t *p, uint32_t w,
   return ld_stream(p + w);
 }
 
-// Memory operand of one column at word w: c = OR of middle bins,
-// u = leftmost | rightmost (primitive.cuh:216-226, 294-300).
-template <bool ACCT>
-__device__ __forceinline__ uint64_t memop_c(const DevCol &c, uint32_t w,
-                                            const Acct *acct) {
-  uint64_t m = 0;
-  for (uint32_t k = 0; k < c.nm; ++k) m |= ld_bmp<ACCT>(c.m[k], w, acct);
-  return m;
+// Memory operands of one column at the WPL words w0 + 64k (k < WPL) whose
+// lanes are live: c = OR of middle bins, u = leftmost | rightmost
+// (primitive.cuh:216-226, 294-300).  All loads are independent, so the WPL
+// words of an operand cost one memory round trip.
+template <bool ACCT, int WPL>
+__device__ __forceinline__ void memop_c(const DevCol &c, uint32_t w0,
+                                        const bool (&live)[WPL],
+                                        uint64_t (&m)[WPL], const Acct *acct) {
+#pragma unroll
+  for (int k = 0; k < WPL; ++k) m[k] = 0;
+  for (uint32_t b = 0; b < c.nm; ++b)
+#pragma unroll
+    for (int k = 0; k < WPL; ++k)
+      if (live[k]) m[k] |= ld_bmp<ACCT>(c.m[b], w0 + 64u * k, acct);
 }
-template <bool ACCT>
-__device__ __forceinline__ uint64_t memop_u(const DevCol &c, uint32_t w,
-                                            const Acct *acct) {
-  uint64_t e = 0;
-  if (c.l) e |= ld_bmp<ACCT>(c.l, w, acct);
-  if (c.r) e |= ld_bmp<ACCT>(c.r, w, acct);
-  return e;
+template <bool ACCT, int WPL>
+__device__ __forceinline__ void memop_u(const DevCol &c, uint32_t w0,
+                                        const bool (&live)[WPL],
+                                        uint64_t (&e)[WPL], const Acct *acct) {
+#pragma unroll
+  for (int k = 0; k < WPL; ++k) {
+    e[k] = 0;
+    if (live[k] && c.l) e[k] |= ld_bmp<ACCT>(c.l, w0 + 64u * k, acct);
+    if (live[k] && c.r) e[k] |= ld_bmp<ACCT>(c.r, w0 + 64u * k, acct);
+  }
 }
 
-// Evaluates the program on 64-row word w.  nochk form: primitive.cuh:201-252;
-// chk form: primitive.cuh:255-332, with the three-valued NOT corrected to
-// P' = ~P_s | U_s, U' = U_s (the reference's P' = ~P_s drops uncertain rows).
-// EAGER = false keeps the ANDM early-out of primitive.cuh:215/292: a lane
+// Evaluates the program on the WPL 64-row words w0 + 64k of this lane.
+// nochk form: primitive.cuh:201-252; chk form: primitive.cuh:255-332, with
+// the three-valued NOT corrected to P' = ~P_s | U_s, U' = U_s (the
+// reference's P' = ~P_s drops uncertain rows).
+// EAGER = false keeps the ANDM early-out of primitive.cuh:215/292: a word
 // whose destination is already empty loads nothing, so a 128-byte line of
 // the operand is fetched only if one of its 16 words is live -- at the cost
-// of one memory round trip per instruction.  EAGER = true issues every
-// operand load of the program up front (one round trip for the word) and
+// of one memory round trip per memory instruction.  EAGER = true issues
+// every operand load of the program up front (one round trip in all) and
 // combines afterwards; the plan picks it when the early-out would skip
-// almost no lines.  Both give the same bits.
-template <bool CHK, bool ACCT, bool EAGER = false>
-__device__ __forceinline__ void eval_word(const DevPrg &P, uint32_t w,
-                                          uint64_t &po, uint64_t &uo,
-                                          const Acct *acct) {
-  uint64_t pc[MBX_MAXNINSTR], pu[MBX_MAXNINSTR];
+// almost no lines.  Both give the same bits: an empty destination has no
+// uncertain rows (U is a subset of P after every instruction), so ANDM on it
+// yields the same empty pair whether or not the operand is read.
+template <bool CHK, bool ACCT, bool EAGER, int WPL>
+__device__ __forceinline__ void eval_words(const DevPrg &P, uint32_t w0,
+                                           uint64_t (&po)[WPL],
+                                           uint64_t (&uo)[WPL],
+                                           const Acct *acct) {
+  bool in[WPL];
+#pragma unroll
+  for (int k = 0; k < WPL; ++k) in[k] = w0 + 64u * k < P.nwords;
+  uint64_t pc[MBX_MAXNINSTR][WPL], pu[MBX_MAXNINSTR][WPL];
   if (EAGER) {
 #pragma unroll
     for (uint32_t i = 0; i < MBX_MAXNINSTR; ++i) {
-      pc[i] = 0;
-      pu[i] = 0;
+#pragma unroll
+      for (int k = 0; k < WPL; ++k) pc[i][k] = pu[i][k] = 0;
       if (i < P.nins && (P.ins[i][0] == MBX_ANDM || P.ins[i][0] == MBX_ORM)) {
         const DevCol &c = P.cols[P.ins[i][2]];
-        pc[i] = memop_c<ACCT>(c, w, acct);
-        if (CHK) pu[i] = memop_u<ACCT>(c, w, acct);
+        memop_c<ACCT, WPL>(c, w0, in, pc[i], acct);
+        if (CHK) memop_u<ACCT, WPL>(c, w0, in, pu[i], acct);
       }
     }
   }
-  uint64_t p0 = 0, p1 = 0, p2 = 0, u0 = 0, u1 = 0, u2 = 0;
+  uint64_t p0[WPL], p1[WPL], p2[WPL], u0[WPL], u1[WPL], u2[WPL];
+#pragma unroll
+  for (int k = 0; k < WPL; ++k) p0[k] = p1[k] = p2[k] = u0[k] = u1[k] = u2[k] = 0;
 #pragma unroll
   for (uint32_t i = 0; i < MBX_MAXNINSTR; ++i) {
     if (i >= P.nins) break;
     const uint32_t op = P.ins[i][0], d = P.ins[i][1], s = P.ins[i][2];
-    const uint64_t pd = rget(p0, p1, p2, d);
+    uint64_t pd[WPL];
+#pragma unroll
+    for (int k = 0; k < WPL; ++k) pd[k] = rget(p0[k], p1[k], p2[k], d);
     if (op == MBX_AND) {
-      const uint64_t np = pd & rget(p0, p1, p2, s);
-      rset(p0, p1, p2, d, np);
-      if (CHK)
-        rset(u0, u1, u2, d,
-             (rget(u0, u1, u2, d) | rget(u0, u1, u2, s)) & np);
+#pragma unroll
+      for (int k = 0; k < WPL; ++k) {
+        const uint64_t np = pd[k] & rget(p0[k], p1[k], p2[k], s);
+        rset(p0[k], p1[k], p2[k], d, np);
+        if (CHK)
+          rset(u0[k], u1[k], u2[k], d,
+               (rget(u0[k], u1[k], u2[k], d) | rget(u0[k], u1[k], u2[k], s)) & np);
+      }
     } else if (op == MBX_OR) {
-      const uint64_t ps = rget(p0, p1, p2, s);
-      if (CHK) {
-        const uint64_t us = rget(u0, u1, u2, s), cs = ps & ~us;
-        rset(u0, u1, u2, d, (rget(u0, u1, u2, d) & ~cs) | (us & ~pd));
-      }
-      rset(p0, p1, p2, d, pd | ps);
-    } else if (op == MBX_NOT) {
-      const uint64_t ps = rget(p0, p1, p2, s);
-      if (CHK) {
-        const uint64_t us = rget(u0, u1, u2, s);
-        rset(p0, p1, p2, d, ~ps | us);
-        rset(u0, u1, u2, d, us);
-      } else {
-        rset(p0, p1, p2, d, ~ps);
-      }
-    } else if (op == MBX_ANDM) {
-      const DevCol &c = P.cols[s];
-      if (EAGER || pd != 0) {
+#pragma unroll
+      for (int k = 0; k < WPL; ++k) {
+        const uint64_t ps = rget(p0[k], p1[k], p2[k], s);
         if (CHK) {
-          const uint64_t us = EAGER ? pu[i] : memop_u<ACCT>(c, w, acct);
-          const uint64_t cs = EAGER ? pc[i] : memop_c<ACCT>(c, w, acct);
-          const uint64_t np = pd & (us | cs);
-          rset(p0, p1, p2, d, np);
-          rset(u0, u1, u2, d, (rget(u0, u1, u2, d) | us) & np);
+          const uint64_t us = rget(u0[k], u1[k], u2[k], s), cs = ps & ~us;
+          rset(u0[k], u1[k], u2[k], d,
+               (rget(u0[k], u1[k], u2[k], d) & ~cs) | (us & ~pd[k]));
+        }
+        rset(p0[k], p1[k], p2[k], d, pd[k] | ps);
+      }
+    } else if (op == MBX_NOT) {
+#pragma unroll
+      for (int k = 0; k < WPL; ++k) {
+        const uint64_t ps = rget(p0[k], p1[k], p2[k], s);
+        if (CHK) {
+          const uint64_t us = rget(u0[k], u1[k], u2[k], s);
+          rset(p0[k], p1[k], p2[k], d, ~ps | us);
+          rset(u0[k], u1[k], u2[k], d, us);
         } else {
-          rset(p0, p1, p2, d, pd & (EAGER ? pc[i] : memop_c<ACCT>(c, w, acct)));
+          rset(p0[k], p1[k], p2[k], d, ~ps);
         }
       }
-    } else if (op == MBX_ORM) {
+    } else if (op == MBX_ANDM || op == MBX_ORM) {
       const DevCol &c = P.cols[s];
-      const uint64_t cs = EAGER ? pc[i] : memop_c<ACCT>(c, w, acct);
-      if (CHK) {
-        const uint64_t us = EAGER ? pu[i] : memop_u<ACCT>(c, w, acct);
-        rset(u0, u1, u2, d, (rget(u0, u1, u2, d) & ~cs) | (us & ~pd));
-        rset(p0, p1, p2, d, pd | us | cs);
+      uint64_t cs[WPL], us[WPL];
+      if (EAGER) {
+#pragma unroll
+        for (int k = 0; k < WPL; ++k) {
+          cs[k] = pc[i][k];
+          us[k] = pu[i][k];
+        }
       } else {
-        rset(p0, p1, p2, d, pd | cs);
+        bool ld[WPL];
+#pragma unroll
+        for (int k = 0; k < WPL; ++k)
+          ld[k] = in[k] && (op == MBX_ORM || pd[k] != 0);
+        memop_c<ACCT, WPL>(c, w0, ld, cs, acct);
+        if (CHK) memop_u<ACCT, WPL>(c, w0, ld, us, acct);
+        else {
+#pragma unroll
+          for (int k = 0; k < WPL; ++k) us[k] = 0;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < WPL; ++k) {
+        if (op == MBX_ANDM) {
+          const uint64_t np = pd[k] & (us[k] | cs[k]);
+          rset(p0[k], p1[k], p2[k], d, np);
+          if (CHK)
+            rset(u0[k], u1[k], u2[k], d, (rget(u0[k], u1[k], u2[k], d) | us[k]) & np);
+        } else {
+          if (CHK)
+            rset(u0[k], u1[k], u2[k], d,
+                 (rget(u0[k], u1[k], u2[k], d) & ~cs[k]) | (us[k] & ~pd[k]));
+          rset(p0[k], p1[k], p2[k], d, pd[k] | us[k] | cs[k]);
+        }
       }
     }
   }
   // rows >= N never survive (the reference has no such guard, primitive.cuh
-  // :357-360 only bounds the word index)
-  if (w == P.nwords - 1 && (P.nrows & 63u)) {
-    const uint64_t tail = (1ull << (P.nrows & 63u)) - 1ull;
-    p0 &= tail;
-    u0 &= tail;
+  // :357-360 only bounds the word index); words past the end are empty
+#pragma unroll
+  for (int k = 0; k < WPL; ++k) {
+    const uint32_t w = w0 + 64u * k;
+    uint64_t keep = in[k] ? ~0ull : 0ull;
+    if (w == P.nwords - 1 && (P.nrows & 63u)) keep = (1ull << (P.nrows & 63u)) - 1ull;
+    po[k] = p0[k] & keep;
+    uo[k] = CHK ? (u0[k] & keep) : 0;
   }
-  po = p0;
-  uo = CHK ? u0 : 0;
+}
+
+template <bool CHK, bool ACCT, bool EAGER = false>
+__device__ __forceinline__ void eval_word(const DevPrg &P, uint32_t w,
+                                          uint64_t &po, uint64_t &uo,
+                                          const Acct *acct) {
+  uint64_t p[1], u[1];
+  eval_words<CHK, ACCT, EAGER, 1>(P, w, p, u, acct);
+  po = p[0];
+  uo = u[0];
 }
 
 // ----------------------------------------------------------- row operation
@@ -358,6 +411,66 @@ __device__ __forceinline__ void sink_add(const GroupSink &s, uint32_t key,
   else atomicAdd(s.out + key, (unsigned long long)val);
 }
 
+// ------------------------------------------------------------ table flush
+// Device-scope atomics on one address serialise (about 12 ns apiece on
+// MI355X), so ~2000 persistent workgroups adding their LDS tables straight
+// into the output spend tens of microseconds on each hot group.  Instead
+// member m of nmembers adds its non-zero groups into replica m % R of the
+// table (R = min(nmembers, kSpreadRep), so about nmembers / R adds per
+// address); one counter per replica finds the replica's last member, which
+// moves the replica into the output (R adds per output address) and leaves
+// replica and counter zero for the next launch.  Callers: every workgroup of
+// the grid, with its LDS table complete; nmembers must be the same in all.
+//
+// Ordering uses returning device-scope atomics only: a returned atomic has
+// been performed at the device's coherence point, so a member's adds are
+// performed before its counter increment is issued, and the last member's
+// exchanges, issued after it saw the final count, read every add.  No
+// agent-scope fence is needed (on gfx950 one writes back and invalidates the
+// L2, which costs far more than the atomics it would order).
+struct Spread {
+  unsigned long long *rep;   // kSpreadRep tables of gpad int64, zero on entry
+  uint32_t *done;            // kSpreadRep counters, kSpreadDoneStride apart
+  uint32_t gpad;             // table stride: nr_grp rounded up to 16
+};
+
+__device__ __forceinline__ void flush_groups(const Spread &sp,
+                                             unsigned long long *out,
+                                             uint32_t nr_grp, uint32_t member,
+                                             uint32_t nmembers) {
+  __shared__ uint32_t s_last;
+  const uint32_t R = nmembers < kSpreadRep ? nmembers : kSpreadRep;
+  const uint32_t r = member % R;
+  unsigned long long *rep = sp.rep + (size_t)r * sp.gpad;
+  __syncthreads();
+  unsigned long long seen = 0;
+  for (uint32_t i = threadIdx.x; i < nr_grp; i += blockDim.x) {
+    const unsigned long long v = s_grp_dyn[i];
+    if (v)
+      seen |= __hip_atomic_fetch_add(rep + i, v, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT);
+  }
+  asm volatile("" ::"v"(seen));   // every add has returned
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t members = nmembers / R + (r < nmembers % R ? 1u : 0u);
+    const uint32_t before = __hip_atomic_fetch_add(
+        sp.done + (size_t)r * kSpreadDoneStride, 1u, __ATOMIC_RELAXED,
+        __HIP_MEMORY_SCOPE_AGENT);
+    s_last = before == members - 1;
+  }
+  __syncthreads();
+  if (!s_last) return;
+  for (uint32_t i = threadIdx.x; i < nr_grp; i += blockDim.x) {
+    const unsigned long long v = __hip_atomic_exchange(
+        rep + i, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (v) atomicAdd(out + i, v);
+  }
+  if (threadIdx.x == 0)
+    (void)__hip_atomic_exchange(sp.done + (size_t)r * kSpreadDoneStride, 0u,
+                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // List entries: row id, MSB = uncertain (primitive.cuh:406-407).  E entries
 // per lane share their memory round trips.
 template <bool CHK, bool ACCT, int E>
@@ -448,15 +561,10 @@ __device__ __forceinline__ void run_wave_tile(const DevPrg &P, const DevOp &O,
                                               const Acct *acct) {
   const uint32_t lane = lane_id();
   uint64_t p[WPL], u[WPL];
+  eval_words<CHK, ACCT, EAGER, WPL>(P, wt * WPL * 64 + lane, p, u, acct);
   uint32_t cnt = 0;
 #pragma unroll
-  for (int k = 0; k < WPL; ++k) {
-    const uint32_t w = (wt * WPL + k) * 64 + lane;
-    p[k] = 0;
-    u[k] = 0;
-    if (w < P.nwords) eval_word<CHK, ACCT, EAGER>(P, w, p[k], u[k], acct);
-    cnt += __popcll(p[k]);
-  }
+  for (int k = 0; k < WPL; ++k) cnt += __popcll(p[k]);
   const uint32_t incl = wave_incl_scan(cnt);
   const uint32_t total = __shfl(incl, 63, 64);
   const uint32_t off = incl - cnt;

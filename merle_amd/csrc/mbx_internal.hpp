@@ -76,6 +76,11 @@ struct mbx_ctx {
   mbx::Pool *pool = nullptr;
   // scratch reused by the plans (one query at a time per context)
   uint32_t *d_counter = nullptr;   // 4 uint32 work/counter slots
+  // replicated flush targets of the LDS group tables: kSpreadRep tables of
+  // kLdsGroups int64 and kSpreadRep counters 128 B apart; all zero between
+  // launches (the last workgroup of each replica clears what it used)
+  unsigned long long *spread = nullptr;
+  uint32_t *spread_done = nullptr;
 };
 
 struct mbx_timer {
@@ -138,5 +143,8 @@ int launch_fill_u64(hipStream_t s, void *dst, uint64_t v, size_t count);
 constexpr uint32_t kLdsGroups = 4096;
 // Row-list capacity of one compaction round (uint32 entries).
 constexpr uint32_t kListCap = 2048;
+// Replicas of the group table that workgroups flush into (see flush_groups).
+constexpr uint32_t kSpreadRep = 64;
+constexpr uint32_t kSpreadDoneStride = 32;   // uint32 counters, 128 B apart
 
 }  // namespace mbx

@@ -20,13 +20,13 @@ struct QArgs {
   uint32_t nr_grp;
   uint32_t ntiles;
   const Acct *acct;
+  Spread sp;
 };
 
 // ------------------------------------------------------------ fused plan
-// Persistent: each workgroup walks tiles blockIdx.x, +gridDim.x, ... and
-// keeps its group table in LDS until the end, so the HBM output sees at most
-// one atomic per (workgroup, non-empty group) -- never more than the rows the
-// workgroup aggregated.
+// Persistent: wave v of workgroup b walks wave tiles 4b+v, +4*gridDim.x, ...
+// and the workgroup keeps its group table in LDS until the end; the table
+// then goes out through the replicated flush (flush_groups).
 template <bool CHK, bool LDSG, bool ACCT, int WPL, bool EAGER>
 __global__ __launch_bounds__(256) void fused_kernel(QArgs a) {
   __shared__ uint32_t s_list[4][kWaveList];
@@ -40,13 +40,7 @@ __global__ __launch_bounds__(256) void fused_kernel(QArgs a) {
   for (uint32_t t = blockIdx.x * 4 + wv; t < a.ntiles; t += gridDim.x * 4)
     run_wave_tile<CHK, ACCT, WPL, EAGER>(a.prg, a.op, a.nr_grp, t, sink,
                                          s_list[wv], a.acct);
-  if (LDSG) {
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < a.nr_grp; i += 256) {
-      const unsigned long long v = s_grp_dyn[i];
-      if (v) atomicAdd(a.out + i, v);
-    }
-  }
+  if (LDSG) flush_groups(a.sp, a.out, a.nr_grp, blockIdx.x, gridDim.x);
 }
 
 // ------------------------------------------------------------- scan plan
@@ -68,13 +62,7 @@ __global__ __launch_bounds__(256) void scan_kernel(QArgs a) {
     if (row_keyval<false>(a.op, row, a.nr_grp, key, val, nullptr))
       sink_add(sink, key, val, a.nr_grp);
   }
-  if (LDSG) {
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < a.nr_grp; i += 256) {
-      const unsigned long long v = s_grp_dyn[i];
-      if (v) atomicAdd(a.out + i, v);
-    }
-  }
+  if (LDSG) flush_groups(a.sp, a.out, a.nr_grp, blockIdx.x, gridDim.x);
 }
 
 // ------------------------------------------------------- materialise plan
@@ -92,39 +80,60 @@ __global__ __launch_bounds__(256) void stage1_kernel(DevPrg P,
   }
 }
 
-// Stage 2: popcount -> block scan -> one atomicAdd per workgroup for its
-// segment of the global row list (primitive_abla.cuh:290-354).
+// Stage 2: popcount -> list offsets -> row list (primitive_abla.cuh:290-354).
+// Workgroup b owns the contiguous chunks [b*cpw, (b+1)*cpw) of 256 x WPL
+// words: a first pass counts its rows, one atomicAdd reserves its segment of
+// the list (one per workgroup of a grid of ~num_cus workgroups, so few adds
+// meet on the counter), and a second pass -- its reads mostly from L2 --
+// block-scans each chunk and writes the row ids in row order.
 template <bool CHK, int WPL>
 __global__ __launch_bounds__(256) void stage2_kernel(
     const uint64_t *__restrict__ po, const uint64_t *__restrict__ uo,
-    uint32_t nwords, uint32_t *__restrict__ rows, uint32_t *count) {
+    uint32_t nwords, uint32_t *__restrict__ rows, uint32_t *count,
+    uint32_t cpw) {
   __shared__ uint32_t s_wave[4];
   __shared__ uint32_t s_base;
   const uint32_t tid = threadIdx.x;
-  uint64_t p[WPL], u[WPL];
-  uint32_t cnt = 0;
+  const uint32_t nchunks = (nwords + 256 * WPL - 1) / (256 * WPL);
+  const uint32_t c0 = blockIdx.x * cpw;
+  const uint32_t c1 = c0 + cpw < nchunks ? c0 + cpw : nchunks;
+  uint32_t mine = 0;
+  for (uint32_t c = c0; c < c1; ++c)
 #pragma unroll
-  for (int k = 0; k < WPL; ++k) {
-    const uint32_t w = (blockIdx.x * WPL + k) * 256 + tid;
-    p[k] = w < nwords ? po[w] : 0;
-    u[k] = (CHK && w < nwords) ? uo[w] : 0;
-    cnt += __popcll(p[k]);
-  }
+    for (int k = 0; k < WPL; ++k) {
+      const uint32_t w = (c * WPL + k) * 256 + tid;
+      if (w < nwords) mine += __popcll(po[w]);
+    }
   uint32_t total;
-  uint32_t off = block_excl_scan(cnt, s_wave, &total);
+  (void)block_excl_scan(mine, s_wave, &total);
   if (tid == 0) s_base = total ? atomicAdd(count, total) : 0;
   __syncthreads();
-  off += s_base;
+  if (total == 0) return;   // workgroup-uniform
+  uint32_t base = s_base;
+  for (uint32_t c = c0; c < c1; ++c) {
+    uint64_t p[WPL], u[WPL];
+    uint32_t cnt = 0;
 #pragma unroll
-  for (int k = 0; k < WPL; ++k) {
-    uint64_t m = p[k];
-    const uint32_t row0 = ((blockIdx.x * WPL + k) * 256 + tid) * 64;
-    while (m) {
-      const uint32_t b = (uint32_t)__builtin_ctzll(m);
-      m &= m - 1;
-      uint32_t e = row0 + b;
-      if (CHK && ((u[k] >> b) & 1ull)) e |= 0x80000000u;
-      rows[off++] = e;
+    for (int k = 0; k < WPL; ++k) {
+      const uint32_t w = (c * WPL + k) * 256 + tid;
+      p[k] = w < nwords ? po[w] : 0;
+      u[k] = (CHK && w < nwords) ? uo[w] : 0;
+      cnt += __popcll(p[k]);
+    }
+    uint32_t ctot;
+    uint32_t off = base + block_excl_scan(cnt, s_wave, &ctot);
+    base += ctot;
+#pragma unroll
+    for (int k = 0; k < WPL; ++k) {
+      uint64_t m = p[k];
+      const uint32_t row0 = ((c * WPL + k) * 256 + tid) * 64;
+      while (m) {
+        const uint32_t b = (uint32_t)__builtin_ctzll(m);
+        m &= m - 1;
+        uint32_t e = row0 + b;
+        if (CHK && ((u[k] >> b) & 1ull)) e |= 0x80000000u;
+        rows[off++] = e;
+      }
     }
   }
 }
@@ -145,13 +154,7 @@ __global__ __launch_bounds__(256) void stage3_kernel(
   for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j < n;
        j += gridDim.x * 256)
     process_entry<CHK, false>(a.prg, a.op, a.nr_grp, rows[j], sink, nullptr);
-  if (LDSG) {
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < a.nr_grp; i += 256) {
-      const unsigned long long v = s_grp_dyn[i];
-      if (v) atomicAdd(a.out + i, v);
-    }
-  }
+  if (LDSG) flush_groups(a.sp, a.out, a.nr_grp, blockIdx.x, gridDim.x);
 }
 
 // Popcount of accounting bitsets: lines[e] = set bits of entry e.
@@ -227,6 +230,14 @@ static unsigned persistent_grid(mbx_ctx *ctx, uint32_t work, unsigned per_cu) {
   return g ? (unsigned)g : 1u;
 }
 
+static Spread spread_of(mbx_ctx *ctx, const QueryPlan &q) {
+  Spread sp;
+  sp.rep = ctx->spread;
+  sp.done = ctx->spread_done;
+  sp.gpad = (uint32_t)((q.nr_grp + 15) / 16 * 16);
+  return sp;
+}
+
 static size_t grp_lds(const QueryPlan &q) {
   return q.lds ? (size_t)q.nr_grp * 8 : 0;
 }
@@ -268,6 +279,7 @@ int launch_fused(mbx_ctx *ctx, const QueryPlan &q, const Acct *acct) {
   a.nr_grp = q.nr_grp;
   a.ntiles = (uint32_t)ceil_div(q.prg.nwords, 64 * kWPL);   // wave tiles
   a.acct = acct;
+  a.sp = spread_of(ctx, q);
   const size_t lds = acct ? 0 : grp_lds(q);
   const bool chk = q.prg.chk != 0;
   // accounting runs the early-out form: the compulsory-line model
@@ -302,6 +314,7 @@ int launch_scan(mbx_ctx *ctx, const QueryPlan &q) {
   a.op = q.op;
   a.out = q.out;
   a.nr_grp = q.nr_grp;
+  a.sp = spread_of(ctx, q);
   const unsigned grid =
       persistent_grid(ctx, (uint32_t)ceil_div(q.prg.nrows, 256), 8);
   if (q.lds)
@@ -331,13 +344,18 @@ int launch_stage2(mbx_ctx *ctx, const uint64_t *po, const uint64_t *uo,
                   uint32_t nwords, uint32_t *rows, uint32_t *count) {
   int st = launch_fill_u32(ctx->stream, count, 0, 1);
   if (st) return st;
-  const unsigned grid = (unsigned)ceil_div(nwords, 256 * kWPL);
+  const uint32_t nchunks = (uint32_t)ceil_div(nwords, 256 * kWPL);
+  const uint32_t g0 = nchunks < (uint32_t)ctx->num_cus ? nchunks : ctx->num_cus;
+  const uint32_t cpw = (uint32_t)ceil_div(nchunks, g0 ? g0 : 1);
+  const unsigned grid = cpw ? (unsigned)ceil_div(nchunks, cpw) : 1u;
   if (uo)
     hipLaunchKernelGGL((stage2_kernel<true, kWPL>), dim3(grid ? grid : 1),
-                       dim3(256), 0, ctx->stream, po, uo, nwords, rows, count);
+                       dim3(256), 0, ctx->stream, po, uo, nwords, rows, count,
+                       cpw);
   else
     hipLaunchKernelGGL((stage2_kernel<false, kWPL>), dim3(grid ? grid : 1),
-                       dim3(256), 0, ctx->stream, po, uo, nwords, rows, count);
+                       dim3(256), 0, ctx->stream, po, uo, nwords, rows, count,
+                       cpw);
   MBX_HIP(hipGetLastError());
   return MBX_OK;
 }
@@ -350,6 +368,7 @@ int launch_stage3(mbx_ctx *ctx, const QueryPlan &q, const uint32_t *rows,
   a.op = q.op;
   a.out = q.out;
   a.nr_grp = q.nr_grp;
+  a.sp = spread_of(ctx, q);
   const unsigned grid = (unsigned)ctx->num_cus * 4;
   const size_t lds = grp_lds(q);
 #define MBX_S3(C, L)                                                           \

@@ -98,7 +98,9 @@ def test_admission_by_memory(dev, tables):
     qs = _queries(sch, bins, specs)
     alone = _alone(dev, qs, specs)
     arrivals = np.zeros(len(specs))           # everything arrives at once
-    budget = 200 * 1024                       # < 40,000 x 8 bytes
+    # 64 KiB: below the 320,000 B of the 40,000-group request and below the
+    # 86,528 B the other 39 outputs need together, above any one of them
+    budget = 64 * 1024
     cfg = abi.ServeCfg(2, 4, budget, 0, 0, abi.Tune())
     mixed, lat, st = _replay(dev, qs, arrivals, cfg, specs)
     assert lat[3] < 0 and st.rejected == 1
