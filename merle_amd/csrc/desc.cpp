@@ -84,8 +84,25 @@ int make_dev_prg(const mbx_vprg *p, DevPrg *out) {
     out->ins[n][1] = in.dstreg;
     out->ins[n][2] = in.src1;
     out->ins[n][3] = in.src2;
+    if (mem) out->memcols |= 1u << in.src1;
   }
   out->nins = n;
+  for (uint32_t c = 0; c < MBX_MAXCOLS; ++c) {
+    out->beg[c] = (uint8_t)out->nfl;
+    if ((out->memcols >> c) & 1u) {
+      const DevCol &d = out->cols[c];
+      auto add = [&](const uint64_t *b, bool edge) {
+        out->fl[out->nfl] = b;
+        out->flcol[out->nfl] = (uint8_t)c;
+        out->fledge[out->nfl] = edge ? 1 : 0;
+        ++out->nfl;
+      };
+      for (uint32_t k = 0; k < d.nm; ++k) add(d.m[k], false);
+      if (d.l) add(d.l, true);
+      if (d.r) add(d.r, true);   // an empty column reads as all-zero
+    }
+    out->end[c] = (uint8_t)out->nfl;
+  }
   out->nrows = p->factsz;
   out->nwords = (uint32_t)ceil_div(p->factsz, 64);
   out->chk = chk ? 1u : 0u;

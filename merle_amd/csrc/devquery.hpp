@@ -79,6 +79,15 @@ __device__ __forceinline__ void rset(uint64_t &a, uint64_t &b, uint64_t &c,
   c = i == 2 ? v : c;
 }
 
+__device__ __forceinline__ uint64_t sel4(uint64_t a, uint64_t b, uint64_t c,
+                                         uint64_t d, uint32_t i) {
+  uint64_t r = d;
+  r = i == 2 ? c : r;
+  r = i == 1 ? b : r;
+  r = i == 0 ? a : r;
+  return r;
+}
+
 template <bool ACCT>
 __device__ __forceinline__ uint64_t ld_bmp(const uint64_t *p, uint32_t w,
                                            const Acct *acct) {
@@ -86,66 +95,166 @@ __device__ __forceinline__ uint64_t ld_bmp(const uint64_t *p, uint32_t w,
   return ld_stream(p + w);
 }
 
-// Memory operands of one column at the WPL words w0 + 64k (k < WPL) whose
-// lanes are live: c = OR of middle bins, u = leftmost | rightmost
-// (primitive.cuh:216-226, 294-300).  All loads are independent, so the WPL
-// words of an operand cost one memory round trip.
-template <bool ACCT, int WPL>
-__device__ __forceinline__ void memop_c(const DevCol &c, uint32_t w0,
-                                        const bool (&live)[WPL],
-                                        uint64_t (&m)[WPL], const Acct *acct) {
-#pragma unroll
-  for (int k = 0; k < WPL; ++k) m[k] = 0;
-  for (uint32_t b = 0; b < c.nm; ++b)
-#pragma unroll
-    for (int k = 0; k < WPL; ++k)
-      if (live[k]) m[k] |= ld_bmp<ACCT>(c.m[b], w0 + 64u * k, acct);
-}
-template <bool ACCT, int WPL>
-__device__ __forceinline__ void memop_u(const DevCol &c, uint32_t w0,
-                                        const bool (&live)[WPL],
-                                        uint64_t (&e)[WPL], const Acct *acct) {
+// Loads the operand bitmaps e0 <= e < e1 of the program's flat list at this
+// lane's words w0 + k * ws whose flag ld[k] is set, and ORs them into
+// c[col][k] (middle bins) / u[col][k] (edge bins) -- c = OR of middle bins,
+// u = leftmost | rightmost (primitive.cuh:216-226, 294-300).  kOpGroup
+// bitmaps at a time, every load of a group issued before any is used, so a
+// group costs one memory round trip.  A load is skipped (uniform branch)
+// when no lane of the wave wants its word; otherwise every lane loads, an
+// unwanted lane at the address of the first wanted lane (a line the wave
+// fetches anyway), and its value is dropped.  Loads behind lane-divergent
+// branches would each be waited for inside their branch.
+constexpr int kOpGroup = 4;
+
+template <bool ACCT, bool CHK, int WPL, int NC>
+__device__ __forceinline__ void load_ops(const DevPrg &P, uint32_t e0,
+                                         uint32_t e1, uint32_t w0, uint32_t ws,
+                                         const bool (&ld)[WPL],
+                                         uint64_t (&c)[NC][WPL],
+                                         uint64_t (&u)[NC][WPL],
+                                         const Acct *acct) {
+  uint32_t wa[WPL];
+  bool any[WPL];
 #pragma unroll
   for (int k = 0; k < WPL; ++k) {
-    e[k] = 0;
-    if (live[k] && c.l) e[k] |= ld_bmp<ACCT>(c.l, w0 + 64u * k, acct);
-    if (live[k] && c.r) e[k] |= ld_bmp<ACCT>(c.r, w0 + 64u * k, acct);
+    const uint32_t w = w0 + ws * k;
+    const uint64_t m = __ballot(ld[k]);
+    any[k] = m != 0;
+    const uint32_t wf = __shfl(w, any[k] ? (int)__builtin_ctzll(m) : 0, 64);
+    wa[k] = ld[k] ? w : wf;
+  }
+  for (uint32_t g = e0; g < e1; g += kOpGroup) {
+    uint64_t v[kOpGroup][WPL];
+#pragma unroll
+    for (int j = 0; j < kOpGroup; ++j) {
+#pragma unroll
+      for (int k = 0; k < WPL; ++k) v[j][k] = 0;
+      if (g + j < e1) {
+        const uint64_t *b = P.fl[g + j];
+#pragma unroll
+        for (int k = 0; k < WPL; ++k)
+          if (any[k]) {
+            if (ACCT && ld[k]) mark<true>(acct, b + wa[k]);
+            v[j][k] = ld_stream(b + wa[k]);
+          }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kOpGroup; ++j) {
+      if (g + j >= e1) break;
+      const uint32_t col = P.flcol[g + j];
+      const bool edge = P.fledge[g + j] != 0;
+#pragma unroll
+      for (int k = 0; k < WPL; ++k) {
+        const uint64_t x = ld[k] ? v[j][k] : 0ull;
+#pragma unroll
+        for (int cc = 0; cc < NC; ++cc) {
+          const bool here = NC == 1 || col == (uint32_t)cc;
+          c[cc][k] |= (here && !edge) ? x : 0ull;
+          if (CHK) u[cc][k] |= (here && edge) ? x : 0ull;
+        }
+      }
+    }
   }
 }
 
-// Evaluates the program on the WPL 64-row words w0 + 64k of this lane.
-// nochk form: primitive.cuh:201-252; chk form: primitive.cuh:255-332, with
-// the three-valued NOT corrected to P' = ~P_s | U_s, U' = U_s (the
-// reference's P' = ~P_s drops uncertain rows).
-// EAGER = false keeps the ANDM early-out of primitive.cuh:215/292: a word
-// whose destination is already empty loads nothing, so a 128-byte line of
-// the operand is fetched only if one of its 16 words is live -- at the cost
-// of one memory round trip per memory instruction.  EAGER = true issues
-// every operand load of the program up front (one round trip in all) and
-// combines afterwards; the plan picks it when the early-out would skip
-// almost no lines.  Both give the same bits: an empty destination has no
-// uncertain rows (U is a subset of P after every instruction), so ANDM on it
-// yields the same empty pair whether or not the operand is read.
+// One instruction on one 64-row word; (cs, us) is the memory operand of
+// ANDM / ORM (certain middle bins, uncertain edge bins).  nochk form:
+// primitive.cuh:201-252; chk form: primitive.cuh:255-332, with the
+// three-valued NOT corrected to P' = ~P_s | U_s, U' = U_s (the reference's
+// P' = ~P_s drops uncertain rows).
+template <bool CHK>
+__device__ __forceinline__ void apply_ins(uint32_t op, uint32_t d, uint32_t s,
+                                          uint64_t cs, uint64_t us,
+                                          uint64_t &p0, uint64_t &p1,
+                                          uint64_t &p2, uint64_t &u0,
+                                          uint64_t &u1, uint64_t &u2) {
+  const uint64_t pd = rget(p0, p1, p2, d);
+  if (op == MBX_AND) {
+    const uint64_t np = pd & rget(p0, p1, p2, s);
+    rset(p0, p1, p2, d, np);
+    if (CHK)
+      rset(u0, u1, u2, d, (rget(u0, u1, u2, d) | rget(u0, u1, u2, s)) & np);
+  } else if (op == MBX_OR) {
+    const uint64_t ps = rget(p0, p1, p2, s);
+    if (CHK) {
+      const uint64_t ust = rget(u0, u1, u2, s), cst = ps & ~ust;
+      rset(u0, u1, u2, d, (rget(u0, u1, u2, d) & ~cst) | (ust & ~pd));
+    }
+    rset(p0, p1, p2, d, pd | ps);
+  } else if (op == MBX_NOT) {
+    const uint64_t ps = rget(p0, p1, p2, s);
+    if (CHK) {
+      const uint64_t ust = rget(u0, u1, u2, s);
+      rset(p0, p1, p2, d, ~ps | ust);
+      rset(u0, u1, u2, d, ust);
+    } else {
+      rset(p0, p1, p2, d, ~ps);
+    }
+  } else if (op == MBX_ANDM) {
+    const uint64_t np = pd & (us | cs);
+    rset(p0, p1, p2, d, np);
+    if (CHK) rset(u0, u1, u2, d, (rget(u0, u1, u2, d) | us) & np);
+  } else if (op == MBX_ORM) {
+    if (CHK) rset(u0, u1, u2, d, (rget(u0, u1, u2, d) & ~cs) | (us & ~pd));
+    rset(p0, p1, p2, d, pd | us | cs);
+  }
+}
+
+// Evaluates the program on this lane's words w0 + k * ws (k < WPL, word <
+// wend).  EAGER = false keeps the ANDM early-out of primitive.cuh:215/292: a
+// word whose destination is already empty loads nothing, so a 128-byte line
+// of the operand is fetched only if one of its 16 words is live -- at the
+// cost of one memory round trip per memory instruction (the WPL words of an
+// instruction share it).  EAGER = true loads the operand of every column a
+// memory instruction reads up front (one round trip in all), then runs the
+// program word by word, so the register state is not replicated per word.
+// Both give the same bits: an empty destination has no uncertain rows (U is
+// a subset of P after every instruction), so ANDM on it yields the same
+// empty pair whether or not the operand is read.
 template <bool CHK, bool ACCT, bool EAGER, int WPL>
 __device__ __forceinline__ void eval_words(const DevPrg &P, uint32_t w0,
+                                           uint32_t ws, uint32_t wend,
                                            uint64_t (&po)[WPL],
                                            uint64_t (&uo)[WPL],
                                            const Acct *acct) {
   bool in[WPL];
+  uint64_t keep[WPL];
 #pragma unroll
-  for (int k = 0; k < WPL; ++k) in[k] = w0 + 64u * k < P.nwords;
-  uint64_t pc[MBX_MAXNINSTR][WPL], pu[MBX_MAXNINSTR][WPL];
+  for (int k = 0; k < WPL; ++k) {
+    const uint32_t w = w0 + ws * k;
+    in[k] = w < wend;
+    // rows >= N never survive (the reference has no such guard,
+    // primitive.cuh:357-360 only bounds the word index)
+    keep[k] = in[k] ? ~0ull : 0ull;
+    if (in[k] && w == P.nwords - 1 && (P.nrows & 63u))
+      keep[k] = (1ull << (P.nrows & 63u)) - 1ull;
+  }
   if (EAGER) {
+    uint64_t pc[MBX_MAXCOLS][WPL], pu[MBX_MAXCOLS][WPL];
 #pragma unroll
-    for (uint32_t i = 0; i < MBX_MAXNINSTR; ++i) {
+    for (uint32_t c = 0; c < MBX_MAXCOLS; ++c)
 #pragma unroll
-      for (int k = 0; k < WPL; ++k) pc[i][k] = pu[i][k] = 0;
-      if (i < P.nins && (P.ins[i][0] == MBX_ANDM || P.ins[i][0] == MBX_ORM)) {
-        const DevCol &c = P.cols[P.ins[i][2]];
-        memop_c<ACCT, WPL>(c, w0, in, pc[i], acct);
-        if (CHK) memop_u<ACCT, WPL>(c, w0, in, pu[i], acct);
+      for (int k = 0; k < WPL; ++k) pc[c][k] = pu[c][k] = 0;
+    load_ops<ACCT, CHK, WPL, MBX_MAXCOLS>(P, 0, P.nfl, w0, ws, in, pc, pu,
+                                          acct);
+#pragma unroll
+    for (int k = 0; k < WPL; ++k) {
+      uint64_t p0 = 0, p1 = 0, p2 = 0, u0 = 0, u1 = 0, u2 = 0;
+#pragma unroll
+      for (uint32_t i = 0; i < MBX_MAXNINSTR; ++i) {
+        if (i >= P.nins) break;
+        const uint32_t op = P.ins[i][0], d = P.ins[i][1], s = P.ins[i][2];
+        const uint64_t cs = sel4(pc[0][k], pc[1][k], pc[2][k], pc[3][k], s);
+        const uint64_t us =
+            CHK ? sel4(pu[0][k], pu[1][k], pu[2][k], pu[3][k], s) : 0;
+        apply_ins<CHK>(op, d, s, cs, us, p0, p1, p2, u0, u1, u2);
       }
+      po[k] = p0 & keep[k];
+      uo[k] = CHK ? (u0 & keep[k]) : 0;
     }
+    return;
   }
   uint64_t p0[WPL], p1[WPL], p2[WPL], u0[WPL], u1[WPL], u2[WPL];
 #pragma unroll
@@ -154,87 +263,26 @@ __device__ __forceinline__ void eval_words(const DevPrg &P, uint32_t w0,
   for (uint32_t i = 0; i < MBX_MAXNINSTR; ++i) {
     if (i >= P.nins) break;
     const uint32_t op = P.ins[i][0], d = P.ins[i][1], s = P.ins[i][2];
-    uint64_t pd[WPL];
+    uint64_t cs[1][WPL], us[1][WPL];
 #pragma unroll
-    for (int k = 0; k < WPL; ++k) pd[k] = rget(p0[k], p1[k], p2[k], d);
-    if (op == MBX_AND) {
+    for (int k = 0; k < WPL; ++k) cs[0][k] = us[0][k] = 0;
+    if (op == MBX_ANDM || op == MBX_ORM) {
+      bool ld[WPL];
 #pragma unroll
-      for (int k = 0; k < WPL; ++k) {
-        const uint64_t np = pd[k] & rget(p0[k], p1[k], p2[k], s);
-        rset(p0[k], p1[k], p2[k], d, np);
-        if (CHK)
-          rset(u0[k], u1[k], u2[k], d,
-               (rget(u0[k], u1[k], u2[k], d) | rget(u0[k], u1[k], u2[k], s)) & np);
-      }
-    } else if (op == MBX_OR) {
-#pragma unroll
-      for (int k = 0; k < WPL; ++k) {
-        const uint64_t ps = rget(p0[k], p1[k], p2[k], s);
-        if (CHK) {
-          const uint64_t us = rget(u0[k], u1[k], u2[k], s), cs = ps & ~us;
-          rset(u0[k], u1[k], u2[k], d,
-               (rget(u0[k], u1[k], u2[k], d) & ~cs) | (us & ~pd[k]));
-        }
-        rset(p0[k], p1[k], p2[k], d, pd[k] | ps);
-      }
-    } else if (op == MBX_NOT) {
-#pragma unroll
-      for (int k = 0; k < WPL; ++k) {
-        const uint64_t ps = rget(p0[k], p1[k], p2[k], s);
-        if (CHK) {
-          const uint64_t us = rget(u0[k], u1[k], u2[k], s);
-          rset(p0[k], p1[k], p2[k], d, ~ps | us);
-          rset(u0[k], u1[k], u2[k], d, us);
-        } else {
-          rset(p0[k], p1[k], p2[k], d, ~ps);
-        }
-      }
-    } else if (op == MBX_ANDM || op == MBX_ORM) {
-      const DevCol &c = P.cols[s];
-      uint64_t cs[WPL], us[WPL];
-      if (EAGER) {
-#pragma unroll
-        for (int k = 0; k < WPL; ++k) {
-          cs[k] = pc[i][k];
-          us[k] = pu[i][k];
-        }
-      } else {
-        bool ld[WPL];
-#pragma unroll
-        for (int k = 0; k < WPL; ++k)
-          ld[k] = in[k] && (op == MBX_ORM || pd[k] != 0);
-        memop_c<ACCT, WPL>(c, w0, ld, cs, acct);
-        if (CHK) memop_u<ACCT, WPL>(c, w0, ld, us, acct);
-        else {
-#pragma unroll
-          for (int k = 0; k < WPL; ++k) us[k] = 0;
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < WPL; ++k) {
-        if (op == MBX_ANDM) {
-          const uint64_t np = pd[k] & (us[k] | cs[k]);
-          rset(p0[k], p1[k], p2[k], d, np);
-          if (CHK)
-            rset(u0[k], u1[k], u2[k], d, (rget(u0[k], u1[k], u2[k], d) | us[k]) & np);
-        } else {
-          if (CHK)
-            rset(u0[k], u1[k], u2[k], d,
-                 (rget(u0[k], u1[k], u2[k], d) & ~cs[k]) | (us[k] & ~pd[k]));
-          rset(p0[k], p1[k], p2[k], d, pd[k] | us[k] | cs[k]);
-        }
-      }
+      for (int k = 0; k < WPL; ++k)
+        ld[k] = in[k] && (op == MBX_ORM || rget(p0[k], p1[k], p2[k], d) != 0);
+      load_ops<ACCT, CHK, WPL, 1>(P, P.beg[s], P.end[s], w0, ws, ld, cs, us,
+                                  acct);
     }
+#pragma unroll
+    for (int k = 0; k < WPL; ++k)
+      apply_ins<CHK>(op, d, s, cs[0][k], us[0][k], p0[k], p1[k], p2[k], u0[k],
+                     u1[k], u2[k]);
   }
-  // rows >= N never survive (the reference has no such guard, primitive.cuh
-  // :357-360 only bounds the word index); words past the end are empty
 #pragma unroll
   for (int k = 0; k < WPL; ++k) {
-    const uint32_t w = w0 + 64u * k;
-    uint64_t keep = in[k] ? ~0ull : 0ull;
-    if (w == P.nwords - 1 && (P.nrows & 63u)) keep = (1ull << (P.nrows & 63u)) - 1ull;
-    po[k] = p0[k] & keep;
-    uo[k] = CHK ? (u0[k] & keep) : 0;
+    po[k] = p0[k] & keep[k];
+    uo[k] = CHK ? (u0[k] & keep[k]) : 0;
   }
 }
 
@@ -243,7 +291,7 @@ __device__ __forceinline__ void eval_word(const DevPrg &P, uint32_t w,
                                           uint64_t &po, uint64_t &uo,
                                           const Acct *acct) {
   uint64_t p[1], u[1];
-  eval_words<CHK, ACCT, EAGER, 1>(P, w, p, u, acct);
+  eval_words<CHK, ACCT, EAGER, 1>(P, w, 64u, P.nwords, p, u, acct);
   po = p[0];
   uo = u[0];
 }
@@ -545,23 +593,26 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// One wave tile = 64 lanes x WPL 64-row words: program -> popcount -> wave
-// scan -> the wave's own LDS row list (rounds of kWaveList entries) -> row
-// op -> sink.  Waves never wait for each other (no workgroup barrier), so a
-// CU overlaps the bitmap loads of some waves with the gathers of others.
-// The reference's method 1 / method 2 switch (primitive.cuh:373-397)
-// becomes rounds: a dense tile takes more rounds, each using all lanes.
+// One wave's share: lane l holds the words w0 + l + k * ws (k < WPL) below
+// wend -- WPL chunks of 64 consecutive words, ws apart.  Program -> popcount
+// -> wave scan -> the wave's own LDS row list (rounds of kWaveList entries)
+// -> row op -> sink.
+// Waves never wait for each other (no workgroup barrier), so a CU overlaps
+// the bitmap loads of some waves with the gathers of others.  The
+// reference's method 1 / method 2 switch (primitive.cuh:373-397) becomes
+// rounds: a dense range takes more rounds, each using all lanes.
 constexpr uint32_t kWaveList = kListCap / 4;
 
-template <bool CHK, bool ACCT, int WPL, bool EAGER>
-__device__ __forceinline__ void run_wave_tile(const DevPrg &P, const DevOp &O,
-                                              uint32_t nr_grp, uint32_t wt,
-                                              const GroupSink &sink,
-                                              uint32_t *s_list,
-                                              const Acct *acct) {
+template <bool CHK, bool ACCT, int WPL, bool EAGER, int E = 2>
+__device__ __forceinline__ void run_wave_range(const DevPrg &P, const DevOp &O,
+                                               uint32_t nr_grp, uint32_t w0,
+                                               uint32_t ws, uint32_t wend,
+                                               const GroupSink &sink,
+                                               uint32_t *s_list,
+                                               const Acct *acct) {
   const uint32_t lane = lane_id();
   uint64_t p[WPL], u[WPL];
-  eval_words<CHK, ACCT, EAGER, WPL>(P, wt * WPL * 64 + lane, p, u, acct);
+  eval_words<CHK, ACCT, EAGER, WPL>(P, w0 + lane, ws, wend, p, u, acct);
   uint32_t cnt = 0;
 #pragma unroll
   for (int k = 0; k < WPL; ++k) cnt += __popcll(p[k]);
@@ -575,7 +626,7 @@ __device__ __forceinline__ void run_wave_tile(const DevPrg &P, const DevOp &O,
 #pragma unroll
       for (int k = 0; k < WPL; ++k) {
         uint64_t m = p[k];
-        const uint32_t row0 = ((wt * WPL + k) * 64 + lane) * 64;
+        const uint32_t row0 = (w0 + ws * k + lane) * 64u;
         while (m && pos < end) {
           const uint32_t b = (uint32_t)__builtin_ctzll(m);
           m &= m - 1;
@@ -590,17 +641,31 @@ __device__ __forceinline__ void run_wave_tile(const DevPrg &P, const DevOp &O,
     }
     wave_lds_sync();
     const uint32_t nl = min(kWaveList, total - base);
-    for (uint32_t j = lane; j < nl; j += 128) {
-      uint32_t ent[2];
-      bool valid[2];
-      ent[0] = s_list[j];
-      valid[0] = true;
-      valid[1] = j + 64 < nl;
-      ent[1] = valid[1] ? s_list[j + 64] : 0;
-      process_entries<CHK, ACCT, 2>(P, O, nr_grp, ent, valid, sink, acct);
+    for (uint32_t j = lane; j < nl; j += 64 * E) {
+      uint32_t ent[E];
+      bool valid[E];
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        valid[e] = j + 64u * e < nl;
+        ent[e] = valid[e] ? s_list[j + 64u * e] : 0u;
+      }
+      process_entries<CHK, ACCT, E>(P, O, nr_grp, ent, valid, sink, acct);
     }
     wave_lds_sync();
   }
+}
+
+// Wave tile wt of 64 x WPL words (the batched kernel's unit).
+template <bool CHK, bool ACCT, int WPL, bool EAGER>
+__device__ __forceinline__ void run_wave_tile(const DevPrg &P, const DevOp &O,
+                                              uint32_t nr_grp, uint32_t wt,
+                                              const GroupSink &sink,
+                                              uint32_t *s_list,
+                                              const Acct *acct) {
+  const uint32_t w0 = wt * WPL * 64u;
+  const uint32_t wend = min(w0 + WPL * 64u, P.nwords);
+  run_wave_range<CHK, ACCT, WPL, EAGER>(P, O, nr_grp, w0, 64u, wend, sink,
+                                        s_list, acct);
 }
 
 }  // namespace mbx

@@ -93,6 +93,9 @@ namespace mbx {
 // Kernel-argument forms of mbx_vprg / mbx_rowop: pointers are typed as
 // 64-row words, middle-bin counts and instruction counts are precomputed, so
 // the kernels branch only on wave-uniform values.
+// Operand bitmaps a program can read: 4 columns x (4 middle + 2 edge bins).
+constexpr uint32_t kMaxOperands = MBX_MAXCOLS * (MBX_MAXBIN_PERCOL + 2);
+
 struct DevCol {
   const uint64_t *m[MBX_MAXBIN_PERCOL];
   const uint64_t *l, *r;
@@ -106,6 +109,14 @@ struct DevPrg {
   uint32_t nrows;
   uint32_t nwords;  // ceil(nrows / 64)
   uint32_t chk;     // any edge bin -> three-valued evaluation
+  uint32_t memcols; // bit c: some ANDM/ORM reads column c
+  // Flat list of the operand bitmaps of the columns in memcols, column by
+  // column (middle bins, then edge bins); column c owns [beg[c], end[c]).
+  const uint64_t *fl[kMaxOperands];
+  uint8_t flcol[kMaxOperands];
+  uint8_t fledge[kMaxOperands];
+  uint8_t beg[MBX_MAXCOLS], end[MBX_MAXCOLS];
+  uint32_t nfl;
 };
 struct DevSrc {
   const void *data;

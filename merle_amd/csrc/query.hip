@@ -18,15 +18,18 @@ struct QArgs {
   DevOp op;
   unsigned long long *out;
   uint32_t nr_grp;
-  uint32_t ntiles;
   const Acct *acct;
   Spread sp;
 };
 
 // ------------------------------------------------------------ fused plan
-// Persistent: wave v of workgroup b walks wave tiles 4b+v, +4*gridDim.x, ...
-// and the workgroup keeps its group table in LDS until the end; the table
-// then goes out through the replicated flush (flush_groups).
+// Wave r of the R = 4 * gridDim.x waves takes the 64-word chunks r, r + R,
+// r + 2R, ... -- WPL of them per pass, all loaded at once, and normally one
+// pass: the launch sizes R so that WPL chunks per wave cover the bitmaps.
+// At any moment the waves together read one contiguous window of each
+// bitmap (DRAM-page friendly), not R scattered ranges.  The workgroup keeps
+// its group table in LDS until the end; the table then goes out through the
+// replicated flush (flush_groups).
 template <bool CHK, bool LDSG, bool ACCT, int WPL, bool EAGER>
 __global__ __launch_bounds__(256) void fused_kernel(QArgs a) {
   __shared__ uint32_t s_list[4][kWaveList];
@@ -37,9 +40,12 @@ __global__ __launch_bounds__(256) void fused_kernel(QArgs a) {
     for (uint32_t i = threadIdx.x; i < a.nr_grp; i += 256) s_grp_dyn[i] = 0;
   __syncthreads();
   const uint32_t wv = threadIdx.x >> 6;
-  for (uint32_t t = blockIdx.x * 4 + wv; t < a.ntiles; t += gridDim.x * 4)
-    run_wave_tile<CHK, ACCT, WPL, EAGER>(a.prg, a.op, a.nr_grp, t, sink,
-                                         s_list[wv], a.acct);
+  const uint32_t ws = gridDim.x * 4 * 64;   // words between a wave's chunks
+  for (uint32_t w0 = (blockIdx.x * 4 + wv) * 64; w0 < a.prg.nwords;
+       w0 += ws * WPL)
+    run_wave_range<CHK, ACCT, WPL, EAGER>(a.prg, a.op, a.nr_grp, w0, ws,
+                                          a.prg.nwords, sink, s_list[wv],
+                                          a.acct);
   if (LDSG) flush_groups(a.sp, a.out, a.nr_grp, blockIdx.x, gridDim.x);
 }
 
@@ -261,23 +267,46 @@ static unsigned occupancy(const void *fn, size_t lds) {
   return (unsigned)nb;
 }
 
+// Grid: enough waves that WPL chunks each cover the program's words in one
+// pass, no more than are resident; small WPL is tried first (fewer
+// registers, more resident waves).  Past the resident limit the waves make
+// several passes.
+template <bool C, bool L, bool A, bool E, int W>
+static bool fused_try(mbx_ctx *ctx, const QArgs &a, size_t lds, bool last) {
+  const void *fn = reinterpret_cast<const void *>(&fused_kernel<C, L, A, W, E>);
+  const unsigned per_cu = occupancy(fn, lds);
+  const uint64_t chunks = ceil_div(a.prg.nwords, 64);
+  const uint64_t waves = ceil_div(chunks, W);
+  const uint64_t resident = (uint64_t)ctx->num_cus * per_cu * 4;
+  if (waves > resident && !last) return false;
+  const unsigned grid = (unsigned)ceil_div(waves < resident ? waves : resident, 4);
+  hipLaunchKernelGGL((fused_kernel<C, L, A, W, E>), dim3(grid), dim3(256),
+                     lds, ctx->stream, a);
+  return true;
+}
+
 template <bool C, bool L, bool A, bool E>
 static void fused_launch(mbx_ctx *ctx, const QArgs &a, size_t lds) {
-  const void *fn = reinterpret_cast<const void *>(&fused_kernel<C, L, A, kWPL, E>);
-  const unsigned per_cu = occupancy(fn, lds);
-  const unsigned grid =
-      persistent_grid(ctx, (uint32_t)ceil_div(a.ntiles, 4), per_cu);
-  hipLaunchKernelGGL((fused_kernel<C, L, A, kWPL, E>), dim3(grid), dim3(256),
-                     lds, ctx->stream, a);
+  if (A) {
+    (void)fused_try<C, L, A, E, 2>(ctx, a, lds, true);
+    return;
+  }
+  if (fused_try<C, L, A, E, 2>(ctx, a, lds, false)) return;
+  if (C) {   // three-valued state doubles the registers: stop at 4 words
+    (void)fused_try<C, L, A, E, 4>(ctx, a, lds, true);
+    return;
+  }
+  if (fused_try<C, L, A, E, 4>(ctx, a, lds, false)) return;
+  (void)fused_try<C, L, A, E, C ? 4 : 6>(ctx, a, lds, true);
 }
 
 int launch_fused(mbx_ctx *ctx, const QueryPlan &q, const Acct *acct) {
   QArgs a;
+  std::memset(&a, 0, sizeof(a));
   a.prg = q.prg;
   a.op = q.op;
   a.out = q.out;
   a.nr_grp = q.nr_grp;
-  a.ntiles = (uint32_t)ceil_div(q.prg.nwords, 64 * kWPL);   // wave tiles
   a.acct = acct;
   a.sp = spread_of(ctx, q);
   const size_t lds = acct ? 0 : grp_lds(q);
