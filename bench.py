@@ -31,6 +31,23 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+PROFILE = os.path.join(ROOT, "profiles", "fused_query_config2.md")
+
+
+def profiled_traffic():
+    """Calibrated HBM bytes (FETCH_SIZE x2 + WRITE_SIZE x1, rocprofv3 --pmc)
+    per fused launch of this workload, averaged over the 12 queries, as
+    committed with the profile; None if the profile is absent."""
+    try:
+        with open(PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(float(line.split(":")[1]))
+    except OSError:
+        pass
+    return None
+
+
 def build_index(dev, sch, attrs):
     """One bitmap per attribute value, projected through the fact FK
     (create_bin with fk).  Slabs are allocated first, so the timed region
@@ -251,7 +268,9 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": profiled_traffic(),
+            "alg_bytes_per_launch": int(sum(balg) / len(balg)),
         },
         "index_build": {"ms": round(cms, 2),
                         "GB_per_s": round(cbytes / cms / 1e6, 1)},
