@@ -145,6 +145,238 @@ def cpu_baseline(dev, sch, specs, target_s=12.0):
                        f"{threads} OpenMP threads, {dt:.1f} s)"), R, results
 
 
+def make_bins(dev, sch, bounds_of):
+    """One create_bin_into slab per column -> {col: [bin ptrs]}, [slabs]."""
+    from merle_amd import abi
+    from merle_amd.host import check, stride_words
+    sw = stride_words(sch.nrows)
+    out, slabs = {}, []
+    for col, b in bounds_of.items():
+        nb = len(b) - 1
+        slab = dev.alloc(sw * 4 * nb)
+        ref = abi.ColRef(sch[col].dev.ptr, None, 0, 4)
+        check(dev.lib.mbx_create_bin_into(dev.ctx, C.byref(ref), sch.nrows,
+                                          (C.c_uint64 * nb)(*b[:-1]),
+                                          (C.c_uint64 * nb)(*b[1:]), nb,
+                                          slab.ptr, sw))
+        out[col] = [slab.ptr + j * sw * 4 for j in range(nb)]
+        slabs.append(slab)
+    dev.sync()
+    return out, slabs
+
+
+def query_array(sch, specs, bins, bounds):
+    from merle_amd import abi, configs, spec as S
+    qs = (abi.Query * len(specs))()
+    for i, s in enumerate(specs):
+        qs[i].prg = S.to_vprg(configs.bin_cols(s, bins, bounds), sch.nrows,
+                              s.instrs)
+        qs[i].op = S.to_rowop(s, sch)
+        qs[i].nr_grp = s.nr_grp
+    return qs
+
+
+def plan_queries(dev, qs):
+    """A copy of the query array with the static heuristic plan applied to
+    every query (sparsest column first; eager loads unless the early-out
+    pays), as the batched and serving paths run them."""
+    from merle_amd import abi
+    from merle_amd.host import check
+    out = (abi.Query * len(qs))()
+    C.memmove(out, qs, C.sizeof(out))
+    for i in range(len(qs)):
+        dens = (C.c_double * 4)()
+        check(dev.lib.mbx_col_density(dev.ctx, C.byref(qs[i].prg), dens))
+        plan = abi.Plan()
+        check(dev.lib.mbx_plan_heuristic(C.byref(qs[i].prg), dens, C.byref(plan)))
+        check(dev.lib.mbx_plan_apply(C.byref(qs[i].prg), C.byref(plan),
+                                     C.byref(out[i].prg)))
+        out[i].flags = plan.flags & abi.PLAN_EAGER
+    return out
+
+
+def run_batched(dev, reps=5):
+    """Config 3: 256 queries over the config-1 tables.  Plain = one fused
+    launch per query, back to back on one stream (the reference's
+    one-query-per-launch driver); batched = one persistent launch of all 256,
+    replayed from a hipGraph, launch shape from the autotuner.  Every query's
+    int64 output must be identical."""
+    from merle_amd import abi, configs
+    from merle_amd.host import Timer, check
+    n = configs.C1_ROWS
+    sch = configs.config1_device(dev, n)
+    bins, slabs = make_bins(dev, sch, {c: configs.C1_BOUNDS for c in configs.C1_COLS})
+    specs = configs.config3_queries(256)
+    qs = query_array(sch, specs, bins, configs.C1_BOUNDS)
+    offs = np.cumsum([0] + [q.nr_grp for q in specs])
+    plain_out = dev.alloc(8 * int(offs[-1]), np.int64, int(offs[-1]))
+    batch_out = dev.alloc(8 * int(offs[-1]), np.int64, int(offs[-1]))
+    for i in range(len(specs)):
+        qs[i].d_out = batch_out.ptr + 8 * int(offs[i])
+    t = Timer(dev)
+
+    pq = plan_queries(dev, qs)
+
+    def plain(arr):
+        check(dev.lib.mbx_fill_u32(dev.ctx, plain_out.ptr, 0, 2 * int(offs[-1])))
+        for i in range(len(specs)):
+            check(dev.lib.mbx_prog_grpby(dev.ctx, C.byref(arr[i].prg),
+                                         C.byref(arr[i].op), arr[i].nr_grp,
+                                         plain_out.ptr + 8 * int(offs[i]),
+                                         abi.PLAN_FUSED | abi.ACCUMULATE |
+                                         arr[i].flags))
+    tph = []
+    plain(pq)
+    for _ in range(reps):
+        t.start()
+        plain(pq)
+        tph.append(t.stop())
+    planned_same = plain_out.host()
+    plain(qs)
+    tp = []
+    for _ in range(reps):
+        t.start()
+        plain(qs)
+        tp.append(t.stop())
+    qs = pq          # the batch runs the planned queries
+    tune, tms = abi.Tune(), C.c_float()
+    check(dev.lib.mbx_autotune(dev.ctx, qs, len(specs), C.byref(tune), C.byref(tms)))
+    b = C.c_void_p()
+    check(dev.lib.mbx_batch_create(dev.ctx, qs, len(specs), C.byref(tune), C.byref(b)))
+    check(dev.lib.mbx_batch_run(dev.ctx, b))
+    tb = []
+    for _ in range(reps):
+        t.start()
+        check(dev.lib.mbx_batch_run(dev.ctx, b))
+        tb.append(t.stop())
+    ref = plain_out.host()
+    same = bool(np.array_equal(ref, batch_out.host()) and
+                np.array_equal(ref, planned_same))
+    nbytes, nsel = C.c_uint64(), C.c_uint64()
+    check(dev.lib.mbx_alg_bytes_batch(dev.ctx, qs, len(specs), C.byref(nbytes),
+                                      C.byref(nsel)))
+    check(dev.lib.mbx_batch_destroy(dev.ctx, b))
+    t.close()
+    mp, mb, mh = float(np.median(tp)), float(np.median(tb)), float(np.median(tph))
+    for a in [plain_out, batch_out] + slabs + [c.dev for c in sch.cols.values()]:
+        a.free()
+    rows = n * len(specs)
+    return {
+        "workload": f"{len(specs)} queries (AND/OR/ANDNOT, selectivity 1e-6..1e-1, "
+                    f"1..10,000 groups) over {n:,} rows x 3 Zipf columns",
+        "plain_ms": round(mp, 3), "plain_heuristic_plan_ms": round(mh, 3),
+        "batched_ms": round(mb, 3),
+        "plain_G_rows_per_s": round(rows / mp / 1e6, 2),
+        "batched_G_rows_per_s": round(rows / mb / 1e6, 2),
+        "batched_vs_plain": round(mp / mb, 3),
+        "alg_GB_per_s_batched": round(nbytes.value / mb / 1e6, 1),
+        "roofline_frac_batched": round(nbytes.value / mb / 1e6 / HBM_PEAK_GBS, 4),
+        "tune": {"tile_words": tune.tile_words, "wg_per_cu": tune.wg_per_cu,
+                 "lds_groups": tune.lds_groups},
+        "identical_to_plain": same,
+        "note": "plain = as-written programs, one fused launch each; batched = "
+                "the same queries with the static heuristic plan in one "
+                "persistent hipGraph launch",
+    }
+
+
+def run_serving(dev, nreq=1000, rate=2000.0, budget=256 << 20):
+    """Config 5: 600M rows x 10 Zipf columns + 640 bucket bitmaps resident;
+    Poisson arrivals replayed against the serving loop, once with continuous
+    batching over 4 streams and once one request at a time; every request's
+    result must be identical in both."""
+    from merle_amd import abi, configs
+    from merle_amd.host import check
+    n = configs.C5_ROWS
+    t0 = time.time()
+    sch = configs.config5_device(dev, n)
+    bins, slabs = make_bins(dev, sch, configs.C5_BOUNDS)
+    log(f"config-5 tables and {sum(len(v) for v in bins.values())} bitmaps "
+        f"ready in {time.time() - t0:.1f} s")
+    specs = configs.config5_queries(nreq)
+    qs = plan_queries(dev, query_array(sch, specs, bins, configs.C5_BOUNDS))
+    rng = np.random.default_rng(5)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    arrivals -= arrivals[0]
+    tot = sum(q.nr_grp for q in specs)
+    out, results = {}, {}
+    for mode in ("batched", "one_at_a_time"):
+        res = np.zeros(tot, np.int64)
+        lat = np.zeros(nreq)
+        st = abi.ServeStats()
+        cfg = abi.ServeCfg(4, 32, budget, 0 if mode == "batched" else 1, 0,
+                           abi.Tune(8192, 4, 4096, 0))
+        check(dev.lib.mbx_serve_replay(dev.ctx, qs, nreq,
+                                       arrivals.ctypes.data_as(C.POINTER(C.c_double)),
+                                       C.byref(cfg), res.ctypes.data,
+                                       lat.ctypes.data_as(C.POINTER(C.c_double)),
+                                       C.byref(st)))
+        results[mode] = res
+        out[mode] = {
+            "completed": st.completed, "rejected": st.rejected,
+            "wall_s": round(st.wall_s, 4),
+            "G_rows_per_s": round(n * st.completed / st.wall_s / 1e9, 2),
+            "p50_ms": round(st.p50_s * 1e3, 3), "p99_ms": round(st.p99_s * 1e3, 3),
+            "mean_ms": round(st.mean_s * 1e3, 3),
+            "batches": st.batches, "mean_batch": round(st.mean_batch, 2),
+            "admission_waits": st.admission_waits,
+            "peak_mem_MB": round(st.peak_mem / 1e6, 2),
+        }
+    # algorithmic bytes from a sample of the requests (the accounting kernel
+    # is slow), scaled to all of them
+    ns = min(nreq, 64)
+    nbytes, nsel = C.c_uint64(), C.c_uint64()
+    check(dev.lib.mbx_alg_bytes_batch(dev.ctx, qs, ns, C.byref(nbytes),
+                                      C.byref(nsel)))
+    est = nbytes.value * nreq / ns
+    for mode in out:
+        gbs = est / out[mode]["wall_s"] / 1e9
+        out[mode]["alg_GB_per_s"] = round(gbs, 1)
+        out[mode]["roofline_frac"] = round(gbs / HBM_PEAK_GBS, 4)
+    for a in slabs + [c.dev for c in sch.cols.values()]:
+        a.free()
+    return {
+        "workload": f"{n:,} rows x 10 Zipf(1.2) columns (1,000..100,000 values), "
+                    f"64 bucket bitmaps per column resident; {nreq} requests, "
+                    f"Poisson arrivals at {rate:.0f}/s, memory budget "
+                    f"{budget >> 20} MiB",
+        **out,
+        "identical_results": bool(np.array_equal(results["batched"],
+                                                 results["one_at_a_time"])),
+        "alg_bytes_note": f"estimated from the first {ns} requests",
+    }
+
+
+def run_learned(dev, nprog=2000, neval=200):
+    """Config 4: runtimes of random programs' candidate plans (evaluation
+    orders x fuse / eager / materialise) measured by the engine, the
+    16-64-1 fp32 MLP trained on them with the library's kernels, then
+    held-out programs run with the as-written plan, the static heuristic and
+    the learned choice (identical sums required)."""
+    from merle_amd import learned
+    t0 = time.time()
+    sch = learned.config4_device(dev)
+    cols = [f"c{c}" for c in range(learned.C4_COLS)]
+    bins = learned.build_bins(dev, sch, cols)
+    t1 = time.time()
+    X, Y = learned.collect(dev, sch, bins, nprog, log=log)
+    t2 = time.time()
+    model = learned.CostModel(dev)
+    loss = model.train(X, Y)
+    t3 = time.time()
+    ev = learned.evaluate(dev, sch, bins, model, neval)
+    model.close()
+    for c in sch.cols.values():
+        c.dev.free()
+    return {
+        "workload": f"{learned.C4_ROWS:,} rows x 8 Zipf columns (s 0.5/1.0/1.5), "
+                    f"{nprog} training programs, {neval} held-out programs",
+        "samples": int(X.shape[0]), "setup_s": round(t1 - t0, 2),
+        "collect_s": round(t2 - t1, 2), "train_s": round(t3 - t2, 2),
+        "train_mse_log_ms": round(loss, 4), **ev,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -152,6 +384,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--rows", type=int, default=None,
                     help="fact rows (default: config 2's 120M)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the batched / learned / serving sections")
+    ap.add_argument("--serve-requests", type=int, default=1000)
+    ap.add_argument("--serve-rate", type=float, default=2000.0,
+                    help="Poisson arrival rate of the serving replay (req/s)")
+    ap.add_argument("--learned-programs", type=int, default=2000,
+                    help="training programs of the plan chooser (20000 = full)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="run only a plain device copy of this many bytes")
@@ -293,6 +532,17 @@ def main():
             prg.factsz = n
         result["parity_on_cpu_sample"] = ok
         log(f"cpu {cb['value']} G rows/s on {cb['cores']} threads; parity {ok}")
+    if not args.no_extras:
+        outbuf.free()
+        log("config 3: batched vs plain")
+        result["batched"] = run_batched(dev)
+        log(f"  {result['batched']}")
+        log("config 4: learned plan chooser")
+        result["learned"] = run_learned(dev, args.learned_programs)
+        log(f"  {result['learned']}")
+        log("config 5: serving")
+        result["serving"] = run_serving(dev, args.serve_requests, args.serve_rate)
+        log(f"  {result['serving']}")
     print(json.dumps(result), flush=True)
     dev.close()
     return 0

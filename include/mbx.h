@@ -216,7 +216,7 @@ typedef struct mbx_query {
   mbx_vprg prg;
   mbx_rowop op;
   uint32_t nr_grp;
-  uint32_t pad_;
+  uint32_t flags;          /* 0 or MBX_PLAN_EAGER (all operand loads up front) */
   int64_t *d_out;          /* device, nr_grp int64 */
 } mbx_query;
 
@@ -227,6 +227,13 @@ typedef struct mbx_tune {
   uint32_t lds_groups;     /* group-table capacity in LDS (int64 slots) */
   uint32_t pad_;
 } mbx_tune;
+
+/* Algorithmic bytes of a set of queries (compulsory 128-B lines of their
+ * bitmaps and gathered columns, a line shared by several queries counted
+ * once, + 8 B per output group).  No reference counterpart: it prices a
+ * batch for the roofline. */
+int mbx_alg_bytes_batch(mbx_ctx *ctx, const mbx_query *qs, uint32_t nq,
+                        uint64_t *bytes, uint64_t *selected);
 
 typedef struct mbx_batch mbx_batch;
 /* Builds device descriptors and captures the whole batch (output zeroing,
@@ -255,8 +262,9 @@ typedef struct mbx_serve_cfg {
   uint32_t nstreams;
   uint32_t max_batch;
   uint64_t mem_budget;      /* bytes; 0 = unbounded */
-  uint32_t plain;
-  uint32_t pad_;
+  uint32_t plain;           /* 1: one request at a time, one stream */
+  uint32_t fused_rows;      /* requests with >= this many fact rows run as
+                               their own fused launches; 0 = 32Mi */
   mbx_tune tune;            /* tile_words == 0: default shape */
 } mbx_serve_cfg;
 typedef struct mbx_serve_stats {

@@ -63,7 +63,7 @@ class RowOp(C.Structure):
 
 class Query(C.Structure):
     _fields_ = [("prg", Vprg), ("op", RowOp), ("nr_grp", C.c_uint32),
-                ("pad_", C.c_uint32), ("d_out", C.c_void_p)]
+                ("flags", C.c_uint32), ("d_out", C.c_void_p)]
 
 
 class Tune(C.Structure):
@@ -80,7 +80,7 @@ class Plan(C.Structure):
 class ServeCfg(C.Structure):
     _fields_ = [("nstreams", C.c_uint32), ("max_batch", C.c_uint32),
                 ("mem_budget", C.c_uint64), ("plain", C.c_uint32),
-                ("pad_", C.c_uint32), ("tune", Tune)]
+                ("fused_rows", C.c_uint32), ("tune", Tune)]
 
 
 class ServeStats(C.Structure):
@@ -137,6 +137,7 @@ PROTOTYPES = {
                               _u32]),
     "mbx_alg_bytes": (_i32, [_vp, _P(Vprg), _P(RowOp), _u32, _P(_u64),
                              _P(_u64)]),
+    "mbx_alg_bytes_batch": (_i32, [_vp, _P(Query), _u32, _P(_u64), _P(_u64)]),
     "mbx_batch_create": (_i32, [_vp, _P(Query), _u32, _P(Tune), _P(_vp)]),
     "mbx_batch_run": (_i32, [_vp, _vp]),
     "mbx_batch_run_eager": (_i32, [_vp, _vp]),

@@ -10,6 +10,12 @@ config 2  star schema: 120M-row fact table, int32 foreign keys into date
           dimensions; one bitmap per attribute value projected onto the fact
           rows; 12 SSB-flight-shaped queries (conjunctions of 2-4
           predicates, selectivity 1e-2..1e-5, up to 7x250x250 groups).
+config 3  the config-1 tables, 256 random AND / OR / ANDNOT programs with
+          log-uniform selectivity in [1e-6, 1e-1] and 1..10,000 groups.
+config 5  serving: 600M-row fact table of 10 int32 columns z0..z9, finite
+          Zipf(s=1.2) over 1,000..100,000 values (log-spaced), 64
+          equal-width bucket bitmaps per column; requests drawn from the
+          config-3 program space over these columns.
 """
 import ctypes as C
 
@@ -303,11 +309,13 @@ def config3_queries(nq=256, seed=SEED + 3, allow_edges=True):
 
 def bin_cols(spec, bin_ptrs, bounds=C1_BOUNDS):
     """BitmapCols of a spec over per-column bin slabs {col: [ptrs]} through
-    col::from_bin (edges when a range is misaligned)."""
+    col::from_bin (edges when a range is misaligned).  bounds: one bound
+    list for every column, or {col: bounds}."""
     from .spec import BitmapCol, from_bin
     cols = []
     for p in spec.preds:
-        r = from_bin(list(bounds), p.lo, p.hi)
+        b = bounds[p.src.col] if isinstance(bounds, dict) else bounds
+        r = from_bin(list(b), p.lo, p.hi)
         if r is None:
             raise ValueError(f"range [{p.lo},{p.hi}) not coverable by bins")
         mid, l, rr = r
@@ -316,3 +324,97 @@ def bin_cols(spec, bin_ptrs, bounds=C1_BOUNDS):
                               None if l is None else ptrs[l],
                               None if rr is None else ptrs[rr]))
     return cols
+
+
+# ---------------------------------------------------------------- config 5
+C5_ROWS = 600_000_000
+C5_COLS = tuple(f"z{c}" for c in range(10))
+C5_SKEW = 1.2
+C5_DOMAINS = tuple(int(round(1000 * 100 ** (c / 9))) for c in range(10))
+C5_NBINS = 64
+
+
+def c5_bounds(c):
+    """64 equal-width buckets covering [0, domain) of column c."""
+    w = -(-C5_DOMAINS[c] // C5_NBINS)
+    return tuple(range(0, w * C5_NBINS + 1, w))
+
+
+C5_BOUNDS = {name: c5_bounds(c) for c, name in enumerate(C5_COLS)}
+
+
+def config5_device(dev, n=C5_ROWS, seed=SEED + 5):
+    """The 10 columns, generated on the device (no host copy)."""
+    sch = Schema(n)
+    for c, name in enumerate(C5_COLS):
+        a = dev.alloc(4 * n, np.uint32, n)
+        check(dev.lib.mbx_gen_zipf_u32(dev.ctx, a.ptr, n, col_seed(seed, 50 + c),
+                                       C5_DOMAINS[c], C5_SKEW))
+        sch.add(name, None, a)
+    dev.sync()
+    return sch
+
+
+def config5_queries(nq, seed=SEED + 55):
+    """nq requests from the config-3 program space over z0..z9: 2-4 range
+    predicates on distinct columns (1-4 buckets, 30% misaligned), AND / OR /
+    ANDNOT, estimated selectivity log-uniform over [1e-6, 1e-1], 1..10,000
+    groups from bucketed keys, SUM(z0) or COUNT."""
+    rng = np.random.default_rng(seed)
+    pmfs = [zipf_pmf(d, C5_SKEW) for d in C5_DOMAINS]
+    pool = []
+    while len(pool) < nq * 20:
+        npred = int(rng.integers(2, 5))
+        cols = rng.choice(len(C5_COLS), size=npred, replace=False)
+        preds, dens = [], []
+        for c in cols:
+            b = C5_BOUNDS[C5_COLS[c]]
+            nb = len(b) - 1
+            j = int(rng.integers(nb))
+            w = int(rng.integers(1, 5))
+            lo, hi = b[j], b[min(nb, j + w)]
+            if rng.random() < 0.3 and hi - lo > 1:
+                lo = min(hi - 1, lo + int(rng.integers(1, b[1])))
+            preds.append(Pred(Src(C5_COLS[c]), lo, hi))
+            dens.append(_range_density(pmfs[c], lo, min(hi, C5_DOMAINS[c])))
+        shape = rng.integers(3)
+        if shape == 0:
+            ins = [(abi.ORM, 0, 0)] + [(abi.ANDM, 0, i) for i in range(1, npred)]
+            sel = float(np.prod(dens))
+        elif shape == 1:
+            ins = [(abi.ORM, 0, 0), (abi.ORM, 0, 1)] + \
+                  [(abi.ANDM, 0, i) for i in range(2, npred)]
+            sel = (dens[0] + dens[1] - dens[0] * dens[1]) * float(np.prod(dens[2:]))
+        else:
+            ins = [(abi.ORM, 0, 0), (abi.ORM, 1, 1), (abi.NOT, 1, 1),
+                   (abi.AND, 0, 1)] + [(abi.ANDM, 0, i) for i in range(2, npred)]
+            sel = dens[0] * (1 - dens[1]) * float(np.prod(dens[2:]))
+        if not (1e-6 <= sel <= 1e-1):
+            continue
+        keys, total = [], 1
+        for _ in range(int(rng.integers(0, 4))):
+            c = int(rng.integers(len(C5_COLS)))
+            wdt = C5_BOUNDS[C5_COLS[c]][1]
+            div = wdt * int(rng.choice([1, 4, 16]))
+            card = -(-C5_DOMAINS[c] // div)
+            if total * card > 10_000:
+                break
+            keys.append(Key(Src(C5_COLS[c]), abi.KEY_DIV, div, 0, total))
+            total *= card
+        meas = abi.MEAS_COL if rng.random() < 0.8 else abi.MEAS_COUNT
+        pool.append((np.log10(sel), QuerySpec(
+            preds, ins, keys, total, meas, Src("z0") if meas else None, None,
+            f"r{len(pool)}")))
+    pool.sort(key=lambda t: t[0])
+    edges = np.linspace(-6, -1, nq + 1)
+    out, i = [], 0
+    for b in range(nq):
+        while i < len(pool) and pool[i][0] < edges[b]:
+            i += 1
+        if i < len(pool):
+            out.append(pool[i][1])
+            i += 1
+    while len(out) < nq:
+        out.append(pool[int(rng.integers(len(pool)))][1])
+    order = rng.permutation(len(out))       # arrival order: mixed selectivities
+    return [out[k] for k in order]

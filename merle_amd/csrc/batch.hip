@@ -1,13 +1,18 @@
 // batch.hip -- many queries in one persistent launch, captured in a hipGraph.
 //
 // The reference runs one query per launch (ssb/lambdas.cuh:179-184).  Here a
-// batch of queries becomes a work list of (query, tile range) items, ordered
-// query by query.  A persistent grid pulls items from a device counter; for
-// each item a workgroup zeroes its LDS group table, runs the same tile
-// pipeline as the single-query kernel (program -> compaction -> row op) over
-// the item's tiles, and flushes the table to that query's int64 output.
-// Output zeroing, counter reset and the launch are captured once into a
-// hipGraph (fill kernels only: no memset nodes) and replayed per run.
+// batch of queries becomes a set of (query, row range) work items.  The
+// queries of a batch mostly read the same columns and bitmaps, so items are
+// ordered row range first: the items of one row range -- that range of every
+// query -- run close together in time, on one XCD, and its lines are fetched
+// from HBM once and then served by that XCD's L2 to the other queries.  Row
+// range r belongs to XCD r mod 8 (workgroup b runs on XCD b mod 8); each XCD
+// drains its own list through its own counter.  For each item a workgroup
+// zeroes its LDS group table, runs the same wave pipeline as the
+// single-query kernel (program -> compaction -> row op) over the range, and
+// flushes the table to the query's int64 output.  Output zeroing, counter
+// reset and the launch are captured once into a hipGraph (fill kernels only:
+// no memset nodes) and replayed per run.
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -21,43 +26,58 @@ namespace mbx {
 struct BatchArgs {
   const BatchQ *qs;
   uint32_t nq;
-  uint32_t total_items;
-  uint32_t *counter;
+  uint32_t max_items;   // row ranges of the longest query
+  uint32_t nx;          // XCD lists (8, fewer for a tiny grid)
+  uint32_t *counter;    // nx counters, kXcdCounterStride apart
 };
+
+template <bool CHK, bool EAGER>
+__device__ __forceinline__ void batch_item(const BatchQ &q, uint32_t w_begin,
+                                           uint32_t w_end, const GroupSink &sink,
+                                           uint32_t *s_list) {
+  // the item's 64-word chunks, interleaved over the workgroup's four waves
+  // (three-valued programs take half as many per pass: twice the registers)
+  constexpr int W = CHK ? kBWPL / 2 : kBWPL;
+  const uint32_t wv = threadIdx.x >> 6;
+  for (uint32_t w0 = w_begin + wv * 64; w0 < w_end; w0 += 4 * 64 * W)
+    run_wave_range<CHK, false, W, EAGER>(q.prg, q.op, q.nr_grp, w0, 4 * 64,
+                                         w_end, sink, s_list, nullptr);
+}
 
 __global__ __launch_bounds__(256) void batch_kernel(BatchArgs a) {
   __shared__ uint32_t s_list[4][kWaveList];
   __shared__ uint32_t s_item;
   const uint32_t wv = threadIdx.x >> 6;
+  const uint32_t x = blockIdx.x % a.nx;
+  // this XCD's row ranges x, x + nx, ...: nq items each
+  const uint32_t nranges = a.max_items > x ? (a.max_items - x + a.nx - 1) / a.nx : 0;
+  const uint32_t nitems = nranges * a.nq;
   for (;;) {
-    if (threadIdx.x == 0) s_item = atomicAdd(a.counter, 1u);
+    if (threadIdx.x == 0)
+      s_item = atomicAdd(a.counter + x * kXcdCounterStride, 1u);
     __syncthreads();
-    const uint32_t item = __builtin_amdgcn_readfirstlane(s_item);
-    if (item >= a.total_items) break;   // every thread sees the same item
-    // the query owning this item: last q with item0 <= item
-    uint32_t lo = 0, hi = a.nq - 1;
-    while (lo < hi) {
-      const uint32_t mid = (lo + hi + 1) >> 1;
-      if (a.qs[mid].item0 <= item) lo = mid;
-      else hi = mid - 1;
+    const uint32_t j = __builtin_amdgcn_readfirstlane(s_item);
+    if (j >= nitems) break;   // every thread sees the same item
+    const uint32_t r = x + a.nx * (j / a.nq);
+    const BatchQ &q = a.qs[j % a.nq];
+    if (r >= q.nitems) {      // a shorter query: no such range
+      __syncthreads();        // s_item is rewritten next round
+      continue;
     }
-    const BatchQ &q = a.qs[lo];
     GroupSink sink;
     sink.lds = q.lds != 0;
     sink.out = q.out;
     if (q.lds)
       for (uint32_t i = threadIdx.x; i < q.nr_grp; i += 256) s_grp_dyn[i] = 0;
     __syncthreads();
-    // the item's wave tiles, spread over the workgroup's four waves
-    const uint32_t t0 = (item - q.item0) * q.tiles_per_item;
-    const uint32_t t1 = min(t0 + q.tiles_per_item, q.ntiles);
-    for (uint32_t t = t0 + wv; t < t1; t += 4) {
-      if (q.chk)
-        run_wave_tile<true, false, kBWPL, false>(q.prg, q.op, q.nr_grp, t, sink,
-                                                 s_list[wv], nullptr);
-      else
-        run_wave_tile<false, false, kBWPL, false>(q.prg, q.op, q.nr_grp, t,
-                                                  sink, s_list[wv], nullptr);
+    const uint32_t w_begin = r * q.words_per_item;
+    const uint32_t w_end = min(w_begin + q.words_per_item, q.prg.nwords);
+    if (q.chk) {
+      if (q.eager) batch_item<true, true>(q, w_begin, w_end, sink, s_list[wv]);
+      else batch_item<true, false>(q, w_begin, w_end, sink, s_list[wv]);
+    } else {
+      if (q.eager) batch_item<false, true>(q, w_begin, w_end, sink, s_list[wv]);
+      else batch_item<false, false>(q, w_begin, w_end, sink, s_list[wv]);
     }
     if (q.lds) {
       __syncthreads();
@@ -83,9 +103,9 @@ __global__ void zero_outputs_kernel(const BatchQ *qs, uint32_t nq) {
 using namespace mbx;
 
 struct mbx_batch {
-  BatchQ *d_qs = nullptr;     // pool memory: descriptors then the counter
+  BatchQ *d_qs = nullptr;     // pool memory: descriptors then the counters
   uint32_t *d_counter = nullptr;
-  uint32_t nq = 0, total_items = 0;
+  uint32_t nq = 0, total_items = 0, max_items = 0;
   size_t lds_bytes = 0;
   unsigned grid = 0;
   hipGraph_t graph = nullptr;
@@ -96,17 +116,18 @@ struct mbx_batch {
 namespace mbx {
 
 int batch_launch(hipStream_t s, const BatchQ *d_qs, uint32_t nq,
-                 uint32_t total_items, uint32_t *d_counter, size_t lds_bytes,
+                 uint32_t max_items, uint32_t *d_counter, size_t lds_bytes,
                  unsigned grid) {
   hipLaunchKernelGGL(zero_outputs_kernel, dim3(std::min(nq, 1024u)),
                      dim3(256), 0, s, d_qs, nq);
   MBX_HIP(hipGetLastError());
-  int st = launch_fill_u32(s, d_counter, 0, 1);
+  int st = launch_fill_u32(s, d_counter, 0, kXcdLists * kXcdCounterStride);
   if (st) return st;
   BatchArgs a;
   a.qs = d_qs;
   a.nq = nq;
-  a.total_items = total_items;
+  a.max_items = max_items;
+  a.nx = grid < kXcdLists ? grid : kXcdLists;
   a.counter = d_counter;
   hipLaunchKernelGGL(batch_kernel, dim3(grid), dim3(256), lds_bytes, s, a);
   MBX_HIP(hipGetLastError());
@@ -114,19 +135,20 @@ int batch_launch(hipStream_t s, const BatchQ *d_qs, uint32_t nq,
 }
 
 unsigned batch_grid(mbx_ctx *ctx, const mbx_tune &t, uint32_t total_items) {
-  return (unsigned)std::max<uint64_t>(
-      1, std::min<uint64_t>((uint64_t)ctx->num_cus * t.wg_per_cu, total_items));
+  uint64_t g = std::min<uint64_t>((uint64_t)ctx->num_cus * t.wg_per_cu, total_items);
+  if (g > kXcdLists) g -= g % kXcdLists;   // whole rounds over the XCDs
+  return (unsigned)std::max<uint64_t>(1, g);
 }
 
 static int enqueue_batch(mbx_ctx *ctx, mbx_batch *b) {
-  return batch_launch(ctx->stream, b->d_qs, b->nq, b->total_items,
+  return batch_launch(ctx->stream, b->d_qs, b->nq, b->max_items,
                       b->d_counter, b->lds_bytes, b->grid);
 }
 
 int batch_prepare(mbx_ctx *ctx, const mbx_query *qs, uint32_t nq,
                   const mbx_tune &t, BatchQ *h, uint32_t *total_items,
-                  size_t *lds_bytes) {
-  uint32_t items = 0, max_lds_grp = 0;
+                  size_t *lds_bytes, uint32_t *max_items) {
+  uint32_t items = 0, max_lds_grp = 0, maxi = 0;
   for (uint32_t i = 0; i < nq; ++i) {
     QueryPlan p;
     int st = prepare(ctx, &qs[i].prg, &qs[i].op, qs[i].nr_grp, qs[i].d_out, &p);
@@ -143,13 +165,15 @@ int batch_prepare(mbx_ctx *ctx, const mbx_query *qs, uint32_t nq,
     b.lds = p.nr_grp <= t.lds_groups ? 1u : 0u;
     if (b.lds) max_lds_grp = std::max(max_lds_grp, p.nr_grp);
     b.chk = p.prg.chk;
-    b.ntiles = (uint32_t)ceil_div(p.prg.nwords, kTileWords);
-    b.tiles_per_item = t.tile_words / kTileWords;
-    b.nitems = (uint32_t)ceil_div(b.ntiles, b.tiles_per_item);
+    b.eager = (qs[i].flags & MBX_PLAN_EAGER) ? 1u : 0u;
+    b.words_per_item = t.tile_words;
+    b.nitems = (uint32_t)ceil_div(p.prg.nwords, b.words_per_item);
     b.item0 = items;
     items += b.nitems;
+    maxi = std::max(maxi, b.nitems);
   }
   *total_items = items;
+  if (max_items) *max_items = maxi;
   *lds_bytes = (size_t)max_lds_grp * 8;
   return MBX_OK;
 }
@@ -163,7 +187,7 @@ static void destroy_batch(mbx_ctx *ctx, mbx_batch *b) {
 }
 
 static int valid_tune(const mbx_tune *t) {
-  if (t->tile_words == 0 || t->tile_words % (4 * kTileWords))
+  if (t->tile_words == 0 || t->tile_words % kItemQuantum)
     return fail(MBX_ERR_INVALID, "tune: tile_words must be a multiple of 512");
   if (t->wg_per_cu == 0 || t->wg_per_cu > 16)
     return fail(MBX_ERR_INVALID, "tune: wg_per_cu must be in 1..16");
@@ -185,17 +209,19 @@ static int build_batch(mbx_ctx *ctx, const mbx_query *qs, uint32_t nq,
   std::vector<BatchQ> h(nq);
   uint32_t items = 0;
   size_t lds_bytes = 0;
-  st = batch_prepare(ctx, qs, nq, t, h.data(), &items, &lds_bytes);
+  uint32_t max_items = 0;
+  st = batch_prepare(ctx, qs, nq, t, h.data(), &items, &lds_bytes, &max_items);
   if (st) return st;
   auto *b = new mbx_batch;
   b->nq = nq;
   b->total_items = items;
+  b->max_items = max_items;
   b->tune = t;
   b->lds_bytes = lds_bytes;
   b->grid = batch_grid(ctx, t, items);
   void *mem = nullptr;
   const size_t qbytes = ceil_div(sizeof(BatchQ) * nq, 256) * 256;
-  st = ctx->pool->alloc(qbytes + 256, &mem);
+  st = ctx->pool->alloc(qbytes + kCounterBytes, &mem);
   if (st) {
     delete b;
     return st;

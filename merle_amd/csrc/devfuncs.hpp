@@ -43,6 +43,13 @@ __device__ __forceinline__ bool fetch_src(const DevSrc &s, size_t row,
 // wave64 helpers ---------------------------------------------------------
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
 
+// Sum of x over the 64 lanes of a wave (every lane gets it).
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long x) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) x += __shfl_xor(x, d, 64);
+  return x;
+}
+
 // Inclusive prefix sum across the 64 lanes of a wave.
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
   const uint32_t lane = lane_id();

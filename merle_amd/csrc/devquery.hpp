@@ -16,7 +16,7 @@ struct AcctEntry {
   uint32_t off;       // first bitset word
   uint32_t pad;
 };
-constexpr int kAcctMax = 48;
+constexpr int kAcctMax = 64;
 struct Acct {
   AcctEntry e[kAcctMax];
   uint32_t n, pad;
@@ -544,9 +544,31 @@ __device__ __forceinline__ void process_entries(const DevPrg &P,
   rows_keyval<ACCT, E>(O, row, live, nr_grp, key, val, ok, acct);
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    if (!ok[e]) continue;
-    if (ACCT) atomicAdd(acct->selected, 1ull);
-    else sink_add(sink, key[e], val[e], nr_grp);
+    if (ACCT) {
+      if (ok[e]) atomicAdd(acct->selected, 1ull);
+      continue;
+    }
+    if (sink.lds) {
+      if (ok[e]) sink_add(sink, key[e], val[e], nr_grp);
+      continue;
+    }
+    // HBM output: atomics on one address serialise, so the rows of the
+    // wave's two leading keys are summed across the wave first (one atomic
+    // per key); the remaining rows add directly.
+    bool pend = ok[e];
+#pragma unroll
+    for (int round = 0; round < 2; ++round) {
+      const unsigned long long act = __ballot(pend);
+      if (!act) break;
+      const int leader = (int)__builtin_ctzll(act);
+      const uint32_t lk = __shfl(key[e], leader, 64);
+      const bool mine = pend && key[e] == lk;
+      const unsigned long long v =
+          wave_sum_u64(mine ? (unsigned long long)val[e] : 0ull);
+      if (lane_id() == (uint32_t)leader) sink_add(sink, lk, (int64_t)v, nr_grp);
+      pend = pend && !mine;
+    }
+    if (pend) sink_add(sink, key[e], val[e], nr_grp);
   }
 }
 

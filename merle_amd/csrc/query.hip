@@ -3,6 +3,7 @@
 //   materialise  _stg1 / _stg2 / _stg3 (primitive_abla.cuh:262-453)
 //   scan         grpby                 (primitive.cuh:16-48)
 // plus the algorithmic-byte accounting kernel behind mbx_alg_bytes.
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -236,10 +237,15 @@ static unsigned persistent_grid(mbx_ctx *ctx, uint32_t work, unsigned per_cu) {
   return g ? (unsigned)g : 1u;
 }
 
+size_t spread_bytes() {
+  return (size_t)kSpreadRep * kLdsGroups * 8 +
+         (size_t)kSpreadRep * kSpreadDoneStride * 4;
+}
+
 static Spread spread_of(mbx_ctx *ctx, const QueryPlan &q) {
   Spread sp;
-  sp.rep = ctx->spread;
-  sp.done = ctx->spread_done;
+  sp.rep = q.spread ? q.spread : ctx->spread;
+  sp.done = q.spread ? q.spread_done : ctx->spread_done;
   sp.gpad = (uint32_t)((q.nr_grp + 15) / 16 * 16);
   return sp;
 }
@@ -505,62 +511,91 @@ int mbx_rows_grpby(mbx_ctx *ctx, const uint32_t *d_rows,
   return launch_stage3(ctx, q, d_rows, d_count);
 }
 
-int mbx_alg_bytes(mbx_ctx *ctx, const mbx_vprg *prg, const mbx_rowop *op,
-                  uint32_t nr_grp, uint64_t *bytes, uint64_t *selected) {
-  QueryPlan q;
+}  // extern "C"
+
+namespace mbx {
+
+// Compulsory-line accounting of a set of queries: the ACCT form of the fused
+// kernel marks every 128-byte line its loads touch, in one bitset per
+// distinct memory range, shared by all the queries (a line several queries
+// read counts once).  Ranges are collected per query, sorted and merged
+// (adjacent bins of one slab become one range).
+static int alg_bytes_multi(mbx_ctx *ctx, const mbx_query *qs, uint32_t nq,
+                           uint64_t *bytes, uint64_t *selected) {
+  if (!ctx || !qs || !nq) return fail(MBX_ERR_INVALID, "alg_bytes: null");
   void *dummy_out = nullptr;
-  if (!ctx) return fail(MBX_ERR_INVALID, "null ctx");
-  int st = ctx->pool->alloc(8, &dummy_out);
+  int st = ctx->pool->alloc(8 * 4096, &dummy_out);
   if (st) return st;
-  st = prepare(ctx, prg, op, nr_grp, static_cast<int64_t *>(dummy_out), &q);
+  std::vector<QueryPlan> plans(nq);
+  std::vector<std::pair<uint64_t, uint64_t>> ranges;   // [base, end)
+  auto add = [&](const void *p, uint64_t nbytes) {
+    if (!p || !nbytes) return;
+    const uint64_t b = reinterpret_cast<uint64_t>(p);
+    ranges.emplace_back(b, b + nbytes);
+  };
+  uint64_t groups = 0;
+  for (uint32_t qi = 0; qi < nq && !st; ++qi) {
+    QueryPlan &q = plans[qi];
+    st = prepare(ctx, &qs[qi].prg, &qs[qi].op, qs[qi].nr_grp,
+                 static_cast<int64_t *>(dummy_out), &q);
+    if (st) break;
+    groups += qs[qi].nr_grp;
+    const uint64_t N = q.prg.nrows, bmp_bytes = (uint64_t)q.prg.nwords * 8;
+    for (uint32_t e = 0; e < q.prg.nfl; ++e) add(q.prg.fl[e], bmp_bytes);
+    auto add_src = [&](const DevSrc &sr) {
+      if (!sr.data) return;
+      if (sr.fk) {
+        add(sr.fk, N * 4);
+        add(sr.data, (uint64_t)sr.dimsz * sr.eb);
+      } else {
+        add(sr.data, N * sr.eb);
+      }
+    };
+    if (q.prg.chk)
+      for (uint32_t k = 0; k < q.op.npreds; ++k) add_src(q.op.pred[k]);
+    for (uint32_t k = 0; k < q.op.nkeys; ++k) add_src(q.op.keys[k].src);
+    if (q.op.measure != MBX_MEAS_COUNT) add_src(q.op.ma);
+    if (q.op.measure == MBX_MEAS_MUL || q.op.measure == MBX_MEAS_SUB)
+      add_src(q.op.mb);
+  }
   if (st) {
     ctx->pool->free(dummy_out);
     return st;
   }
-  // distinct arrays the plan can touch
+  // Ranges closer than kMergeGap merge (the bins of one slab, read by
+  // different queries, become one range); a gap costs 1 bit per 128 bytes.
+  constexpr uint64_t kMergeGap = 8ull << 30;
+  std::sort(ranges.begin(), ranges.end());
+  std::vector<std::pair<uint64_t, uint64_t>> merged;
+  for (const auto &r : ranges) {
+    if (!merged.empty() && r.first <= merged.back().second + kMergeGap)
+      merged.back().second = std::max(merged.back().second, r.second);
+    else
+      merged.push_back(r);
+  }
+  uint64_t span = 0;
+  for (const auto &r : merged) span += r.second - r.first;
+  if (span / 128 / 8 > (1ull << 30)) {
+    ctx->pool->free(dummy_out);
+    return fail(MBX_ERR_CAPACITY, "alg_bytes: address span too large");
+  }
+  if (merged.size() > (size_t)kAcctMax) {
+    ctx->pool->free(dummy_out);
+    return fail(MBX_ERR_CAPACITY, "alg_bytes: too many distinct arrays");
+  }
   Acct h;
   std::memset(&h, 0, sizeof(h));
   std::vector<uint32_t> offs(1, 0);
-  auto add = [&](const void *p, uint64_t nbytes) {
-    if (!p || !nbytes) return;
-    const uint64_t b = reinterpret_cast<uint64_t>(p);
-    for (uint32_t i = 0; i < h.n; ++i)
-      if (h.e[i].base == b) return;
-    if (h.n >= (uint32_t)kAcctMax) return;
+  for (const auto &r : merged) {
     AcctEntry &e = h.e[h.n++];
-    e.base = b;
-    e.nbytes = nbytes;
+    e.base = r.first;
+    e.nbytes = r.second - r.first;
     e.off = offs.back();
-    offs.push_back(offs.back() + (uint32_t)ceil_div(ceil_div(nbytes, 128), 32));
-  };
-  const uint64_t N = q.prg.nrows, bmp_bytes = (uint64_t)q.prg.nwords * 8;
-  for (uint32_t i = 0; i < q.prg.nins; ++i) {
-    const uint32_t o = q.prg.ins[i][0];
-    if (o != MBX_ANDM && o != MBX_ORM) continue;
-    const DevCol &c = q.prg.cols[q.prg.ins[i][2]];
-    for (uint32_t k = 0; k < c.nm; ++k) add(c.m[k], bmp_bytes);
-    add(c.l, bmp_bytes);
-    add(c.r, bmp_bytes);
+    offs.push_back(offs.back() + (uint32_t)ceil_div(ceil_div(e.nbytes, 128), 32));
   }
-  auto add_src = [&](const DevSrc &s) {
-    if (!s.data) return;
-    if (s.fk) {
-      add(s.fk, N * 4);
-      add(s.data, (uint64_t)s.dimsz * s.eb);
-    } else {
-      add(s.data, N * s.eb);
-    }
-  };
-  if (q.prg.chk)
-    for (uint32_t i = 0; i < q.op.npreds; ++i) add_src(q.op.pred[i]);
-  for (uint32_t i = 0; i < q.op.nkeys; ++i) add_src(q.op.keys[i].src);
-  if (q.op.measure != MBX_MEAS_COUNT) add_src(q.op.ma);
-  if (q.op.measure == MBX_MEAS_MUL || q.op.measure == MBX_MEAS_SUB)
-    add_src(q.op.mb);
-
   const size_t nbits_words = offs.back();
   const size_t need = sizeof(Acct) + 256 + nbits_words * 4 + 8 +
-                      (offs.size()) * 4 + h.n * 8 + 256;
+                      offs.size() * 4 + h.n * 8 + 256;
   void *buf = nullptr;
   st = ctx->pool->alloc(need, &buf);
   if (st) {
@@ -580,7 +615,7 @@ int mbx_alg_bytes(mbx_ctx *ctx, const mbx_vprg *prg, const mbx_rowop *op,
   uint32_t *d_offs = reinterpret_cast<uint32_t *>(cur);
   h.bits = d_bits;
   h.selected = d_sel;
-  st = launch_fill_u32(ctx->stream, d_bits, 0, nbits_words);
+  if (nbits_words) st = launch_fill_u32(ctx->stream, d_bits, 0, nbits_words);
   if (!st) st = launch_fill_u64(ctx->stream, d_sel, 0, 1 + h.n);
   if (!st) {
     hipError_t e = hipMemcpyAsync(d_acct, &h, sizeof(Acct),
@@ -590,7 +625,7 @@ int mbx_alg_bytes(mbx_ctx *ctx, const mbx_vprg *prg, const mbx_rowop *op,
                          hipMemcpyHostToDevice, ctx->stream);
     if (e != hipSuccess) st = hip_fail(e, "alg_bytes upload");
   }
-  if (!st) st = launch_fused(ctx, q, d_acct);
+  for (uint32_t qi = 0; qi < nq && !st; ++qi) st = launch_fused(ctx, plans[qi], d_acct);
   if (!st && h.n) {
     hipLaunchKernelGGL(acct_count_kernel, dim3(h.n), dim3(256), 0, ctx->stream,
                        d_bits, d_offs, h.n, d_lines);
@@ -610,11 +645,31 @@ int mbx_alg_bytes(mbx_ctx *ctx, const mbx_vprg *prg, const mbx_rowop *op,
   ctx->pool->free(dummy_out);
   if (st) return st;
   uint64_t tot = 0;
-  for (uint32_t i = 0; i < h.n; ++i) tot += lines[1 + i] * 128ull;
-  tot += 8ull * nr_grp;
+  for (uint32_t k = 0; k < h.n; ++k) tot += lines[1 + k] * 128ull;
+  tot += 8ull * groups;
   if (bytes) *bytes = tot;
   if (selected) *selected = lines[0];
   return MBX_OK;
+}
+
+}  // namespace mbx
+
+extern "C" {
+
+int mbx_alg_bytes(mbx_ctx *ctx, const mbx_vprg *prg, const mbx_rowop *op,
+                  uint32_t nr_grp, uint64_t *bytes, uint64_t *selected) {
+  if (!prg || !op) return fail(MBX_ERR_INVALID, "alg_bytes: null");
+  mbx_query q;
+  std::memset(&q, 0, sizeof(q));
+  q.prg = *prg;
+  q.op = *op;
+  q.nr_grp = nr_grp;
+  return alg_bytes_multi(ctx, &q, 1, bytes, selected);
+}
+
+int mbx_alg_bytes_batch(mbx_ctx *ctx, const mbx_query *qs, uint32_t nq,
+                        uint64_t *bytes, uint64_t *selected) {
+  return alg_bytes_multi(ctx, qs, nq, bytes, selected);
 }
 
 int mbx_col_density(mbx_ctx *ctx, const mbx_vprg *prg, double dens[MBX_MAXCOLS]) {

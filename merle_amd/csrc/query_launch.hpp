@@ -13,7 +13,15 @@ struct QueryPlan {
   uint32_t nr_grp = 0;
   bool lds = false;   // group table fits in LDS
   bool eager = false; // all operand loads up front (no ANDM early-out)
+  // replica tables + counters of the LDS-table flush (zero between
+  // launches); null = the context's own.  Launches that may overlap on the
+  // device need their own.
+  unsigned long long *spread = nullptr;
+  uint32_t *spread_done = nullptr;
 };
+
+// Bytes of one set of flush replicas (tables, then counters).
+size_t spread_bytes();
 
 // Validates the query and builds its device descriptors.
 int prepare(mbx_ctx *ctx, const mbx_vprg *prg, const mbx_rowop *op,

@@ -5,10 +5,11 @@
 // arrived request is admitted when its output fits the memory budget (the
 // output is then allocated from the pool); admitted requests wait for a free
 // slot.  A slot is a HIP stream with a descriptor buffer: whenever one is
-// free, up to max_batch admitted requests go out as one persistent batched
-// launch (the same kernel as mbx_batch_*), while batches on the other slots
-// are still running, so work of requests that arrived later overlaps on the
-// device with work already in flight.  When a slot's completion event fires,
+// free, up to max_batch admitted requests go out on it -- large ones as
+// fused launches of their own, the rest as one persistent batched launch
+// (the same kernel as mbx_batch_*) -- while the other slots are still
+// running, so work of requests that arrived later overlaps on the device
+// with work already in flight.  When a slot's completion event fires,
 // its results are copied to the host, their memory is released and the next
 // admitted requests take the slot.  Latency = result on the host - arrival.
 #include <algorithm>
@@ -32,9 +33,16 @@ struct Slot {
   BatchQ *h = nullptr;       // pinned staging of the descriptors
   BatchQ *d = nullptr;       // device descriptors (+ counter after them)
   uint32_t *counter = nullptr;
+  unsigned long long *spread = nullptr;   // flush replicas of fused launches
+  uint32_t *spread_done = nullptr;
   std::vector<uint32_t> qs;  // requests in flight on this slot
   bool busy = false;
 };
+
+// By default, requests at least this large (fact rows) go out as their own
+// fused launches -- a whole-GPU grid each -- rather than as items of a
+// batched launch, whose per-item passes suit many small queries.
+constexpr uint32_t kServeFusedRows = 32u << 20;
 
 double now_s(std::chrono::steady_clock::time_point t0) {
   return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0)
@@ -61,6 +69,7 @@ int mbx_serve_replay(mbx_ctx *ctx, const mbx_query *reqs, uint32_t nq,
   mbx_tune tune = cfg->tune;
   if (tune.tile_words == 0) mbx_default_tune(&tune);
   const bool plain = cfg->plain != 0;
+  const uint32_t fused_rows = cfg->fused_rows ? cfg->fused_rows : kServeFusedRows;
   const uint32_t nslot = plain ? 1 : cfg->nstreams;
   const uint32_t maxb = plain ? 1 : cfg->max_batch;
 
@@ -69,7 +78,7 @@ int mbx_serve_replay(mbx_ctx *ctx, const mbx_query *reqs, uint32_t nq,
 
   // slots
   std::vector<Slot> slots(nslot);
-  const size_t dbytes = round256(sizeof(BatchQ) * maxb) + 256;
+  const size_t dbytes = round256(sizeof(BatchQ) * maxb) + kCounterBytes;
   int st = MBX_OK;
   size_t used = 0, peak = 0;
   for (auto &sl : slots) {
@@ -88,6 +97,17 @@ int mbx_serve_replay(mbx_ctx *ctx, const mbx_query *reqs, uint32_t nq,
     sl.counter = reinterpret_cast<uint32_t *>(static_cast<char *>(p) +
                                               round256(sizeof(BatchQ) * maxb));
     used += dbytes;
+    // fixed working memory of the loop (like the context's own scratch),
+    // outside the request budget
+    if (!plain) {
+      st = ctx->pool->alloc(spread_bytes(), &p);
+      if (st) break;
+      sl.spread = static_cast<unsigned long long *>(p);
+      sl.spread_done = reinterpret_cast<uint32_t *>(
+          static_cast<char *>(p) + (size_t)kSpreadRep * kLdsGroups * 8);
+      st = launch_fill_u32(sl.s, sl.spread, 0, spread_bytes() / 4);
+      if (st) break;
+    }
   }
   const size_t budget = cfg->mem_budget ? cfg->mem_budget : (size_t)-1;
   std::vector<void *> outp(nq, nullptr);
@@ -177,19 +197,43 @@ int mbx_serve_replay(mbx_ctx *ctx, const mbx_query *reqs, uint32_t nq,
       if (plain) {
         ctx->stream = sl.s;
         st = mbx_prog_grpby(ctx, &tmp[0].prg, &tmp[0].op, tmp[0].nr_grp,
-                            tmp[0].d_out, MBX_PLAN_FUSED);
+                            tmp[0].d_out,
+                            MBX_PLAN_FUSED | (tmp[0].flags & MBX_PLAN_EAGER));
         ctx->stream = keep;
       } else {
-        uint32_t items = 0;
+        // large requests: one fused launch each (slot-private flush
+        // replicas, so slots overlap safely); the rest: one batched launch
+        uint32_t nsmall = 0;
+        ctx->stream = sl.s;
+        for (uint32_t i = 0; i < k && !st; ++i) {
+          if (tmp[i].prg.factsz < fused_rows) {
+            tmp[nsmall++] = tmp[i];
+            continue;
+          }
+          QueryPlan qp;
+          st = prepare(ctx, &tmp[i].prg, &tmp[i].op, tmp[i].nr_grp,
+                       tmp[i].d_out, &qp);
+          if (st) break;
+          qp.eager = (tmp[i].flags & MBX_PLAN_EAGER) != 0;
+          qp.spread = sl.spread;
+          qp.spread_done = sl.spread_done;
+          st = launch_fill_u32(sl.s, tmp[i].d_out, 0, 2ull * tmp[i].nr_grp);
+          if (!st) st = launch_fused(ctx, qp, nullptr);
+        }
+        ctx->stream = keep;
+        uint32_t items = 0, max_items = 0;
         size_t lds = 0;
-        st = batch_prepare(ctx, tmp.data(), k, tune, sl.h, &items, &lds);
-        if (!st) {
-          hipError_t e = hipMemcpyAsync(sl.d, sl.h, sizeof(BatchQ) * k,
+        const uint32_t kb = nsmall;
+        if (!st && kb)
+          st = batch_prepare(ctx, tmp.data(), kb, tune, sl.h, &items, &lds,
+                             &max_items);
+        if (!st && kb) {
+          hipError_t e = hipMemcpyAsync(sl.d, sl.h, sizeof(BatchQ) * kb,
                                         hipMemcpyHostToDevice, sl.s);
           if (e != hipSuccess) st = hip_fail(e, "serve: descriptors");
         }
-        if (!st)
-          st = batch_launch(sl.s, sl.d, k, items, sl.counter, lds,
+        if (!st && kb)
+          st = batch_launch(sl.s, sl.d, kb, max_items, sl.counter, lds,
                             batch_grid(ctx, tune, items));
       }
       if (!st) {
@@ -208,6 +252,7 @@ int mbx_serve_replay(mbx_ctx *ctx, const mbx_query *reqs, uint32_t nq,
   for (auto &sl : slots) {
     if (sl.s) (void)hipStreamSynchronize(sl.s);
     if (sl.d) ctx->pool->free(sl.d);
+    if (sl.spread) ctx->pool->free(sl.spread);
     if (sl.h) (void)hipHostFree(sl.h);
     if (sl.done) (void)hipEventDestroy(sl.done);
     if (sl.s) (void)hipStreamDestroy(sl.s);

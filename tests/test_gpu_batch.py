@@ -91,6 +91,8 @@ def test_batch_launch_shapes_agree(dev, tables, tune):
     specs = configs.config3_queries(24, seed=12)
     qs, outs = make_queries(dev, sch, bins, specs)
     plain = plain_results(dev, qs, specs)
+    for i in range(0, len(specs), 2):       # eager loads for every other query
+        qs[i].flags = abi.PLAN_EAGER
     t = abi.Tune(*tune)
     b = C.c_void_p()
     check(dev.lib.mbx_batch_create(dev.ctx, qs, len(specs), C.byref(t), C.byref(b)))

@@ -67,7 +67,7 @@ def _replay(dev, qs, arrivals, cfg, specs):
     return [res[offs[i]:offs[i + 1]] for i in range(len(specs))], lat, st
 
 
-@pytest.mark.parametrize("mode", ["batched", "plain"])
+@pytest.mark.parametrize("mode", ["batched", "fused", "plain"])
 def test_same_alone_and_in_the_mix(dev, tables, mode):
     sch, bins = tables
     specs = configs.config3_queries(80, seed=31)
@@ -77,11 +77,13 @@ def test_same_alone_and_in_the_mix(dev, tables, mode):
         np.testing.assert_array_equal(alone[i], cpuref.query(specs[i], sch)[0])
     rng = np.random.default_rng(5)
     arrivals = np.cumsum(rng.exponential(1 / 200000.0, len(specs)))  # 200k req/s
-    cfg = abi.ServeCfg(4, 16, 64 << 20, 1 if mode == "plain" else 0, 0, abi.Tune())
+    # "fused": every request large enough for its own fused launch
+    cfg = abi.ServeCfg(4, 16, 64 << 20, 1 if mode == "plain" else 0,
+                       1 if mode == "fused" else 0, abi.Tune())
     mixed, lat, st = _replay(dev, qs, arrivals, cfg, specs)
     assert st.completed == len(specs) and st.rejected == 0
     assert np.all(lat > 0) and st.p99_s >= st.p50_s > 0
-    if mode == "batched":
+    if mode != "plain":
         assert st.mean_batch > 1.0          # requests actually shared launches
     for i in range(len(specs)):
         np.testing.assert_array_equal(mixed[i], alone[i], err_msg=specs[i].name)

@@ -1,0 +1,189 @@
+"""The SSB flights on data written by the reference's own generator.
+
+ssb-dbgen-lite.c (the reference's deterministic SSB generator) is compiled
+from its source into oracle/_ref/ by `make -C oracle ref` (git-ignored; the
+tests skip where it is absent).  Its binary column files (4-byte header =
+element size in bytes, ssb-dbgen-lite.c:192-200) are read through the
+library's loader and the 13 flights (ssb/cpuref.c:244-302) are checked:
+
+- CPU: the oracle's straight loops (s1ref..s4ref restated) == the
+  descriptor form the GPU path uses, on the generated scale-1 data;
+- GPU: the reference's sparse and dense bin sets (ssb/bmpidx-create.cu:8-30)
+  are built with create_bin (FK projection for customer city and part
+  mfgr), each flight is planned with col::from_bin (edge bins -> candidate
+  checks) and run as fused, eager, materialise and scan plans; every result
+  equals the oracle's.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from merle_amd import abi, load, spec as S, ssb as ssbq
+from merle_amd.host import check
+
+from oracle import cpuref
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DBGEN = os.path.join(ROOT, "oracle", "_ref", "dbgen-lite")
+FACT = ("loCustKey", "loPartKey", "loOrderDate", "loExtendedPrice",
+        "loRevenue", "loSupplyCost", "loQuantity", "loDiscount", "loSuppCity")
+DIMS = ("custCity", "partMfgr")
+
+
+def _dbgen_dir(tmp_path_factory):
+    if not os.path.exists(DBGEN) and os.path.isdir("/root/reference"):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"],
+                       check=False, capture_output=True)
+    if not os.path.exists(DBGEN):
+        pytest.skip("oracle/_ref/dbgen-lite not built (needs the reference source)")
+    d = tmp_path_factory.mktemp("ssb1")
+    subprocess.run([DBGEN, "-s", "1", "-r", "7", "-b", "-d", str(d)], check=True,
+                   capture_output=True)
+    return str(d)
+
+
+def _read(path):
+    lib = load()
+    p, nb, h = C.c_void_p(), C.c_size_t(), C.c_uint32()
+    check(lib.mbx_colfile_read(path.encode(), C.byref(p), C.byref(nb), C.byref(h)))
+    dt = {1: np.uint8, 2: np.uint16, 4: np.uint32, 8: np.uint64}[h.value]
+    a = np.frombuffer(C.string_at(p.value, nb.value), dtype=dt).copy()
+    lib.mbx_colfile_free(p)
+    return a
+
+
+@pytest.fixture(scope="module")
+def ssb_host(tmp_path_factory):
+    d = _dbgen_dir(tmp_path_factory)
+    cols = {}
+    for name in FACT + DIMS:
+        a = _read(os.path.join(d, name))
+        raw = np.fromfile(os.path.join(d, name), dtype=np.uint8)
+        assert int.from_bytes(raw[:4].tobytes(), "little") == a.itemsize
+        assert raw.size == 4 + a.nbytes
+        cols[name] = a
+    n = cols["loCustKey"].size
+    assert n == 6_000_000
+    s = S.Schema(n)
+    for k, v in cols.items():
+        s.add(k, v)
+    return s
+
+
+def _ssb_struct(s):
+    r = cpuref.RefSSB()
+    for name, _ in cpuref.RefSSB._fields_:
+        setattr(r, name, s[name].host.ctypes.data)
+    return r
+
+
+def _oracle(s, q):
+    """The flight as a straight loop (s1ref..s4ref restated)."""
+    L = cpuref.lib()
+    st = _ssb_struct(s)
+    out = np.zeros(q.ngrp, dtype=np.int64)
+    if q.kind == 1:
+        out[0] = L.ref_s1(C.byref(st), s.nrows, *q.args)
+    elif q.kind == 2:
+        L.ref_s2(C.byref(st), s.nrows, *q.args, out.ctypes.data)
+    elif q.kind == 3:
+        L.ref_s3(C.byref(st), s.nrows, *q.args, out.ctypes.data)
+    else:
+        L.ref_s4(C.byref(st), s.nrows, *q.args, out.ctypes.data)
+    return out
+
+
+def test_dbgen_flights_loops_equal_descriptors(ssb_host):
+    """On the reference generator's data: straight loops == descriptor form."""
+    for q in ssbq.FLIGHTS:
+        want = _oracle(ssb_host, q)
+        got, _ = cpuref.query(ssbq.flight_spec(q), ssb_host)
+        np.testing.assert_array_equal(got, want, err_msg=q.name)
+        if q.kind == 1:
+            assert want[0] > 0, q.name   # the flight selects rows at all
+
+
+# bin sets of ssb/bmpidx-create.cu:8-30 (sparse, dense) per predicate source
+def _bin_sets():
+    return {
+        "loOrderDate": (ssbq.DATE_SP, ssbq.DATE_DE),
+        "loDiscount": (ssbq.DISCNT_SP, ssbq.DISCNT_DE),
+        "loQuantity": (ssbq.QTY_SP, ssbq.QTY_DE),
+        "loSuppCity": (ssbq.CITY_SP, ssbq.CITY_DE),
+        "custCity": (ssbq.CITY_SP, ssbq.CITY_DE),
+        "partMfgr": (ssbq.MFGR_SP, ssbq.MFGR_DE),
+    }
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("density", ["sparse", "dense"])
+def test_dbgen_flights_on_gpu(dev, ssb_host, density):
+    s = ssb_host
+    for c in s.cols.values():
+        c.dev = dev.upload(c.host)
+    srcs = {"loOrderDate": S.Src("loOrderDate"), "loDiscount": S.Src("loDiscount"),
+            "loQuantity": S.Src("loQuantity"), "loSuppCity": S.Src("loSuppCity"),
+            "custCity": S.Src("custCity", "loCustKey"),
+            "partMfgr": S.Src("partMfgr", "loPartKey")}
+    bins, allocs = {}, []
+    for col, (sp, de) in _bin_sets().items():
+        b = list(sp if density == "sparse" else de)
+        ref = S.colref(s, srcs[col])
+        nb = len(b) - 1
+        outs = (C.c_void_p * nb)()
+        single = C.c_size_t()
+        check(dev.lib.mbx_create_bin(dev.ctx, C.byref(ref), s.nrows,
+                                     (C.c_uint64 * nb)(*b[:-1]),
+                                     (C.c_uint64 * nb)(*b[1:]), nb, outs,
+                                     C.byref(single)))
+        bins[col] = (b, [outs[j] for j in range(nb)])
+        allocs.append(outs[0])
+    out = dev.alloc(8 * 1024, np.int64, 1024)
+    nplans = 0
+    for q in ssbq.FLIGHTS:
+        spec = ssbq.flight_spec(q)
+        want = _oracle(s, q)
+        cols = []
+        for p in spec.preds:
+            b, ptrs = bins[p.src.col]
+            r = S.from_bin(b, p.lo, p.hi)
+            if r is None:
+                # not coverable by this bin set: one exact bitmap instead
+                # (the reference builds such bitmaps per query, recipe::perfect)
+                e = C.c_void_p()
+                ref = S.colref(s, p.src)
+                if p.src.fk:
+                    check(dev.lib.mbx_create_join(dev.ctx, ref.fk, ref.data,
+                                                  ref.elem_bytes, s.nrows,
+                                                  ref.dimsz, p.lo, p.hi, C.byref(e)))
+                else:
+                    check(dev.lib.mbx_create_select(dev.ctx, ref.data,
+                                                    ref.elem_bytes, s.nrows,
+                                                    p.lo, p.hi, C.byref(e)))
+                allocs.append(e.value)
+                cols.append(S.BitmapCol([e.value]))
+                continue
+            mid, l, rr = r
+            cols.append(S.BitmapCol([ptrs[i] for i in mid],
+                                    None if l is None else ptrs[l],
+                                    None if rr is None else ptrs[rr]))
+        prg = S.to_vprg(cols, s.nrows, spec.instrs)
+        op = S.to_rowop(spec, s)
+        for plan in (abi.PLAN_FUSED, abi.PLAN_FUSED | abi.PLAN_EAGER,
+                     abi.PLAN_MATERIALISE, abi.PLAN_SCAN):
+            check(dev.lib.mbx_prog_grpby(dev.ctx, C.byref(prg), C.byref(op),
+                                         spec.nr_grp, out.ptr, plan))
+            dev.sync()
+            np.testing.assert_array_equal(out.host(spec.nr_grp), want,
+                                          err_msg=f"{q.name} plan {plan}")
+            nplans += 1
+    assert nplans == 4 * len(ssbq.FLIGHTS)
+    out.free()
+    for p in allocs:
+        check(dev.lib.mbx_free(dev.ctx, p))
+    for c in s.cols.values():
+        c.dev.free()
+        c.dev = None
