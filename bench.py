@@ -280,11 +280,12 @@ def run_batched(dev, reps=5):
     }
 
 
-def run_serving(dev, nreq=1000, rate=2000.0, budget=256 << 20):
+def run_serving(dev, nreq=1000, rates=(2000.0, 20000.0), budget=256 << 20):
     """Config 5: 600M rows x 10 Zipf columns + 640 bucket bitmaps resident;
-    Poisson arrivals replayed against the serving loop, once with continuous
-    batching over 4 streams and once one request at a time; every request's
-    result must be identical in both."""
+    Poisson arrivals replayed against the serving loop at a moderate and at a
+    saturating rate, once with continuous batching (2 streams, up to 4
+    requests per dispatch, large requests as their own fused launches) and
+    once one request at a time; every request's result must be identical."""
     from merle_amd import abi, configs
     from merle_amd.host import check
     n = configs.C5_ROWS
@@ -295,33 +296,7 @@ def run_serving(dev, nreq=1000, rate=2000.0, budget=256 << 20):
         f"ready in {time.time() - t0:.1f} s")
     specs = configs.config5_queries(nreq)
     qs = plan_queries(dev, query_array(sch, specs, bins, configs.C5_BOUNDS))
-    rng = np.random.default_rng(5)
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
-    arrivals -= arrivals[0]
     tot = sum(q.nr_grp for q in specs)
-    out, results = {}, {}
-    for mode in ("batched", "one_at_a_time"):
-        res = np.zeros(tot, np.int64)
-        lat = np.zeros(nreq)
-        st = abi.ServeStats()
-        cfg = abi.ServeCfg(4, 32, budget, 0 if mode == "batched" else 1, 0,
-                           abi.Tune(8192, 4, 4096, 0))
-        check(dev.lib.mbx_serve_replay(dev.ctx, qs, nreq,
-                                       arrivals.ctypes.data_as(C.POINTER(C.c_double)),
-                                       C.byref(cfg), res.ctypes.data,
-                                       lat.ctypes.data_as(C.POINTER(C.c_double)),
-                                       C.byref(st)))
-        results[mode] = res
-        out[mode] = {
-            "completed": st.completed, "rejected": st.rejected,
-            "wall_s": round(st.wall_s, 4),
-            "G_rows_per_s": round(n * st.completed / st.wall_s / 1e9, 2),
-            "p50_ms": round(st.p50_s * 1e3, 3), "p99_ms": round(st.p99_s * 1e3, 3),
-            "mean_ms": round(st.mean_s * 1e3, 3),
-            "batches": st.batches, "mean_batch": round(st.mean_batch, 2),
-            "admission_waits": st.admission_waits,
-            "peak_mem_MB": round(st.peak_mem / 1e6, 2),
-        }
     # algorithmic bytes from a sample of the requests (the accounting kernel
     # is slow), scaled to all of them
     ns = min(nreq, 64)
@@ -329,20 +304,51 @@ def run_serving(dev, nreq=1000, rate=2000.0, budget=256 << 20):
     check(dev.lib.mbx_alg_bytes_batch(dev.ctx, qs, ns, C.byref(nbytes),
                                       C.byref(nsel)))
     est = nbytes.value * nreq / ns
-    for mode in out:
-        gbs = est / out[mode]["wall_s"] / 1e9
-        out[mode]["alg_GB_per_s"] = round(gbs, 1)
-        out[mode]["roofline_frac"] = round(gbs / HBM_PEAK_GBS, 4)
+    out, same = {}, True
+    for rate in rates:
+        rng = np.random.default_rng(5)
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
+        arrivals -= arrivals[0]
+        res_by_mode, block = {}, {}
+        for mode in ("batched", "one_at_a_time"):
+            res = np.zeros(tot, np.int64)
+            lat = np.zeros(nreq)
+            st = abi.ServeStats()
+            if mode == "batched":
+                cfg = abi.ServeCfg(2, 4, budget, 0, 0, abi.Tune(8192, 4, 4096, 0))
+            else:
+                cfg = abi.ServeCfg(1, 1, budget, 1, 0, abi.Tune())
+            check(dev.lib.mbx_serve_replay(
+                dev.ctx, qs, nreq, arrivals.ctypes.data_as(C.POINTER(C.c_double)),
+                C.byref(cfg), res.ctypes.data,
+                lat.ctypes.data_as(C.POINTER(C.c_double)), C.byref(st)))
+            res_by_mode[mode] = res
+            gbs = est / st.wall_s / 1e9
+            block[mode] = {
+                "completed": st.completed, "rejected": st.rejected,
+                "wall_s": round(st.wall_s, 4),
+                "G_rows_per_s": round(n * st.completed / st.wall_s / 1e9, 1),
+                "p50_ms": round(st.p50_s * 1e3, 3),
+                "p99_ms": round(st.p99_s * 1e3, 3),
+                "mean_ms": round(st.mean_s * 1e3, 3),
+                "mean_batch": round(st.mean_batch, 2),
+                "peak_mem_MB": round(st.peak_mem / 1e6, 2),
+                "alg_GB_per_s": round(gbs, 1),
+                "roofline_frac": round(gbs / HBM_PEAK_GBS, 4),
+            }
+        same &= bool(np.array_equal(res_by_mode["batched"],
+                                    res_by_mode["one_at_a_time"]))
+        block["batched_vs_one_at_a_time"] = round(
+            block["one_at_a_time"]["wall_s"] / block["batched"]["wall_s"], 3)
+        out[f"rate_{int(rate)}_per_s"] = block
     for a in slabs + [c.dev for c in sch.cols.values()]:
         a.free()
     return {
         "workload": f"{n:,} rows x 10 Zipf(1.2) columns (1,000..100,000 values), "
-                    f"64 bucket bitmaps per column resident; {nreq} requests, "
-                    f"Poisson arrivals at {rate:.0f}/s, memory budget "
-                    f"{budget >> 20} MiB",
+                    f"64 bucket bitmaps per column resident; {nreq} requests per "
+                    f"replay, Poisson arrivals, memory budget {budget >> 20} MiB",
         **out,
-        "identical_results": bool(np.array_equal(results["batched"],
-                                                 results["one_at_a_time"])),
+        "identical_results": same,
         "alg_bytes_note": f"estimated from the first {ns} requests",
     }
 
@@ -387,8 +393,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched / learned / serving sections")
     ap.add_argument("--serve-requests", type=int, default=1000)
-    ap.add_argument("--serve-rate", type=float, default=2000.0,
-                    help="Poisson arrival rate of the serving replay (req/s)")
+    ap.add_argument("--serve-rates", default="2000,20000",
+                    help="Poisson arrival rates of the serving replays (req/s)")
     ap.add_argument("--learned-programs", type=int, default=2000,
                     help="training programs of the plan chooser (20000 = full)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -541,7 +547,9 @@ def main():
         result["learned"] = run_learned(dev, args.learned_programs)
         log(f"  {result['learned']}")
         log("config 5: serving")
-        result["serving"] = run_serving(dev, args.serve_requests, args.serve_rate)
+        result["serving"] = run_serving(
+            dev, args.serve_requests,
+            tuple(float(r) for r in args.serve_rates.split(",")))
         log(f"  {result['serving']}")
     print(json.dumps(result), flush=True)
     dev.close()
