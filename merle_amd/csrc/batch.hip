@@ -31,7 +31,7 @@ struct BatchArgs {
   uint32_t *counter;    // nx counters, kXcdCounterStride apart
 };
 
-template <bool CHK, bool EAGER>
+template <bool CHK, int EAGER>
 __device__ __forceinline__ void batch_item(const BatchQ &q, uint32_t w_begin,
                                            uint32_t w_end, const GroupSink &sink,
                                            uint32_t *s_list) {
@@ -72,12 +72,15 @@ __global__ __launch_bounds__(256) void batch_kernel(BatchArgs a) {
     __syncthreads();
     const uint32_t w_begin = r * q.words_per_item;
     const uint32_t w_end = min(w_begin + q.words_per_item, q.prg.nwords);
+    // eager evaluation only straight into registers (small operand lists of
+    // two-valued programs); other queries take the early-out form (same
+    // bits, fewer registers for the whole kernel)
+    const bool direct = q.eager && q.prg.nfl <= kDirectOps;
     if (q.chk) {
-      if (q.eager) batch_item<true, true>(q, w_begin, w_end, sink, s_list[wv]);
-      else batch_item<true, false>(q, w_begin, w_end, sink, s_list[wv]);
+      batch_item<true, 0>(q, w_begin, w_end, sink, s_list[wv]);
     } else {
-      if (q.eager) batch_item<false, true>(q, w_begin, w_end, sink, s_list[wv]);
-      else batch_item<false, false>(q, w_begin, w_end, sink, s_list[wv]);
+      if (direct) batch_item<false, 2>(q, w_begin, w_end, sink, s_list[wv]);
+      else batch_item<false, 0>(q, w_begin, w_end, sink, s_list[wv]);
     }
     if (q.lds) {
       __syncthreads();

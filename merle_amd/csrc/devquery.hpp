@@ -213,7 +213,12 @@ __device__ __forceinline__ void apply_ins(uint32_t op, uint32_t d, uint32_t s,
 // Both give the same bits: an empty destination has no uncertain rows (U is
 // a subset of P after every instruction), so ANDM on it yields the same
 // empty pair whether or not the operand is read.
-template <bool CHK, bool ACCT, bool EAGER, int WPL>
+// Eager evaluation with every operand load of the program in flight at
+// once, straight into registers (no per-column accumulators): for programs
+// whose flat operand list has at most kDirectOps bitmaps.
+constexpr uint32_t kDirectOps = 5;
+
+template <bool CHK, bool ACCT, int EAGER, int WPL>
 __device__ __forceinline__ void eval_words(const DevPrg &P, uint32_t w0,
                                            uint32_t ws, uint32_t wend,
                                            uint64_t (&po)[WPL],
@@ -230,6 +235,59 @@ __device__ __forceinline__ void eval_words(const DevPrg &P, uint32_t w0,
     keep[k] = in[k] ? ~0ull : 0ull;
     if (in[k] && w == P.nwords - 1 && (P.nrows & 63u))
       keep[k] = (1ull << (P.nrows & 63u)) - 1ull;
+  }
+  if (EAGER == 2) {
+    // EAGER 2: the flat list holds <= kDirectOps bitmaps; v[e][k] is entry e
+    // at word k; a column's operand is the OR of its entries [beg, end)
+    uint64_t v[kDirectOps][WPL];
+    uint32_t wa[WPL];
+    bool any[WPL];
+#pragma unroll
+    for (int k = 0; k < WPL; ++k) {
+      const uint32_t w = w0 + ws * k;
+      const uint64_t m = __ballot(in[k]);
+      any[k] = m != 0;
+      const uint32_t wf = __shfl(w, any[k] ? (int)__builtin_ctzll(m) : 0, 64);
+      wa[k] = in[k] ? w : wf;
+    }
+#pragma unroll
+    for (uint32_t e = 0; e < kDirectOps; ++e) {
+#pragma unroll
+      for (int k = 0; k < WPL; ++k) v[e][k] = 0;
+      if (e < P.nfl) {
+        const uint64_t *b = P.fl[e];
+#pragma unroll
+        for (int k = 0; k < WPL; ++k)
+          if (any[k]) {
+            if (ACCT && in[k]) mark<true>(acct, b + wa[k]);
+            v[e][k] = ld_stream(b + wa[k]);
+          }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < WPL; ++k) {
+      uint64_t p0 = 0, p1 = 0, p2 = 0, u0 = 0, u1 = 0, u2 = 0;
+#pragma unroll
+      for (uint32_t i = 0; i < MBX_MAXNINSTR; ++i) {
+        if (i >= P.nins) break;
+        const uint32_t op = P.ins[i][0], d = P.ins[i][1], s = P.ins[i][2];
+        uint64_t cs = 0, us = 0;
+        if (op == MBX_ANDM || op == MBX_ORM) {
+          const uint32_t eb = P.beg[s], ee = P.end[s];
+#pragma unroll
+          for (uint32_t e = 0; e < kDirectOps; ++e) {
+            const bool sel = e >= eb && e < ee;
+            const bool edge = P.fledge[e] != 0;
+            cs |= (sel && !edge) ? v[e][k] : 0ull;
+            if (CHK) us |= (sel && edge) ? v[e][k] : 0ull;
+          }
+        }
+        apply_ins<CHK>(op, d, s, cs, us, p0, p1, p2, u0, u1, u2);
+      }
+      po[k] = p0 & keep[k];
+      uo[k] = CHK ? (u0 & keep[k]) : 0;
+    }
+    return;
   }
   if (EAGER) {
     uint64_t pc[MBX_MAXCOLS][WPL], pu[MBX_MAXCOLS][WPL];
@@ -286,7 +344,7 @@ __device__ __forceinline__ void eval_words(const DevPrg &P, uint32_t w0,
   }
 }
 
-template <bool CHK, bool ACCT, bool EAGER = false>
+template <bool CHK, bool ACCT, int EAGER = 0>
 __device__ __forceinline__ void eval_word(const DevPrg &P, uint32_t w,
                                           uint64_t &po, uint64_t &uo,
                                           const Acct *acct) {
@@ -625,7 +683,7 @@ __device__ __forceinline__ void wave_lds_sync() {
 // rounds: a dense range takes more rounds, each using all lanes.
 constexpr uint32_t kWaveList = kListCap / 4;
 
-template <bool CHK, bool ACCT, int WPL, bool EAGER, int E = 2>
+template <bool CHK, bool ACCT, int WPL, int EAGER, int E = 2>
 __device__ __forceinline__ void run_wave_range(const DevPrg &P, const DevOp &O,
                                                uint32_t nr_grp, uint32_t w0,
                                                uint32_t ws, uint32_t wend,
@@ -678,7 +736,7 @@ __device__ __forceinline__ void run_wave_range(const DevPrg &P, const DevOp &O,
 }
 
 // Wave tile wt of 64 x WPL words (the batched kernel's unit).
-template <bool CHK, bool ACCT, int WPL, bool EAGER>
+template <bool CHK, bool ACCT, int WPL, int EAGER>
 __device__ __forceinline__ void run_wave_tile(const DevPrg &P, const DevOp &O,
                                               uint32_t nr_grp, uint32_t wt,
                                               const GroupSink &sink,
