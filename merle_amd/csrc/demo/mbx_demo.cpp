@@ -1,0 +1,393 @@
+// mbx_demo -- the SSB driver of the reference's demo API
+// (ssb_demoall, main.c:6-12; ssb/fixedlayout.cu, ssb/lambdas.cuh:154-243)
+// written against the mybmpidx mirror.
+//
+//   mbx_demo <ssb_dir> [reps]
+//
+// <ssb_dir> holds the binary column files of ssb-dbgen-lite -b (4-byte
+// header = element size in bytes, ssb-dbgen-lite.c:192-200).  The driver
+// loads the fact and dimension columns it needs, builds the sparse and dense
+// bin sets of ssb/bmpidx-create.cu:8-30 with create_bin (customer city and
+// part mfgr projected through their foreign keys), then runs the 13 flights
+// (the arguments of ssb_cpujoin, ssb/cpuref.c:244-302) on every bin set:
+// col::from_bin plans (edge bins -> candidate checks; one exact bitmap when a
+// range is not coverable), timed per plan as the mean of `reps` runs.  It
+// prints the reference's five-block report (drawFigs/drawFigs.py:15-46):
+// index creation and the SSB block in full, the WAH / RTScan, method and
+// synthetic blocks with headers and n/a values only.  SSB columns: Join =
+// the bitmap-free scan, Ours = fused, PrgStg1 / Stg2 / Stg3 = the three
+// materialise stages; the plans this library does not build (Basefus,
+// NfuStg1, Dedicat) read 0.  Every plan's int64 sums must equal the scan's,
+// or the driver exits with status 1.
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mbx_bmpidx.hpp"
+
+using namespace mybmpidx;
+
+namespace {
+
+struct Col {
+  void *d = nullptr;
+  size_t n = 0;
+  uint32_t eb = 0;
+};
+
+bool load(mbx_ctx *ctx, const std::string &dir, const char *name, Col &c) {
+  size_t nbytes = 0;
+  uint32_t header = 0;
+  const std::string path = dir + "/" + name;
+  if (mbx_colfile_load(ctx, path.c_str(), &c.d, &nbytes, &header) != MBX_OK) {
+    std::fprintf(stderr, "cannot load %s: %s\n", path.c_str(), mbx_last_error());
+    return false;
+  }
+  c.eb = header;
+  c.n = nbytes / header;
+  return true;
+}
+
+// ssb/ssbdemo.h:10-26, day offsets from 1992-01-01
+enum : uint64_t {
+  D920101 = 0, D930101 = 366, D940101 = 731, D940201 = 762, D940204 = 765,
+  D940211 = 772, D940301 = 790, D950101 = 1096, D960101 = 1461,
+  D970101 = 1827, D971201 = 2161, D980101 = 2192, D990101 = 2557
+};
+// ssbDateToYear (ssb/ssbdemo.h:29-39) as bucket bounds: year = #{b <= date}
+const uint64_t kYear[] = {D930101, D940101, D950101, D960101, D970101, D980101};
+
+struct Flight {
+  const char *name;
+  int kind;            // 1..4: s1ref..s4ref
+  uint64_t a[8];
+  uint32_t ngrp;       // NGRP_S* (ssb/ssbdemo.h:79-87)
+};
+const Flight kFlights[] = {
+    {"SSB11", 1, {D930101, D940101, 1, 4, 0, 25}, 1},
+    {"SSB12", 1, {D940101, D940201, 4, 7, 26, 36}, 1},
+    {"SSB13", 1, {D940204, D940211, 5, 8, 26, 36}, 1},
+    {"SSB21", 2, {40, 80, 150, 200}, 280},
+    {"SSB22", 2, {260, 268, 200, 250}, 56},
+    {"SSB23", 2, {260, 261, 50, 100}, 7},
+    {"SSB31", 3, {200, 250, 200, 250, D920101, D980101}, 150},
+    {"SSB32", 3, {190, 200, 190, 200, D920101, D980101}, 600},
+    {"SSB33", 3, {51, 55, 51, 55, D920101, D980101}, 96},
+    {"SSB34", 3, {51, 55, 51, 55, D971201, D980101}, 16},
+    {"SSB41", 4, {150, 200, 150, 200, 0, 400, D920101, D990101}, 350},
+    {"SSB42", 4, {150, 200, 150, 200, 0, 400, D970101, D990101}, 100},
+    {"SSB43", 4, {150, 200, 190, 200, 120, 160, D970101, D990101}, 800},
+};
+
+// predicate sources, in bin-set order
+enum Src { DATE, DISC, QTY, SCITY, CCITY, MFGR, NSRC };
+const char *kSrcName[NSRC] = {"loOrderDate", "loDiscount", "loQuantity",
+                              "loSuppCity",  "custCity",   "partMfgr"};
+// ssb/bmpidx-create.cu:8-30: sparse and dense bin bounds per source
+const std::vector<size_t> kBounds[2][NSRC] = {
+    {{D920101, D930101, D940101, D950101, D960101, D970101, D980101, D990101},
+     {0, 2, 5, 8, 11},
+     {1, 11, 21, 31, 41, 51},
+     {0, 50, 100, 150, 200, 250},
+     {0, 50, 100, 150, 200, 250},
+     {0, 200, 400, 600, 800, 1000}},
+    {{D940101, D940201, D940301, D971201, D980101},
+     {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11},
+     {1, 6, 11, 16, 21, 26, 31, 36, 41, 46, 51},
+     {50, 60, 190, 200},
+     {50, 60, 190, 200},
+     {40, 80, 120, 160, 240, 280}}};
+
+struct Tables {
+  Col c[NSRC], fk_cust, fk_part, extprice, revenue, supplycost;
+  uint32_t nrows = 0;
+  mbx_colref ref(int s) const {
+    mbx_colref r{};
+    r.data = c[s].d;
+    r.elem_bytes = c[s].eb;
+    if (s == CCITY) {
+      r.fk = static_cast<const uint32_t *>(fk_cust.d);
+      r.dimsz = (uint32_t)c[s].n;
+    } else if (s == MFGR) {
+      r.fk = static_cast<const uint32_t *>(fk_part.d);
+      r.dimsz = (uint32_t)c[s].n;
+    }
+    return r;
+  }
+  mbx_colref plain(const Col &col) const {
+    mbx_colref r{};
+    r.data = col.d;
+    r.elem_bytes = col.eb;
+    return r;
+  }
+};
+
+struct Pred {
+  int src;
+  uint64_t lo, hi;
+};
+
+void key(mbx_keypart &k, const mbx_colref &src, uint32_t kind, uint32_t div,
+         int64_t base, int64_t stride) {
+  std::memset(&k, 0, sizeof(k));
+  k.src = src;
+  k.kind = kind;
+  k.div = div;
+  k.base = base;
+  k.stride = stride;
+}
+void year_key(mbx_keypart &k, const mbx_colref &date, uint64_t dmin,
+              int64_t stride) {
+  key(k, date, MBX_KEY_BUCKET, 1, 0, stride);
+  int y = 0;
+  for (uint64_t b : kYear) y += b <= dmin;
+  k.base = y;
+  k.nbounds = 6;
+  for (int i = 0; i < 6; ++i) k.bounds[i] = kYear[i];
+}
+// (kind, div, base, width) of a scaled key term (s3ref/s4ref)
+void scaled(uint64_t lo, uint64_t hi, uint64_t span, uint32_t div,
+            uint32_t &kind, uint32_t &d, int64_t &base, int64_t &width) {
+  if (hi - lo >= span) {
+    kind = MBX_KEY_DIV;
+    d = div;
+    base = lo / div;
+    width = hi / div - lo / div;
+  } else {
+    kind = MBX_KEY_IDENT;
+    d = 1;
+    base = lo;
+    width = hi - lo;
+  }
+}
+
+// The flight as predicates + row op: group formulas of s1ref..s4ref
+// (ssb/cpuref.c:141-242) and s1op..s4op (ssb/lambdas.cuh:19-152).
+std::vector<Pred> flight_op(const Flight &f, const Tables &t, mbx_rowop &op) {
+  std::memset(&op, 0, sizeof(op));
+  const uint64_t *a = f.a;
+  std::vector<Pred> p;
+  if (f.kind == 1) {
+    p = {{DATE, a[0], a[1]}, {DISC, a[2], a[3]}, {QTY, a[4], a[5]}};
+    op.measure = MBX_MEAS_MUL;
+    op.ma = t.plain(t.extprice);
+    op.mb = t.ref(DISC);
+  } else if (f.kind == 2) {
+    p = {{MFGR, a[0], a[1]}, {SCITY, a[2], a[3]}};
+    op.nkeys = 2;
+    key(op.keys[0], t.ref(MFGR), MBX_KEY_IDENT, 1, (int64_t)a[0], 1);
+    year_key(op.keys[1], t.ref(DATE), 0, (int64_t)(a[1] - a[0]));
+    op.keys[1].base = 0;
+    op.measure = MBX_MEAS_COL;
+    op.ma = t.plain(t.revenue);
+  } else if (f.kind == 3) {
+    p = {{CCITY, a[0], a[1]}, {SCITY, a[2], a[3]}, {DATE, a[4], a[5]}};
+    uint32_t ck, cd, sk, sd;
+    int64_t cb, ca, sb, sw;
+    if (a[1] - a[0] >= 50) {   // both cities scale by 10 when the c-range does
+      ck = sk = MBX_KEY_DIV;
+      cd = sd = 10;
+      cb = a[0] / 10; ca = a[1] / 10 - a[0] / 10;
+      sb = a[2] / 10; sw = a[3] / 10 - a[2] / 10;
+    } else {
+      ck = sk = MBX_KEY_IDENT;
+      cd = sd = 1;
+      cb = a[0]; ca = a[1] - a[0];
+      sb = a[2]; sw = a[3] - a[2];
+    }
+    op.nkeys = 3;
+    year_key(op.keys[0], t.ref(DATE), a[4], ca * sw);
+    key(op.keys[1], t.ref(CCITY), ck, cd, cb, ca);
+    key(op.keys[2], t.ref(SCITY), sk, sd, sb, 1);
+    op.measure = MBX_MEAS_COL;
+    op.ma = t.plain(t.revenue);
+  } else {
+    p = {{CCITY, a[0], a[1]}, {SCITY, a[2], a[3]}, {MFGR, a[4], a[5]},
+         {DATE, a[6], a[7]}};
+    uint32_t sk, sd, pk, pd;
+    int64_t sb, sa, pb, pw;
+    scaled(a[2], a[3], 50, 10, sk, sd, sb, sa);
+    scaled(a[4], a[5], 200, 40, pk, pd, pb, pw);
+    op.nkeys = 3;
+    year_key(op.keys[0], t.ref(DATE), a[6], sa * pw);
+    key(op.keys[1], t.ref(SCITY), sk, sd, sb, sa);
+    key(op.keys[2], t.ref(MFGR), pk, pd, pb, 1);
+    op.measure = MBX_MEAS_SUB;
+    op.ma = t.plain(t.revenue);
+    op.mb = t.plain(t.supplycost);
+  }
+  op.npreds = (uint32_t)p.size();
+  for (size_t i = 0; i < p.size(); ++i) {
+    op.preds[i].src = t.ref(p[i].src);
+    op.preds[i].lo = p[i].lo;
+    op.preds[i].hi = p[i].hi;
+  }
+  return p;
+}
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  if (argc < 2) {
+    std::fprintf(stderr, "usage: %s <ssb_dir> [reps]\n", argv[0]);
+    return 2;
+  }
+  const std::string dir = argv[1];
+  const int reps = argc > 2 ? std::atoi(argv[2]) : 20;
+  mbx_ctx *ctx = context();
+  if (!ctx) {
+    std::fprintf(stderr, "no device: %s\n", mbx_last_error());
+    return 2;
+  }
+  Tables t;
+  bool ok = true;
+  for (int s = 0; s < NSRC; ++s) ok = ok && load(ctx, dir, kSrcName[s], t.c[s]);
+  ok = ok && load(ctx, dir, "loCustKey", t.fk_cust) &&
+       load(ctx, dir, "loPartKey", t.fk_part) &&
+       load(ctx, dir, "loExtendedPrice", t.extprice) &&
+       load(ctx, dir, "loRevenue", t.revenue) &&
+       load(ctx, dir, "loSupplyCost", t.supplycost);
+  if (!ok) return 2;
+  t.nrows = (uint32_t)t.c[DATE].n;
+  mbx_timer *tm = nullptr;
+  mbx_timer_create(ctx, &tm);
+
+  // ---------------------------------------------- block 1: index creation
+  std::vector<uint *> bins[2][NSRC];
+  std::printf("Case\tWAH\tRTScan\n");
+  for (const Flight &f : kFlights) std::printf("%s\t99999.9\t99999.9\n", f.name);
+  std::printf("\nColumn\tMy(ms)\tWAH(ms)\tMy(MB)\tWAH(MB)\n");
+  const char *kSet[2] = {"Sparse", "Dense"};
+  for (int d = 0; d < 2; ++d) {
+    double tot_ms = 0, tot_mb = 0;
+    for (int s = 0; s < NSRC; ++s) {
+      const std::vector<size_t> &b = kBounds[d][s];
+      const size_t nb = b.size() - 1;
+      std::vector<uint64_t> lo(b.begin(), b.end() - 1), hi(b.begin() + 1, b.end());
+      bins[d][s].assign(nb, nullptr);
+      const mbx_colref r = t.ref(s);
+      mbx_sync(ctx);
+      const double t0 = now_ms();   // like the reference, allocation included
+      const size_t bytes =
+          create_bin(r.fk, r.data, 8 * r.elem_bytes, t.nrows, r.dimsz,
+                     lo.data(), hi.data(), nb, bins[d][s].data());
+      mbx_sync(ctx);
+      const double ms = now_ms() - t0;
+      if (!bytes) {
+        std::fprintf(stderr, "create_bin %s: %s\n", kSrcName[s], mbx_last_error());
+        return 2;
+      }
+      const double mb = (double)bytes * nb / (1 << 20);
+      tot_ms += ms;
+      tot_mb += mb;
+      std::printf("%s_%s\t%.2f\t0\t%.2f\t0\n", kSrcName[s], kSet[d], ms, mb);
+    }
+    std::printf("Total_%s\t%.2f\t0\t%.2f\t0\n", kSet[d], tot_ms, tot_mb);
+  }
+
+  // ------------------------------------------------------ block 2: SSB
+  std::printf("\nCase\tBinType\tJoin\tOurs\tBasefus\tNfuStg1\tPrgStg1\tStg2\tStg3\tDedicat");
+  const size_t nw = mbx_bitmap_stride_words(t.nrows);
+  uint32_t *poss = nullptr, *unc = nullptr, *rows = nullptr, *count = nullptr;
+  int64_t *out = nullptr, *want = nullptr;
+  if (mbx_alloc(ctx, nw * 4, (void **)&poss) || mbx_alloc(ctx, nw * 4, (void **)&unc) ||
+      mbx_alloc(ctx, (size_t)t.nrows * 4, (void **)&rows) ||
+      mbx_alloc(ctx, 256, (void **)&count) || mbx_alloc(ctx, 8 * 1024, (void **)&out) ||
+      mbx_alloc(ctx, 8 * 1024, (void **)&want)) {
+    std::fprintf(stderr, "alloc: %s\n", mbx_last_error());
+    return 2;
+  }
+  int bad = 0;
+  std::vector<int64_t> h_want(1024), h_got(1024);
+  auto timed = [&](auto &&fn) {
+    float ms = 0, sum = 0;
+    for (int r = 0; r < reps; ++r) {
+      mbx_timer_start(ctx, tm);
+      if (fn() != MBX_OK) {
+        std::fprintf(stderr, "launch: %s\n", mbx_last_error());
+        std::exit(2);
+      }
+      mbx_timer_stop(ctx, tm, &ms);
+      sum += ms;
+    }
+    return sum / reps;
+  };
+  for (const Flight &f : kFlights) {
+    mbx_rowop op;
+    const std::vector<Pred> preds = flight_op(f, t, op);
+    for (int d = 0; d < 2; ++d) {
+      vprg p;
+      p.factsz = t.nrows;
+      std::vector<uint *> own;   // exact bitmaps made for this plan
+      for (size_t i = 0; i < preds.size(); ++i) {
+        const Pred &pr = preds[i];
+        const std::vector<size_t> &b = kBounds[d][pr.src];
+        col c = col::from_bin(b.size() - 1, bins[d][pr.src].data(), b.data(),
+                              pr.lo, pr.hi);
+        if (c.empty()) {   // not coverable: one exact bitmap (recipe::perfect)
+          const mbx_colref r = t.ref(pr.src);
+          uint *e = r.fk ? create_join(r.fk, r.data, 8 * r.elem_bytes, t.nrows,
+                                       r.dimsz, (uint)pr.lo, (uint)pr.hi)
+                         : create_select(r.data, 8 * r.elem_bytes, t.nrows,
+                                         (uint)pr.lo, (uint)pr.hi);
+          if (!e) {
+            std::fprintf(stderr, "exact bitmap: %s\n", mbx_last_error());
+            return 2;
+          }
+          own.push_back(e);
+          c.middle[0] = e;
+        }
+        p.cols[i] = c;
+        p.instrs[i].opcode = i == 0 ? vprg::ORM : vprg::ANDM;
+        p.instrs[i].dstreg = 0;
+        p.instrs[i].src1 = (uint8_t)i;
+      }
+      const mbx_vprg ap = p.to_abi();
+      const uint32_t g = f.ngrp;
+      const float join = timed([&] {
+        return mbx_prog_grpby(ctx, &ap, &op, g, want, MBX_PLAN_SCAN);
+      });
+      const float ours = timed([&] {
+        return mbx_prog_grpby(ctx, &ap, &op, g, out, MBX_PLAN_FUSED);
+      });
+      mbx_memcpy_d2h(ctx, h_want.data(), want, 8 * g);
+      mbx_memcpy_d2h(ctx, h_got.data(), out, 8 * g);
+      bad += std::memcmp(h_want.data(), h_got.data(), 8 * g) != 0;
+      const float s1 = timed([&] {
+        return mbx_prog_eval(ctx, &ap, poss, p.is_nochk() ? nullptr : unc);
+      });
+      const float s2 = timed([&] {
+        return mbx_compact(ctx, poss, p.is_nochk() ? nullptr : unc, t.nrows,
+                           rows, count);
+      });
+      const float s3 = timed([&] {
+        return mbx_rows_grpby(ctx, rows, count, &ap, &op, g, out, 0);
+      });
+      mbx_memcpy_d2h(ctx, h_got.data(), out, 8 * g);
+      bad += std::memcmp(h_want.data(), h_got.data(), 8 * g) != 0;
+      std::printf("\n%s\t%s\t%.4f\t%.4f\t0\t0\t%.4f\t%.4f\t%.4f\t0", f.name,
+                  kSet[d], join, ours, s1, s2, s3);
+      for (uint *e : own) free_bitmap(e);
+    }
+  }
+  // blocks 3 (method 1 / method 2) and 4 (synthetic) of the report: this
+  // driver runs the SSB part only, so they carry their headers alone
+  std::printf("\n\nSeletiv\tMethod1\tMethod2\n\nSkew\tBitW\tSeletiv\tJoin\t"
+              "Perfect\tManyOrs\tCandChk\tPftStg1\tPftStg2\tPftStg3\tOrsStg1\t"
+              "OrsStg2\tOrsStg3\tChkStg1\tChkStg2\tChkStg3\tWAH\n");
+  for (int d = 0; d < 2; ++d)
+    for (int s = 0; s < NSRC; ++s)
+      if (!bins[d][s].empty()) free_bitmap(bins[d][s][0]);
+  mbx_timer_destroy(tm);
+  if (bad) std::fprintf(stderr, "%d plan results differ from the scan\n", bad);
+  return bad ? 1 : 0;
+}

@@ -187,3 +187,25 @@ def test_dbgen_flights_on_gpu(dev, ssb_host, density):
     for c in s.cols.values():
         c.dev.free()
         c.dev = None
+
+
+@pytest.mark.gpu
+def test_demo_driver_report(ssb_host, tmp_path_factory):
+    """The C++ driver (merle_amd/lib/mbx_demo, the reference's ssb_demoall
+    on the mybmpidx mirror) runs every flight on both bin sets, checks each
+    plan against the scan plan (exit status 0) and prints the five-block
+    report with one SSB row per flight and bin set."""
+    demo = os.path.join(ROOT, "merle_amd", "lib", "mbx_demo")
+    if not os.path.exists(demo):
+        pytest.skip("mbx_demo not built")
+    d = _dbgen_dir(tmp_path_factory)
+    r = subprocess.run([demo, d, "3"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    blocks = r.stdout.strip("\n").split("\n\n")
+    assert len(blocks) == 5
+    ssb_rows = blocks[2].splitlines()
+    assert ssb_rows[0].split("\t")[:4] == ["Case", "BinType", "Join", "Ours"]
+    assert len(ssb_rows) == 1 + 2 * len(ssbq.FLIGHTS)
+    for row in ssb_rows[1:]:
+        f = row.split("\t")
+        assert len(f) == 10 and float(f[3]) > 0
