@@ -102,10 +102,17 @@ int make_dev_prg(const mbx_vprg *p, DevPrg *out) {
       if (d.r) add(d.r, true);   // an empty column reads as all-zero
     }
     out->end[c] = (uint8_t)out->nfl;
+    if (out->end[c] > out->beg[c]) out->flend[out->end[c] - 1] = 1;
   }
   out->nrows = p->factsz;
   out->nwords = (uint32_t)ceil_div(p->factsz, 64);
   out->chk = chk ? 1u : 0u;
+  bool conj = !chk && n >= 1 && out->ins[0][0] == MBX_ORM && out->ins[0][1] == 0;
+  for (uint32_t i = 1; conj && i < n; ++i)
+    conj = out->ins[i][0] == MBX_ANDM && out->ins[i][1] == 0;
+  for (uint32_t c = 0; conj && c < MBX_MAXCOLS; ++c)
+    if ((out->memcols >> c) & 1u) conj = out->end[c] > out->beg[c];
+  out->conj = conj ? 1u : 0u;
   return MBX_OK;
 }
 

@@ -236,6 +236,61 @@ __device__ __forceinline__ void eval_words(const DevPrg &P, uint32_t w0,
     if (in[k] && w == P.nwords - 1 && (P.nrows & 63u))
       keep[k] = (1ull << (P.nrows & 63u)) - 1ull;
   }
+  if (EAGER == 3) {
+    // EAGER 3: a two-valued conjunction (P.conj): r0 = AND over columns of
+    // the OR of their bins.  The flat list is walked in groups of kOpGroup,
+    // every load of a group in flight at once; per word the work is one OR
+    // per bitmap and one AND per column -- no program interpretation.
+    uint32_t wa[WPL];
+    bool any[WPL];
+    uint64_t acc[WPL], cur[WPL];
+#pragma unroll
+    for (int k = 0; k < WPL; ++k) {
+      const uint32_t w = w0 + ws * k;
+      const uint64_t m = __ballot(in[k]);
+      any[k] = m != 0;
+      const uint32_t wf = __shfl(w, any[k] ? (int)__builtin_ctzll(m) : 0, 64);
+      wa[k] = in[k] ? w : wf;
+      acc[k] = keep[k];
+      cur[k] = 0;
+    }
+    for (uint32_t g = 0; g < P.nfl; g += kOpGroup) {
+      uint64_t v[kOpGroup][WPL];
+#pragma unroll
+      for (int j = 0; j < kOpGroup; ++j) {
+#pragma unroll
+        for (int k = 0; k < WPL; ++k) v[j][k] = 0;
+        if (g + j < P.nfl) {
+          const uint64_t *b = P.fl[g + j];
+#pragma unroll
+          for (int k = 0; k < WPL; ++k)
+            if (any[k]) {
+              if (ACCT && in[k]) mark<true>(acct, b + wa[k]);
+              v[j][k] = ld_stream(b + wa[k]);
+            }
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < kOpGroup; ++j) {
+        if (g + j >= P.nfl) break;
+#pragma unroll
+        for (int k = 0; k < WPL; ++k) cur[k] |= v[j][k];
+        if (P.flend[g + j]) {
+#pragma unroll
+          for (int k = 0; k < WPL; ++k) {
+            acc[k] &= cur[k];
+            cur[k] = 0;
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < WPL; ++k) {
+      po[k] = acc[k];
+      uo[k] = 0;
+    }
+    return;
+  }
   if (EAGER == 2) {
     // EAGER 2: the flat list holds <= kDirectOps bitmaps; v[e][k] is entry e
     // at word k; a column's operand is the OR of its entries [beg, end)

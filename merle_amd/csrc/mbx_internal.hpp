@@ -116,7 +116,11 @@ struct DevPrg {
   uint8_t flcol[kMaxOperands];
   uint8_t fledge[kMaxOperands];
   uint8_t beg[MBX_MAXCOLS], end[MBX_MAXCOLS];
+  uint8_t flend[kMaxOperands];   // entry is the last of its column
   uint32_t nfl;
+  // the program is a two-valued conjunction: ORM r0 <- c, then ANDM r0 <- c'
+  // ... (every column non-empty): r0 = AND over columns of OR of their bins
+  uint32_t conj;
 };
 struct DevSrc {
   const void *data;

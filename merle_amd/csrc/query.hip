@@ -308,7 +308,8 @@ static void fused_launch(mbx_ctx *ctx, const QArgs &a, size_t lds) {
 
 template <bool C, bool L>
 static void fused_modes(mbx_ctx *ctx, const QArgs &a, size_t lds, int mode) {
-  if (mode != 0) fused_launch<C, L, false, 1>(ctx, a, lds);
+  if (!C && mode == 3) fused_launch<C, L, false, C ? 1 : 3>(ctx, a, lds);
+  else if (mode != 0) fused_launch<C, L, false, 1>(ctx, a, lds);
   else fused_launch<C, L, false, 0>(ctx, a, lds);
 }
 
@@ -323,12 +324,10 @@ int launch_fused(mbx_ctx *ctx, const QueryPlan &q, const Acct *acct) {
   a.sp = spread_of(ctx, q);
   const size_t lds = acct ? 0 : grp_lds(q);
   const bool chk = q.prg.chk != 0;
-  // evaluation: 0 early-out, 1 eager through per-column accumulators.
-  // (Mode 2, eager straight into registers, gives every program one pass
-  // over the bitmaps at 5 waves per SIMD but measured no faster on the
-  // star-schema queries than mode 1's two passes at 3; the batched kernel
-  // uses it for its lower register count.)
-  const int mode = q.eager ? 1 : 0;
+  // evaluation: 0 early-out; eager: 3 for two-valued conjunctions (ANDs of
+  // column ORs, no program interpretation), else 1 (per-column
+  // accumulators, then the program word by word)
+  const int mode = !q.eager ? 0 : (q.prg.conj ? 3 : 1);
   // accounting runs the early-out form: the compulsory-line model
   if (acct) {
     if (chk) fused_launch<true, false, true, 0>(ctx, a, lds);
