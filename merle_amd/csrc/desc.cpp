@@ -38,6 +38,59 @@ static int check_bmp(const uint32_t *p) {
 
 // vprg validation follows the reference's limits (primitive.cuh:121,164-198):
 // registers r0..r2, column operands 0..3, at most 6 instructions.
+// Symbolic run of a two-valued program: each register holds 0 (empty), a
+// conjunction of column literals (bit c: column c, bit 4+c: NOT column c),
+// or "other".  True (and flneg set per column) when r0 ends as a non-empty
+// conjunction in which no column appears both plain and negated and every
+// column has bins -- then r0 = AND of the literals, word for word.
+static bool conj_literals(DevPrg &P, bool chk) {
+  if (chk) return false;
+  enum { kZero = 0, kConj = 1, kOther = 2 };
+  int kind[3] = {kZero, kZero, kZero};
+  uint32_t lit[3] = {0, 0, 0};
+  for (uint32_t i = 0; i < P.nins; ++i) {
+    const uint32_t op = P.ins[i][0], d = P.ins[i][1], s = P.ins[i][2];
+    switch (op) {
+    case MBX_ORM:
+      if (kind[d] == kZero) { kind[d] = kConj; lit[d] = 1u << s; }
+      else { kind[d] = kOther; }
+      break;
+    case MBX_ANDM:
+      if (kind[d] == kConj) lit[d] |= 1u << s;
+      else if (kind[d] != kZero) kind[d] = kOther;   // 0 & x stays 0
+      break;
+    case MBX_NOT:
+      if (kind[s] == kConj && (lit[s] & (lit[s] - 1)) == 0) {
+        kind[d] = kConj;   // NOT of one literal
+        lit[d] = (lit[s] & 0xfu) ? lit[s] << 4 : lit[s] >> 4;
+      } else {
+        kind[d] = kOther;
+      }
+      break;
+    case MBX_AND:
+      if (kind[d] == kConj && kind[s] == kConj) lit[d] |= lit[s];
+      else if (kind[d] == kZero || kind[s] == kZero) { kind[d] = kZero; lit[d] = 0; }
+      else kind[d] = kOther;
+      break;
+    default:   // OR: a disjunction
+      kind[d] = kOther;
+      break;
+    }
+  }
+  if (kind[0] != kConj) return false;
+  const uint32_t pos = lit[0] & 0xfu, neg = lit[0] >> 4;
+  if (pos & neg) return false;
+  // the flat list holds exactly the columns of memory instructions; every
+  // one of them must be a literal of r0 (none may be read only elsewhere)
+  if ((pos | neg) != P.memcols) return false;
+  for (uint32_t c = 0; c < MBX_MAXCOLS; ++c) {
+    if (!((P.memcols >> c) & 1u)) continue;
+    if (P.end[c] <= P.beg[c]) return false;   // empty column: generic path
+    P.flneg[P.end[c] - 1] = (neg >> c) & 1u;
+  }
+  return true;
+}
+
 int make_dev_prg(const mbx_vprg *p, DevPrg *out) {
   if (!p || !out) return fail(MBX_ERR_INVALID, "vprg: null");
   if (p->factsz == 0) return fail(MBX_ERR_INVALID, "vprg: factsz == 0");
@@ -107,12 +160,7 @@ int make_dev_prg(const mbx_vprg *p, DevPrg *out) {
   out->nrows = p->factsz;
   out->nwords = (uint32_t)ceil_div(p->factsz, 64);
   out->chk = chk ? 1u : 0u;
-  bool conj = !chk && n >= 1 && out->ins[0][0] == MBX_ORM && out->ins[0][1] == 0;
-  for (uint32_t i = 1; conj && i < n; ++i)
-    conj = out->ins[i][0] == MBX_ANDM && out->ins[i][1] == 0;
-  for (uint32_t c = 0; conj && c < MBX_MAXCOLS; ++c)
-    if ((out->memcols >> c) & 1u) conj = out->end[c] > out->beg[c];
-  out->conj = conj ? 1u : 0u;
+  out->conj = conj_literals(*out, chk) ? 1u : 0u;
   return MBX_OK;
 }
 

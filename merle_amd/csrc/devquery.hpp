@@ -237,10 +237,11 @@ __device__ __forceinline__ void eval_words(const DevPrg &P, uint32_t w0,
       keep[k] = (1ull << (P.nrows & 63u)) - 1ull;
   }
   if (EAGER == 3) {
-    // EAGER 3: a two-valued conjunction (P.conj): r0 = AND over columns of
-    // the OR of their bins.  The flat list is walked in groups of kOpGroup,
-    // every load of a group in flight at once; per word the work is one OR
-    // per bitmap and one AND per column -- no program interpretation.
+    // EAGER 3: a two-valued conjunction of column literals (P.conj): r0 =
+    // AND over columns of the OR of their bins, complemented where flneg.
+    // The flat list is walked in groups of kOpGroup, every load of a group
+    // in flight at once; per word the work is one OR per bitmap and one AND
+    // per column -- no program interpretation.
     uint32_t wa[WPL];
     bool any[WPL];
     uint64_t acc[WPL], cur[WPL];
@@ -276,9 +277,10 @@ __device__ __forceinline__ void eval_words(const DevPrg &P, uint32_t w0,
 #pragma unroll
         for (int k = 0; k < WPL; ++k) cur[k] |= v[j][k];
         if (P.flend[g + j]) {
+          const uint64_t flip = P.flneg[g + j] ? ~0ull : 0ull;
 #pragma unroll
           for (int k = 0; k < WPL; ++k) {
-            acc[k] &= cur[k];
+            acc[k] &= cur[k] ^ flip;
             cur[k] = 0;
           }
         }

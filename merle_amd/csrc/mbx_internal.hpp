@@ -117,9 +117,11 @@ struct DevPrg {
   uint8_t fledge[kMaxOperands];
   uint8_t beg[MBX_MAXCOLS], end[MBX_MAXCOLS];
   uint8_t flend[kMaxOperands];   // entry is the last of its column
+  uint8_t flneg[kMaxOperands];   // (last entries) the column enters negated
   uint32_t nfl;
-  // the program is a two-valued conjunction: ORM r0 <- c, then ANDM r0 <- c'
-  // ... (every column non-empty): r0 = AND over columns of OR of their bins
+  // the program computes, two-valued, a conjunction of column literals:
+  // r0 = AND over columns c of (OR of c's bins), complemented where flneg
+  // (e.g. ORM r0<-a; ANDM r0<-b; ORM r1<-c; NOT r1; AND r0,r1 = a & b & ~c)
   uint32_t conj;
 };
 struct DevSrc {
