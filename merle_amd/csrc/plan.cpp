@@ -150,13 +150,14 @@ int mbx_plan_heuristic(const mbx_vprg *p, const double *dens, mbx_plan *out) {
     std::stable_sort(out->order, out->order + out->norder,
                      [&](uint8_t a, uint8_t b) { return dens[a] < dens[b]; });
   out->flags = MBX_PLAN_FUSED;
-  // Load every operand up front unless the ANDM early-out would skip
+  // Load every operand up front unless the ANDM early-out would skip most
   // lines: a 1024-row line of the next operand is skipped only when all its
   // rows are already empty, probability (1-d)^1024 for a first operand of
-  // density d; below 10% the early-out saves too little to pay for a
-  // round trip per instruction.
+  // density d; below one half the early-out saves too little to pay for a
+  // round trip per instruction (eager conjunctions are also evaluated
+  // without program interpretation).
   const double d0 = out->reorderable ? dens[out->order[0]] : 1.0;
-  if (std::pow(1.0 - std::min(1.0, std::max(0.0, d0)), 1024.0) < 0.1)
+  if (std::pow(1.0 - std::min(1.0, std::max(0.0, d0)), 1024.0) < 0.5)
     out->flags |= MBX_PLAN_EAGER;
   return MBX_OK;
 }
