@@ -563,16 +563,20 @@ static int alg_bytes_multi(mbx_ctx *ctx, const mbx_query *qs, uint32_t nq,
     ctx->pool->free(dummy_out);
     return st;
   }
-  // Ranges closer than kMergeGap merge (the bins of one slab, read by
-  // different queries, become one range); a gap costs 1 bit per 128 bytes.
-  constexpr uint64_t kMergeGap = 8ull << 30;
+  // Ranges closer than a gap merge (adjacent bins of one slab; bins of one
+  // slab read by different queries); a gap costs 1 bit per 128 bytes, so the
+  // smallest gap that leaves at most kAcctMax ranges is used.
   std::sort(ranges.begin(), ranges.end());
   std::vector<std::pair<uint64_t, uint64_t>> merged;
-  for (const auto &r : ranges) {
-    if (!merged.empty() && r.first <= merged.back().second + kMergeGap)
-      merged.back().second = std::max(merged.back().second, r.second);
-    else
-      merged.push_back(r);
+  for (uint64_t gap = 1ull << 20;; gap <<= 2) {
+    merged.clear();
+    for (const auto &r : ranges) {
+      if (!merged.empty() && r.first <= merged.back().second + gap)
+        merged.back().second = std::max(merged.back().second, r.second);
+      else
+        merged.push_back(r);
+    }
+    if (merged.size() <= (size_t)kAcctMax || gap >= (8ull << 30)) break;
   }
   uint64_t span = 0;
   for (const auto &r : merged) span += r.second - r.first;
