@@ -9,16 +9,17 @@
 // loads the fact and dimension columns it needs, builds the sparse and dense
 // bin sets of ssb/bmpidx-create.cu:8-30 with create_bin (customer city and
 // part mfgr projected through their foreign keys), then runs the 13 flights
-// (the arguments of ssb_cpujoin, ssb/cpuref.c:244-302) on every bin set:
-// col::from_bin plans (edge bins -> candidate checks; one exact bitmap when a
-// range is not coverable), timed per plan as the mean of `reps` runs.  It
-// prints the reference's five-block report (drawFigs/drawFigs.py:15-46):
-// index creation and the SSB block in full, the WAH / RTScan, method and
-// synthetic blocks with headers and n/a values only.  SSB columns: Join =
-// the bitmap-free scan, Ours = fused, PrgStg1 / Stg2 / Stg3 = the three
-// materialise stages; the plans this library does not build (Basefus,
-// NfuStg1, Dedicat) read 0.  Every plan's int64 sums must equal the scan's,
-// or the driver exits with status 1.
+// (the arguments of ssb_cpujoin, ssb/cpuref.c:244-302) under the Sparse,
+// Medium and Dense programs of ssb/fixedlayout.cu (col::from_bin and
+// col::from_bin2; edge bins -> candidate checks), timed per plan as the mean
+// of `reps` runs.  It prints the reference's five-block report
+// (drawFigs/drawFigs.py:15-46): index creation and the SSB block in full,
+// the WAH / RTScan, method and synthetic blocks with headers and n/a values
+// only.  SSB columns: Join = the bitmap-free scan (Sparse rows only, as
+// DOWORK), Ours = fused, PrgStg1 / Stg2 / Stg3 = the three materialise
+// stages; the plans this library does not build (Basefus, NfuStg1, Dedicat)
+// read 0.  Every plan's int64 sums must equal the scan's, or the driver
+// exits with status 1.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -228,6 +229,70 @@ std::vector<Pred> flight_op(const Flight &f, const Tables &t, mbx_rowop &op) {
   return p;
 }
 
+// The programs of s1fix..s4fix (ssb/fixedlayout.cu:35-140): columns,
+// instructions, and the columns that DENSE() (col::from_bin2, sparse bins as
+// the coarse level) replaces in the Medium and Dense programs.
+struct FixedCol {
+  int src;
+  uint64_t lo, hi;
+};
+struct Fixed {
+  std::vector<FixedCol> cols;
+  std::vector<vprg::instr> ins;
+  unsigned medium = 0, dense = 0;   // column bit masks
+};
+vprg::instr ins(vprg::ops o, uint8_t d, uint8_t s) {
+  vprg::instr i;
+  i.opcode = o;
+  i.dstreg = d;
+  i.src1 = s;
+  return i;
+}
+Fixed fixed_layout(const Flight &f) {
+  Fixed x;
+  const uint64_t *a = f.a;
+  const vprg::ops LOAD = vprg::ORM, ANDM = vprg::ANDM;
+  if (f.kind == 1) {
+    // fixedlayout.cu:147 passes quantity [1, 25) for SSB11 where ssb_cpujoin
+    // passes [0, 25); quantity is at least 1 (ssb-dbgen-lite.c:139)
+    x.cols = {{DATE, a[0], a[1]}, {DISC, a[2], a[3]},
+              {QTY, a[4] ? a[4] : 1, a[5]}};
+    x.ins = {ins(LOAD, 0, 0), ins(ANDM, 0, 1), ins(ANDM, 0, 2)};
+    x.medium = 0b110;
+    x.dense = 0b111;
+  } else if (f.kind == 2) {
+    x.cols = {{MFGR, a[0], a[1]}, {SCITY, a[2], a[3]}};
+    x.ins = {ins(LOAD, 0, 0), ins(ANDM, 0, 1)};
+    x.medium = 0b01;
+    x.dense = 0b11;
+  } else if (f.kind == 3) {
+    uint64_t dmin = a[4], dmax = a[5];
+    x.ins = {ins(LOAD, 0, 0), ins(ANDM, 0, 1)};
+    if (dmin == D920101) {   // date < dmax as NOT(date in [dmax, 1999))
+      dmin = dmax;
+      dmax = D990101;
+      x.ins.push_back(ins(LOAD, 1, 2));
+      x.ins.push_back(ins(vprg::NOT, 1, 1));
+      x.ins.push_back(ins(vprg::AND, 0, 1));
+    } else {
+      x.ins.push_back(ins(ANDM, 0, 2));
+    }
+    x.cols = {{CCITY, a[0], a[1]}, {SCITY, a[2], a[3]}, {DATE, dmin, dmax}};
+    x.medium = 0b001;
+    x.dense = 0b111;
+  } else {
+    x.cols = {{CCITY, a[0], a[1]}, {SCITY, a[2], a[3]}, {MFGR, a[4], a[5]}};
+    x.ins = {ins(LOAD, 0, 0), ins(ANDM, 0, 1), ins(ANDM, 0, 2)};
+    if (a[7] < D990101 || a[6] > D920101) {   // a date column only if bounded
+      x.cols.push_back({DATE, a[6], a[7]});
+      x.ins.push_back(ins(ANDM, 0, 3));
+    }
+    x.medium = 0b0101;
+    x.dense = 0b0111;
+  }
+  return x;
+}
+
 double now_ms() {
   return std::chrono::duration<double, std::milli>(
              std::chrono::steady_clock::now().time_since_epoch())
@@ -321,41 +386,49 @@ int main(int argc, char **argv) {
     }
     return sum / reps;
   };
+  const char *kPlan[3] = {"Sparse", "Medium", "Dense"};
   for (const Flight &f : kFlights) {
     mbx_rowop op;
-    const std::vector<Pred> preds = flight_op(f, t, op);
-    for (int d = 0; d < 2; ++d) {
+    flight_op(f, t, op);
+    const Fixed x = fixed_layout(f);
+    // the row op re-checks an uncertain row by running the program over the
+    // exact predicates of its columns, so they follow the program's columns
+    op.npreds = (uint32_t)x.cols.size();
+    for (size_t i = 0; i < x.cols.size(); ++i) {
+      op.preds[i].src = t.ref(x.cols[i].src);
+      op.preds[i].lo = x.cols[i].lo;
+      op.preds[i].hi = x.cols[i].hi;
+    }
+    for (int level = 0; level < 3; ++level) {
+      const unsigned dense = level == 0 ? 0u : level == 1 ? x.medium : x.dense;
       vprg p;
       p.factsz = t.nrows;
-      std::vector<uint *> own;   // exact bitmaps made for this plan
-      for (size_t i = 0; i < preds.size(); ++i) {
-        const Pred &pr = preds[i];
-        const std::vector<size_t> &b = kBounds[d][pr.src];
-        col c = col::from_bin(b.size() - 1, bins[d][pr.src].data(), b.data(),
-                              pr.lo, pr.hi);
-        if (c.empty()) {   // not coverable: one exact bitmap (recipe::perfect)
-          const mbx_colref r = t.ref(pr.src);
-          uint *e = r.fk ? create_join(r.fk, r.data, 8 * r.elem_bytes, t.nrows,
-                                       r.dimsz, (uint)pr.lo, (uint)pr.hi)
-                         : create_select(r.data, 8 * r.elem_bytes, t.nrows,
-                                         (uint)pr.lo, (uint)pr.hi);
-          if (!e) {
-            std::fprintf(stderr, "exact bitmap: %s\n", mbx_last_error());
-            return 2;
-          }
-          own.push_back(e);
-          c.middle[0] = e;
+      for (size_t i = 0; i < x.cols.size(); ++i) {
+        const FixedCol &c = x.cols[i];
+        const std::vector<size_t> &sb = kBounds[0][c.src], &db = kBounds[1][c.src];
+        p.cols[i] = (dense >> i) & 1
+                        ? col::from_bin2(sb.size() - 1, bins[0][c.src].data(),
+                                         sb.data(), db.size() - 1,
+                                         bins[1][c.src].data(), db.data(), c.lo,
+                                         c.hi)
+                        : col::from_bin(sb.size() - 1, bins[0][c.src].data(),
+                                        sb.data(), c.lo, c.hi);
+        if (p.cols[i].empty()) {
+          std::fprintf(stderr, "%s %s: column %zu not coverable\n", f.name,
+                       kPlan[level], i);
+          return 2;
         }
-        p.cols[i] = c;
-        p.instrs[i].opcode = i == 0 ? vprg::ORM : vprg::ANDM;
-        p.instrs[i].dstreg = 0;
-        p.instrs[i].src1 = (uint8_t)i;
       }
+      for (size_t i = 0; i < x.ins.size(); ++i) p.instrs[i] = x.ins[i];
       const mbx_vprg ap = p.to_abi();
       const uint32_t g = f.ngrp;
-      const float join = timed([&] {
-        return mbx_prog_grpby(ctx, &ap, &op, g, want, MBX_PLAN_SCAN);
-      });
+      // the scan is timed once per flight, as DOWORK(..., join = true) does
+      // on the Sparse program only (ssb/lambdas.cuh:157,168)
+      float join = 0;
+      if (level == 0)
+        join = timed([&] {
+          return mbx_prog_grpby(ctx, &ap, &op, g, want, MBX_PLAN_SCAN);
+        });
       const float ours = timed([&] {
         return mbx_prog_grpby(ctx, &ap, &op, g, out, MBX_PLAN_FUSED);
       });
@@ -374,9 +447,11 @@ int main(int argc, char **argv) {
       });
       mbx_memcpy_d2h(ctx, h_got.data(), out, 8 * g);
       bad += std::memcmp(h_want.data(), h_got.data(), 8 * g) != 0;
-      std::printf("\n%s\t%s\t%.4f\t%.4f\t0\t0\t%.4f\t%.4f\t%.4f\t0", f.name,
-                  kSet[d], join, ours, s1, s2, s3);
-      for (uint *e : own) free_bitmap(e);
+      if (level == 0)
+        std::printf("\n%s\t%s\t%.4f", f.name, kPlan[level], join);
+      else
+        std::printf("\n%s\t%s\t0", f.name, kPlan[level]);
+      std::printf("\t%.4f\t0\t0\t%.4f\t%.4f\t%.4f\t0", ours, s1, s2, s3);
     }
   }
   // blocks 3 (method 1 / method 2) and 4 (synthetic) of the report: this

@@ -106,6 +106,8 @@ __device__ __forceinline__ uint64_t ld_bmp(const uint64_t *p, uint32_t w,
 // fetches anyway), and its value is dropped.  Loads behind lane-divergent
 // branches would each be waited for inside their branch.
 constexpr int kOpGroup = 4;
+// Bitmaps per load group of the conjunction form (EAGER 3).
+constexpr int kConjGroup = 4;
 
 template <bool ACCT, bool CHK, int WPL, int NC>
 __device__ __forceinline__ void load_ops(const DevPrg &P, uint32_t e0,
@@ -239,7 +241,7 @@ __device__ __forceinline__ void eval_words(const DevPrg &P, uint32_t w0,
   if (EAGER == 3) {
     // EAGER 3: a two-valued conjunction of column literals (P.conj): r0 =
     // AND over columns of the OR of their bins, complemented where flneg.
-    // The flat list is walked in groups of kOpGroup, every load of a group
+    // The flat list is walked in groups of kConjGroup, every load of a group
     // in flight at once; per word the work is one OR per bitmap and one AND
     // per column -- no program interpretation.
     uint32_t wa[WPL];
@@ -255,10 +257,10 @@ __device__ __forceinline__ void eval_words(const DevPrg &P, uint32_t w0,
       acc[k] = keep[k];
       cur[k] = 0;
     }
-    for (uint32_t g = 0; g < P.nfl; g += kOpGroup) {
-      uint64_t v[kOpGroup][WPL];
+    for (uint32_t g = 0; g < P.nfl; g += kConjGroup) {
+      uint64_t v[kConjGroup][WPL];
 #pragma unroll
-      for (int j = 0; j < kOpGroup; ++j) {
+      for (int j = 0; j < kConjGroup; ++j) {
 #pragma unroll
         for (int k = 0; k < WPL; ++k) v[j][k] = 0;
         if (g + j < P.nfl) {
@@ -272,7 +274,7 @@ __device__ __forceinline__ void eval_words(const DevPrg &P, uint32_t w0,
         }
       }
 #pragma unroll
-      for (int j = 0; j < kOpGroup; ++j) {
+      for (int j = 0; j < kConjGroup; ++j) {
         if (g + j >= P.nfl) break;
 #pragma unroll
         for (int k = 0; k < WPL; ++k) cur[k] |= v[j][k];

@@ -166,3 +166,16 @@ def from_bin(bounds: Sequence[int], lo: int, hi: int):
     if not mid and left is None and right is None:
         return None
     return mid, left, right
+
+
+def from_bin2(coarse: Sequence[int], fine: Sequence[int], lo: int, hi: int):
+    """col::from_bin2 (bmpidx_create.cu:165-174): the coarse bins unless they
+    need a candidate check, then the fine ones.  -> (0 coarse | 1 fine,
+    (middle, left, right)) or None."""
+    r = from_bin(coarse, lo, hi)
+    if r is not None and r[1] is None and r[2] is None:
+        return 0, r
+    if r is None:
+        return None
+    f = from_bin(fine, lo, hi)
+    return None if f is None else (1, f)

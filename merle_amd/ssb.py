@@ -112,3 +112,103 @@ def flight_spec(f: Flight) -> QuerySpec:
             Key(mfgr, pk, pd, pb, 1)]
     return QuerySpec(preds, CONJ[:4], keys, f.ngrp, abi.MEAS_SUB,
                      Src("loRevenue"), Src("loSupplyCost"), f.name)
+
+
+# ---------------------------------------------------------------------------
+# The fixed-layout plans of ssb/fixedlayout.cu (s1fix..s4fix): per flight,
+# three programs over the prebuilt bin sets.  SPARSE(c) = col::from_bin on the
+# sparse bins; DENSE(c) = col::from_bin2 with the sparse bins as the coarse
+# level and the dense bins as the fine one (fixedlayout.cu:6-12).
+
+SPARSE, DENSE = 0, 1
+BIN_SETS = {
+    "loOrderDate": (DATE_SP, DATE_DE),
+    "loDiscount": (DISCNT_SP, DISCNT_DE),
+    "loQuantity": (QTY_SP, QTY_DE),
+    "loSuppCity": (CITY_SP, CITY_DE),
+    "custCity": (CITY_SP, CITY_DE),       # cCitySpBin = sCitySpBin (:4)
+    "partMfgr": (MFGR_SP, MFGR_DE),
+}
+# fixedlayout.cu:147 passes quantity [1, 25) for SSB11 where ssb_cpujoin
+# passes [0, 25) (cpuref.c:252); both select the same rows since quantity is
+# at least 1 (ssb-dbgen-lite.c:139).
+_FIXED_ARGS = {"SSB11": (D930101, D940101, 1, 4, 1, 25)}
+
+
+@dataclass
+class FixedPlan:
+    label: str                               # Sparse / Medium / Dense
+    cols: Tuple[Tuple[str, int, int, int], ...]   # (source, lo, hi, SPARSE|DENSE)
+    instrs: Tuple[Tuple[int, int, int], ...]
+
+
+def fixed_plans(f: Flight):
+    """The Sparse, Medium and Dense programs of s1fix..s4fix
+    (ssb/fixedlayout.cu:35-140).  The row op stays the flight's own
+    (flight_spec): for flight 3 the op keeps the original date range while
+    the program loads the complement (:93-104)."""
+    a = _FIXED_ARGS.get(f.name, f.args)
+    LOAD, ANDM = abi.ORM, abi.ANDM
+    if f.kind == 1:
+        cols = [("loOrderDate", a[0], a[1]), ("loDiscount", a[2], a[3]),
+                ("loQuantity", a[4], a[5])]
+        ins = [(LOAD, 0, 0), (ANDM, 0, 1), (ANDM, 0, 2)]
+        medium, dense = {1, 2}, {0, 1, 2}
+    elif f.kind == 2:
+        cols = [("partMfgr", a[0], a[1]), ("loSuppCity", a[2], a[3])]
+        ins = [(LOAD, 0, 0), (ANDM, 0, 1)]
+        medium, dense = {0}, {0, 1}
+    elif f.kind == 3:
+        dmin, dmax = a[4], a[5]
+        ins = [(LOAD, 0, 0), (ANDM, 0, 1)]
+        if dmin == D920101:      # date < dmax as NOT(date in [dmax, 1999))
+            dmin, dmax = dmax, D990101
+            ins += [(LOAD, 1, 2), (abi.NOT, 1, 1), (abi.AND, 0, 1)]
+        else:
+            ins += [(ANDM, 0, 2)]
+        cols = [("custCity", a[0], a[1]), ("loSuppCity", a[2], a[3]),
+                ("loOrderDate", dmin, dmax)]
+        medium, dense = {0}, {0, 1, 2}
+    else:
+        cols = [("custCity", a[0], a[1]), ("loSuppCity", a[2], a[3]),
+                ("partMfgr", a[4], a[5])]
+        ins = [(LOAD, 0, 0), (ANDM, 0, 1), (ANDM, 0, 2)]
+        if a[7] < D990101 or a[6] > D920101:   # a date column only if bounded
+            cols.append(("loOrderDate", a[6], a[7]))
+            ins.append((ANDM, 0, 3))
+        medium, dense = {0, 2}, {0, 1, 2}
+    out = []
+    for label, lvl in (("Sparse", set()), ("Medium", medium), ("Dense", dense)):
+        out.append(FixedPlan(label, tuple((c, lo, hi, DENSE if i in lvl else SPARSE)
+                                          for i, (c, lo, hi) in enumerate(cols)),
+                             tuple(ins)))
+    return out
+
+
+def fixed_bins(col: str, lo: int, hi: int, level: int):
+    """-> (set index 0 sparse / 1 dense, (middle, left, right)) of SPARSE(c)
+    or DENSE(c), or None when the range is not coverable."""
+    from .spec import from_bin, from_bin2
+    sp, de = BIN_SETS[col]
+    if level == SPARSE:
+        r = from_bin(list(sp), lo, hi)
+        return None if r is None else (0, r)
+    return from_bin2(list(sp), list(de), lo, hi)
+
+
+_SRC = {"loOrderDate": Src("loOrderDate"), "loDiscount": Src("loDiscount"),
+        "loQuantity": Src("loQuantity"), "loSuppCity": Src("loSuppCity"),
+        "custCity": Src("custCity", "loCustKey"),
+        "partMfgr": Src("partMfgr", "loPartKey")}
+
+
+def fixed_spec(f: Flight, plan: FixedPlan) -> QuerySpec:
+    """The flight under one fixed-layout program: the program's instructions
+    and, as the row op's predicates, the exact predicate behind each program
+    column (an uncertain row is re-checked by running the program over them,
+    so flight 3's complemented date column carries its complemented range)."""
+    import dataclasses
+    spec = flight_spec(f)
+    return dataclasses.replace(
+        spec, preds=[Pred(_SRC[c], lo, hi) for c, lo, hi, _ in plan.cols],
+        instrs=list(plan.instrs))

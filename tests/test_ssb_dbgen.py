@@ -10,9 +10,10 @@ library's loader and the 13 flights (ssb/cpuref.c:244-302) are checked:
   descriptor form the GPU path uses, on the generated scale-1 data;
 - GPU: the reference's sparse and dense bin sets (ssb/bmpidx-create.cu:8-30)
   are built with create_bin (FK projection for customer city and part
-  mfgr), each flight is planned with col::from_bin (edge bins -> candidate
-  checks) and run as fused, eager, materialise and scan plans; every result
-  equals the oracle's.
+  mfgr); each flight runs its Sparse, Medium and Dense programs of
+  ssb/fixedlayout.cu (col::from_bin / from_bin2, edge bins -> candidate
+  checks) as fused, eager, materialise and scan plans; every result equals
+  the oracle's.
 """
 import ctypes as C
 import os
@@ -106,21 +107,40 @@ def test_dbgen_flights_loops_equal_descriptors(ssb_host):
             assert want[0] > 0, q.name   # the flight selects rows at all
 
 
-# bin sets of ssb/bmpidx-create.cu:8-30 (sparse, dense) per predicate source
-def _bin_sets():
-    return {
-        "loOrderDate": (ssbq.DATE_SP, ssbq.DATE_DE),
-        "loDiscount": (ssbq.DISCNT_SP, ssbq.DISCNT_DE),
-        "loQuantity": (ssbq.QTY_SP, ssbq.QTY_DE),
-        "loSuppCity": (ssbq.CITY_SP, ssbq.CITY_DE),
-        "custCity": (ssbq.CITY_SP, ssbq.CITY_DE),
-        "partMfgr": (ssbq.MFGR_SP, ssbq.MFGR_DE),
-    }
+def test_dbgen_fixed_programs_equal_loops(ssb_host):
+    """Each fixed-layout program, run by the oracle over the exact predicates
+    of its columns, selects what the straight loop selects -- including
+    flight 3's date < d as NOT(date in [d, 1999)) (fixedlayout.cu:93-97)."""
+    for q in ssbq.FLIGHTS:
+        want = _oracle(ssb_host, q)
+        for fp in ssbq.fixed_plans(q):
+            got, _ = cpuref.query(ssbq.fixed_spec(q, fp), ssb_host)
+            np.testing.assert_array_equal(got, want, err_msg=f"{q.name} {fp.label}")
+
+
+def test_fixed_plans_cover_every_range(lib):
+    """The Sparse / Medium / Dense programs of ssb/fixedlayout.cu cover every
+    predicate with the prebuilt bins alone: no column comes out empty, the
+    middle bins lie inside the range and middle plus edges cover it."""
+    for f in ssbq.FLIGHTS:
+        plans = ssbq.fixed_plans(f)
+        assert [p.label for p in plans] == ["Sparse", "Medium", "Dense"]
+        for p in plans:
+            for col, lo, hi, lvl in p.cols:
+                r = ssbq.fixed_bins(col, lo, hi, lvl)
+                assert r is not None, (f.name, p.label, col)
+                b = ssbq.BIN_SETS[col][r[0]]
+                mid, left, right = r[1]
+                for i in mid:
+                    assert lo <= b[i] and b[i + 1] <= hi
+                covered = sorted(list(mid) + [j for j in (left, right) if j is not None])
+                assert b[covered[0]] <= lo and b[covered[-1] + 1] >= hi
+                assert all(b[j + 1] == b[k] for j, k in zip(covered, covered[1:]))
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("density", ["sparse", "dense"])
-def test_dbgen_flights_on_gpu(dev, ssb_host, density):
+def test_dbgen_flights_on_gpu(dev, ssb_host):
+    """Every flight under its three fixed-layout programs and four plans."""
     s = ssb_host
     for c in s.cols.values():
         c.dev = dev.upload(c.host)
@@ -129,58 +149,42 @@ def test_dbgen_flights_on_gpu(dev, ssb_host, density):
             "custCity": S.Src("custCity", "loCustKey"),
             "partMfgr": S.Src("partMfgr", "loPartKey")}
     bins, allocs = {}, []
-    for col, (sp, de) in _bin_sets().items():
-        b = list(sp if density == "sparse" else de)
-        ref = S.colref(s, srcs[col])
-        nb = len(b) - 1
-        outs = (C.c_void_p * nb)()
-        single = C.c_size_t()
-        check(dev.lib.mbx_create_bin(dev.ctx, C.byref(ref), s.nrows,
-                                     (C.c_uint64 * nb)(*b[:-1]),
-                                     (C.c_uint64 * nb)(*b[1:]), nb, outs,
-                                     C.byref(single)))
-        bins[col] = (b, [outs[j] for j in range(nb)])
-        allocs.append(outs[0])
+    for col, sets in ssbq.BIN_SETS.items():
+        for d, b in enumerate(sets):
+            ref = S.colref(s, srcs[col])
+            nb = len(b) - 1
+            outs = (C.c_void_p * nb)()
+            single = C.c_size_t()
+            check(dev.lib.mbx_create_bin(dev.ctx, C.byref(ref), s.nrows,
+                                         (C.c_uint64 * nb)(*b[:-1]),
+                                         (C.c_uint64 * nb)(*b[1:]), nb, outs,
+                                         C.byref(single)))
+            bins[col, d] = [outs[j] for j in range(nb)]
+            allocs.append(outs[0])
     out = dev.alloc(8 * 1024, np.int64, 1024)
     nplans = 0
     for q in ssbq.FLIGHTS:
-        spec = ssbq.flight_spec(q)
         want = _oracle(s, q)
-        cols = []
-        for p in spec.preds:
-            b, ptrs = bins[p.src.col]
-            r = S.from_bin(b, p.lo, p.hi)
-            if r is None:
-                # not coverable by this bin set: one exact bitmap instead
-                # (the reference builds such bitmaps per query, recipe::perfect)
-                e = C.c_void_p()
-                ref = S.colref(s, p.src)
-                if p.src.fk:
-                    check(dev.lib.mbx_create_join(dev.ctx, ref.fk, ref.data,
-                                                  ref.elem_bytes, s.nrows,
-                                                  ref.dimsz, p.lo, p.hi, C.byref(e)))
-                else:
-                    check(dev.lib.mbx_create_select(dev.ctx, ref.data,
-                                                    ref.elem_bytes, s.nrows,
-                                                    p.lo, p.hi, C.byref(e)))
-                allocs.append(e.value)
-                cols.append(S.BitmapCol([e.value]))
-                continue
-            mid, l, rr = r
-            cols.append(S.BitmapCol([ptrs[i] for i in mid],
-                                    None if l is None else ptrs[l],
-                                    None if rr is None else ptrs[rr]))
-        prg = S.to_vprg(cols, s.nrows, spec.instrs)
-        op = S.to_rowop(spec, s)
-        for plan in (abi.PLAN_FUSED, abi.PLAN_FUSED | abi.PLAN_EAGER,
-                     abi.PLAN_MATERIALISE, abi.PLAN_SCAN):
-            check(dev.lib.mbx_prog_grpby(dev.ctx, C.byref(prg), C.byref(op),
-                                         spec.nr_grp, out.ptr, plan))
-            dev.sync()
-            np.testing.assert_array_equal(out.host(spec.nr_grp), want,
-                                          err_msg=f"{q.name} plan {plan}")
-            nplans += 1
-    assert nplans == 4 * len(ssbq.FLIGHTS)
+        for fp in ssbq.fixed_plans(q):
+            spec = ssbq.fixed_spec(q, fp)
+            op = S.to_rowop(spec, s)
+            cols = []
+            for col, lo, hi, lvl in fp.cols:
+                d, (mid, l, r) = ssbq.fixed_bins(col, lo, hi, lvl)
+                ptrs = bins[col, d]
+                cols.append(S.BitmapCol([ptrs[i] for i in mid],
+                                        None if l is None else ptrs[l],
+                                        None if r is None else ptrs[r]))
+            prg = S.to_vprg(cols, s.nrows, spec.instrs)
+            for plan in (abi.PLAN_FUSED, abi.PLAN_FUSED | abi.PLAN_EAGER,
+                         abi.PLAN_MATERIALISE, abi.PLAN_SCAN):
+                check(dev.lib.mbx_prog_grpby(dev.ctx, C.byref(prg), C.byref(op),
+                                             spec.nr_grp, out.ptr, plan))
+                dev.sync()
+                np.testing.assert_array_equal(out.host(spec.nr_grp), want,
+                                              err_msg=f"{q.name} {fp.label} plan {plan}")
+                nplans += 1
+    assert nplans == 3 * 4 * len(ssbq.FLIGHTS)
     out.free()
     for p in allocs:
         check(dev.lib.mbx_free(dev.ctx, p))
@@ -192,9 +196,9 @@ def test_dbgen_flights_on_gpu(dev, ssb_host, density):
 @pytest.mark.gpu
 def test_demo_driver_report(ssb_host, tmp_path_factory):
     """The C++ driver (merle_amd/lib/mbx_demo, the reference's ssb_demoall
-    on the mybmpidx mirror) runs every flight on both bin sets, checks each
-    plan against the scan plan (exit status 0) and prints the five-block
-    report with one SSB row per flight and bin set."""
+    on the mybmpidx mirror) runs every flight's Sparse / Medium / Dense
+    programs, checks each plan against the scan plan (exit status 0) and
+    prints the five-block report with three SSB rows per flight."""
     demo = os.path.join(ROOT, "merle_amd", "lib", "mbx_demo")
     if not os.path.exists(demo):
         pytest.skip("mbx_demo not built")
@@ -205,7 +209,11 @@ def test_demo_driver_report(ssb_host, tmp_path_factory):
     assert len(blocks) == 5
     ssb_rows = blocks[2].splitlines()
     assert ssb_rows[0].split("\t")[:4] == ["Case", "BinType", "Join", "Ours"]
-    assert len(ssb_rows) == 1 + 2 * len(ssbq.FLIGHTS)
-    for row in ssb_rows[1:]:
+    assert len(ssb_rows) == 1 + 3 * len(ssbq.FLIGHTS)
+    for i, row in enumerate(ssb_rows[1:]):
         f = row.split("\t")
+        assert f[0] == ssbq.FLIGHTS[i // 3].name
+        assert f[1] == ("Sparse", "Medium", "Dense")[i % 3]
         assert len(f) == 10 and float(f[3]) > 0
+        # DOWORK times the scan on the Sparse row only (ssb/lambdas.cuh:157)
+        assert (float(f[2]) > 0) if i % 3 == 0 else f[2] == "0"
