@@ -255,6 +255,14 @@ def run_batched(dev, reps=5):
     nbytes, nsel = C.c_uint64(), C.c_uint64()
     check(dev.lib.mbx_alg_bytes_batch(dev.ctx, qs, len(specs), C.byref(nbytes),
                                       C.byref(nsel)))
+    # the same bytes counted per query (a line two queries read counts twice):
+    # what the launch reads, most of it served from L2 / Infinity Cache
+    per_query = 0
+    for i in range(len(specs)):
+        qb, qk = C.c_uint64(), C.c_uint64()
+        check(dev.lib.mbx_alg_bytes(dev.ctx, C.byref(qs[i].prg), C.byref(qs[i].op),
+                                    qs[i].nr_grp, C.byref(qb), C.byref(qk)))
+        per_query += qb.value
     check(dev.lib.mbx_batch_destroy(dev.ctx, b))
     t.close()
     mp, mb, mh = float(np.median(tp)), float(np.median(tb)), float(np.median(tph))
@@ -271,6 +279,7 @@ def run_batched(dev, reps=5):
         "batched_vs_plain": round(mp / mb, 3),
         "alg_GB_per_s_batched": round(nbytes.value / mb / 1e6, 1),
         "roofline_frac_batched": round(nbytes.value / mb / 1e6 / HBM_PEAK_GBS, 4),
+        "per_query_alg_GB_per_s_batched": round(per_query / mb / 1e6, 1),
         "tune": {"tile_words": tune.tile_words, "wg_per_cu": tune.wg_per_cu,
                  "lds_groups": tune.lds_groups},
         "identical_to_plain": same,
@@ -353,7 +362,7 @@ def run_serving(dev, nreq=1000, rates=(2000.0, 20000.0), budget=256 << 20):
     }
 
 
-def run_learned(dev, nprog=2000, neval=200):
+def run_learned(dev, nprog=20000, neval=200):
     """Config 4: runtimes of random programs' candidate plans (evaluation
     orders x fuse / eager / materialise) measured by the engine, the
     16-64-1 fp32 MLP trained on them with the library's kernels, then
@@ -395,8 +404,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=1000)
     ap.add_argument("--serve-rates", default="2000,20000",
                     help="Poisson arrival rates of the serving replays (req/s)")
-    ap.add_argument("--learned-programs", type=int, default=2000,
-                    help="training programs of the plan chooser (20000 = full)")
+    ap.add_argument("--learned-programs", type=int, default=20000,
+                    help="training programs of the plan chooser")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="run only a plain device copy of this many bytes")

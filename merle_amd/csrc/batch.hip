@@ -72,12 +72,15 @@ __global__ __launch_bounds__(256) void batch_kernel(BatchArgs a) {
     __syncthreads();
     const uint32_t w_begin = r * q.words_per_item;
     const uint32_t w_end = min(w_begin + q.words_per_item, q.prg.nwords);
-    // eager evaluation for two-valued programs only (conjunctions without
-    // program interpretation, other small operand lists straight into
-    // registers); the rest take the early-out form (same bits, fewer
-    // registers for the whole kernel)
+    // eager evaluation for conjunctions (no program interpretation, two- or
+    // three-valued) and for two-valued programs with small operand lists
+    // (straight into registers); the rest take the early-out form (same
+    // bits, fewer registers for the whole kernel)
     if (q.chk) {
-      batch_item<true, 0>(q, w_begin, w_end, sink, s_list[wv]);
+      if (q.eager && q.prg.conj)
+        batch_item<true, 3>(q, w_begin, w_end, sink, s_list[wv]);
+      else
+        batch_item<true, 0>(q, w_begin, w_end, sink, s_list[wv]);
     } else if (q.eager && q.prg.conj) {
       batch_item<false, 3>(q, w_begin, w_end, sink, s_list[wv]);
     } else if (q.eager && q.prg.nfl <= kDirectOps) {

@@ -38,13 +38,13 @@ static int check_bmp(const uint32_t *p) {
 
 // vprg validation follows the reference's limits (primitive.cuh:121,164-198):
 // registers r0..r2, column operands 0..3, at most 6 instructions.
-// Symbolic run of a two-valued program: each register holds 0 (empty), a
+// Symbolic run of the program: each register holds 0 (empty), a
 // conjunction of column literals (bit c: column c, bit 4+c: NOT column c),
 // or "other".  True (and flneg set per column) when r0 ends as a non-empty
 // conjunction in which no column appears both plain and negated and every
-// column has bins -- then r0 = AND of the literals, word for word.
-static bool conj_literals(DevPrg &P, bool chk) {
-  if (chk) return false;
+// column has bins -- then r0 = AND of the literals, word for word, in the
+// two-valued form and, with edge bins, in the three-valued one.
+static bool conj_literals(DevPrg &P) {
   enum { kZero = 0, kConj = 1, kOther = 2 };
   int kind[3] = {kZero, kZero, kZero};
   uint32_t lit[3] = {0, 0, 0};
@@ -160,7 +160,7 @@ int make_dev_prg(const mbx_vprg *p, DevPrg *out) {
   out->nrows = p->factsz;
   out->nwords = (uint32_t)ceil_div(p->factsz, 64);
   out->chk = chk ? 1u : 0u;
-  out->conj = conj_literals(*out, chk) ? 1u : 0u;
+  out->conj = conj_literals(*out) ? 1u : 0u;
   return MBX_OK;
 }
 

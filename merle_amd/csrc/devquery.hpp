@@ -239,14 +239,20 @@ __device__ __forceinline__ void eval_words(const DevPrg &P, uint32_t w0,
       keep[k] = (1ull << (P.nrows & 63u)) - 1ull;
   }
   if (EAGER == 3) {
-    // EAGER 3: a two-valued conjunction of column literals (P.conj): r0 =
-    // AND over columns of the OR of their bins, complemented where flneg.
-    // The flat list is walked in groups of kConjGroup, every load of a group
-    // in flight at once; per word the work is one OR per bitmap and one AND
-    // per column -- no program interpretation.
+    // EAGER 3: a conjunction of column literals (P.conj): r0 = AND over
+    // columns of the OR of their bins, complemented where flneg.  The flat
+    // list is walked in groups of kConjGroup, every load of a group in
+    // flight at once; per word the work is one OR per bitmap and one AND per
+    // column -- no program interpretation.  Three-valued (CHK), with c = OR
+    // of a column's middle bins and u = OR of its edge bins, the program's
+    // instructions reduce to
+    //   p = AND(plain literals: c | u) & AND(negated literals: ~c)
+    //   u = OR(all literals: u) & p
+    // (ORM into an empty register gives p = c | u, u = u; ANDM and AND give
+    // p &= p_s, u = (u | u_s) & p; NOT gives p = ~p_s | u_s = ~c, u = u_s).
     uint32_t wa[WPL];
     bool any[WPL];
-    uint64_t acc[WPL], cur[WPL];
+    uint64_t acc[WPL], cur[WPL], accu[WPL], curu[WPL];
 #pragma unroll
     for (int k = 0; k < WPL; ++k) {
       const uint32_t w = w0 + ws * k;
@@ -256,6 +262,7 @@ __device__ __forceinline__ void eval_words(const DevPrg &P, uint32_t w0,
       wa[k] = in[k] ? w : wf;
       acc[k] = keep[k];
       cur[k] = 0;
+      accu[k] = curu[k] = 0;
     }
     for (uint32_t g = 0; g < P.nfl; g += kConjGroup) {
       uint64_t v[kConjGroup][WPL];
@@ -276,14 +283,25 @@ __device__ __forceinline__ void eval_words(const DevPrg &P, uint32_t w0,
 #pragma unroll
       for (int j = 0; j < kConjGroup; ++j) {
         if (g + j >= P.nfl) break;
+        if (CHK && P.fledge[g + j]) {
 #pragma unroll
-        for (int k = 0; k < WPL; ++k) cur[k] |= v[j][k];
+          for (int k = 0; k < WPL; ++k) curu[k] |= v[j][k];
+        } else {
+#pragma unroll
+          for (int k = 0; k < WPL; ++k) cur[k] |= v[j][k];
+        }
         if (P.flend[g + j]) {
+          // plain: c | u; negated: ~c = (c | (u & 0)) ^ ~0
           const uint64_t flip = P.flneg[g + j] ? ~0ull : 0ull;
 #pragma unroll
           for (int k = 0; k < WPL; ++k) {
-            acc[k] &= cur[k] ^ flip;
+            const uint64_t lit = CHK ? (cur[k] | (curu[k] & ~flip)) : cur[k];
+            acc[k] &= lit ^ flip;
             cur[k] = 0;
+            if (CHK) {
+              accu[k] |= curu[k];
+              curu[k] = 0;
+            }
           }
         }
       }
@@ -291,7 +309,7 @@ __device__ __forceinline__ void eval_words(const DevPrg &P, uint32_t w0,
 #pragma unroll
     for (int k = 0; k < WPL; ++k) {
       po[k] = acc[k];
-      uo[k] = 0;
+      uo[k] = CHK ? (accu[k] & acc[k]) : 0;
     }
     return;
   }

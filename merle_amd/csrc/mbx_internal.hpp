@@ -119,8 +119,8 @@ struct DevPrg {
   uint8_t flend[kMaxOperands];   // entry is the last of its column
   uint8_t flneg[kMaxOperands];   // (last entries) the column enters negated
   uint32_t nfl;
-  // the program computes, two-valued, a conjunction of column literals:
-  // r0 = AND over columns c of (OR of c's bins), complemented where flneg
+  // the program computes a conjunction of column literals: r0 = AND over
+  // columns c of (OR of c's bins), complemented where flneg
   // (e.g. ORM r0<-a; ANDM r0<-b; ORM r1<-c; NOT r1; AND r0,r1 = a & b & ~c)
   uint32_t conj;
 };

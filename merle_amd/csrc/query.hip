@@ -328,16 +328,18 @@ static void fused_even(mbx_ctx *ctx, const QArgs &a, size_t lds) {
 // pass but leave a third fewer waves for the gathers: 4-9 % slower); with
 // the table in HBM, 2 words in even passes, which spreads the HBM atomics
 // over the whole kernel (9 % faster than 4 words on 437,500 groups).
-template <bool L>
+// Three-valued (candidate-check) conjunctions double the evaluation state:
+// 2 words in even passes throughout (4 words leave 4 waves per SIMD).
+template <bool C, bool L>
 static void fused_conj(mbx_ctx *ctx, const QArgs &a, size_t lds) {
-  if (fused_try<false, L, false, 3, 2>(ctx, a, lds, false)) return;
-  if constexpr (L) (void)fused_try<false, L, false, 3, 4>(ctx, a, lds, true);
-  else fused_even<false, L, false, 3, 2>(ctx, a, lds);
+  if (fused_try<C, L, false, 3, 2>(ctx, a, lds, false)) return;
+  if constexpr (L && !C) (void)fused_try<C, L, false, 3, 4>(ctx, a, lds, true);
+  else fused_even<C, L, false, 3, 2>(ctx, a, lds);
 }
 
 template <bool C, bool L>
 static void fused_modes(mbx_ctx *ctx, const QArgs &a, size_t lds, int mode) {
-  if (!C && mode == 3) fused_conj<L>(ctx, a, lds);
+  if (mode == 3) fused_conj<C, L>(ctx, a, lds);
   else if (mode != 0) fused_launch<C, L, false, 1>(ctx, a, lds);
   else fused_launch<C, L, false, 0>(ctx, a, lds);
 }
@@ -353,8 +355,8 @@ int launch_fused(mbx_ctx *ctx, const QueryPlan &q, const Acct *acct) {
   a.sp = spread_of(ctx, q);
   const size_t lds = acct ? 0 : grp_lds(q);
   const bool chk = q.prg.chk != 0;
-  // evaluation: 0 early-out; eager: 3 for two-valued conjunctions (ANDs of
-  // column ORs, no program interpretation), else 1 (per-column
+  // evaluation: 0 early-out; eager: 3 for conjunctions of column literals
+  // (ANDs of column ORs, no program interpretation), else 1 (per-column
   // accumulators, then the program word by word)
   const int mode = !q.eager ? 0 : (q.prg.conj ? 3 : 1);
   // accounting runs the early-out form: the compulsory-line model
