@@ -77,8 +77,9 @@ def test_bins_fk_projection(dev, n):
 @pytest.mark.parametrize("seed", range(16))
 def test_random_queries_all_plans(dev, seed):
     """Random programs (AND/OR/NOT/ANDM/ORM, exact and edge bins), group
-    counts from 1 to >4096 (LDS table and HBM atomics): fused ==
-    materialise == scan == oracle, bit-exact int64."""
+    counts from 1 to >4096 (LDS table and HBM atomics): fused (early-out
+    and eager, which takes the conjunction form for two- and three-valued
+    conjunctions) == materialise == scan == oracle, bit-exact int64."""
     rng = np.random.default_rng(1000 + seed)
     n = int(rng.choice([1, 63, 64, 65, 5000, 70001, 300007]))
     schema = gpuq.upload_schema(dev, datagen.small_schema(n, seed))
@@ -86,7 +87,8 @@ def test_random_queries_all_plans(dev, seed):
         spec = datagen.random_spec(rng, schema, extra_groups=int(rng.integers(0, 3)))
         want, _ = cpuref.query(spec, schema)
         cols, allocs = gpuq.plan_cols(dev, schema, spec, rng)
-        for plan in (abi.PLAN_FUSED, abi.PLAN_MATERIALISE, abi.PLAN_SCAN):
+        for plan in (abi.PLAN_FUSED, abi.PLAN_FUSED | abi.PLAN_EAGER,
+                     abi.PLAN_MATERIALISE, abi.PLAN_SCAN):
             got = gpuq.run_query(dev, schema, spec, cols, plan)
             np.testing.assert_array_equal(got, want, err_msg=f"plan {plan} {spec}")
         gpuq.free_all(dev, allocs)
@@ -104,7 +106,8 @@ def test_dense_and_empty(dev):
                            [S.Key(S.Src("b32"))], 50, abi.MEAS_COL, S.Src("m32"))
         want, _ = cpuref.query(spec, schema)
         cols, allocs = gpuq.plan_cols(dev, schema, spec, rng, binned_frac=0)
-        for plan in (abi.PLAN_FUSED, abi.PLAN_MATERIALISE, abi.PLAN_SCAN):
+        for plan in (abi.PLAN_FUSED, abi.PLAN_FUSED | abi.PLAN_EAGER,
+                     abi.PLAN_MATERIALISE, abi.PLAN_SCAN):
             np.testing.assert_array_equal(gpuq.run_query(dev, schema, spec, cols, plan), want)
         gpuq.free_all(dev, allocs)
 
@@ -170,7 +173,8 @@ def test_config1_full(dev, config1):
             cols.append(S.BitmapCol([ptrs[i] for i in mid],
                                     None if l is None else ptrs[l],
                                     None if r is None else ptrs[r]))
-        for plan in (abi.PLAN_FUSED, abi.PLAN_MATERIALISE, abi.PLAN_SCAN):
+        for plan in (abi.PLAN_FUSED, abi.PLAN_FUSED | abi.PLAN_EAGER,
+                     abi.PLAN_MATERIALISE, abi.PLAN_SCAN):
             got = gpuq.run_query(dev, config1, spec, cols, plan)
             np.testing.assert_array_equal(got, want, err_msg=f"{spec.name} plan {plan}")
         # COUNT version == number of selected rows
