@@ -195,7 +195,7 @@ def plan_queries(dev, qs):
     return out
 
 
-def run_batched(dev, reps=5):
+def run_batched(dev, reps=5, train_programs=1024):
     """Config 3: 256 queries over the config-1 tables.  Plain = one fused
     launch per query, back to back on one stream (the reference's
     one-query-per-launch driver); batched = one persistent launch of all 256,
@@ -238,20 +238,50 @@ def run_batched(dev, reps=5):
         t.start()
         plain(qs)
         tp.append(t.stop())
+    written = qs
     qs = pq          # the batch runs the planned queries
     tune, tms = abi.Tune(), C.c_float()
     check(dev.lib.mbx_autotune(dev.ctx, qs, len(specs), C.byref(tune), C.byref(tms)))
-    b = C.c_void_p()
-    check(dev.lib.mbx_batch_create(dev.ctx, qs, len(specs), C.byref(tune), C.byref(b)))
-    check(dev.lib.mbx_batch_run(dev.ctx, b))
-    tb = []
-    for _ in range(reps):
-        t.start()
+
+    def batched(arr):
+        b = C.c_void_p()
+        check(dev.lib.mbx_batch_create(dev.ctx, arr, len(specs), C.byref(tune),
+                                       C.byref(b)))
         check(dev.lib.mbx_batch_run(dev.ctx, b))
-        tb.append(t.stop())
+        ts = []
+        for _ in range(reps):
+            t.start()
+            check(dev.lib.mbx_batch_run(dev.ctx, b))
+            ts.append(t.stop())
+        check(dev.lib.mbx_batch_destroy(dev.ctx, b))
+        return float(np.median(ts)), batch_out.host()
+
+    mb, got = batched(qs)
     ref = plain_out.host()
-    same = bool(np.array_equal(ref, batch_out.host()) and
-                np.array_equal(ref, planned_same))
+    same = bool(np.array_equal(ref, got) and np.array_equal(ref, planned_same))
+
+    # the same batch under learned plans: a cost model trained on runtimes
+    # the engine measures for other random queries over the same tables,
+    # choosing each query's evaluation order and eager flag among its fused
+    # candidates (the batched launch is fused-only)
+    from merle_amd import learned
+    t0 = time.time()
+    train = configs.config3_queries(train_programs, seed=configs.SEED + 33)
+    X, Y = learned.collect(dev, sch, bins, 0, specs=train, bounds=configs.C1_BOUNDS)
+    model = learned.CostModel(dev)
+    model.train(X, Y)
+    lq = (abi.Query * len(specs))()
+    C.memmove(lq, written, C.sizeof(lq))
+    for i, s in enumerate(specs):
+        plan = model.choose_fused(learned.Prepared(dev, sch, bins, s, configs.C1_BOUNDS))
+        applied = abi.Vprg()
+        check(dev.lib.mbx_plan_apply(C.byref(lq[i].prg), C.byref(plan), C.byref(applied)))
+        lq[i].prg = applied
+        lq[i].flags = plan.flags & abi.PLAN_EAGER
+    model.close()
+    learn_s = time.time() - t0
+    ml, got = batched(lq)
+    same = same and bool(np.array_equal(ref, got))
     nbytes, nsel = C.c_uint64(), C.c_uint64()
     check(dev.lib.mbx_alg_bytes_batch(dev.ctx, qs, len(specs), C.byref(nbytes),
                                       C.byref(nsel)))
@@ -263,9 +293,8 @@ def run_batched(dev, reps=5):
         check(dev.lib.mbx_alg_bytes(dev.ctx, C.byref(qs[i].prg), C.byref(qs[i].op),
                                     qs[i].nr_grp, C.byref(qb), C.byref(qk)))
         per_query += qb.value
-    check(dev.lib.mbx_batch_destroy(dev.ctx, b))
     t.close()
-    mp, mb, mh = float(np.median(tp)), float(np.median(tb)), float(np.median(tph))
+    mp, mh = float(np.median(tp)), float(np.median(tph))
     for a in [plain_out, batch_out] + slabs + [c.dev for c in sch.cols.values()]:
         a.free()
     rows = n * len(specs)
@@ -277,6 +306,10 @@ def run_batched(dev, reps=5):
         "plain_G_rows_per_s": round(rows / mp / 1e6, 2),
         "batched_G_rows_per_s": round(rows / mb / 1e6, 2),
         "batched_vs_plain": round(mp / mb, 3),
+        "batched_learned_ms": round(ml, 3),
+        "batched_learned_vs_plain": round(mp / ml, 3),
+        "learned_training": f"{len(train)} other random queries over the same "
+                            f"tables, {X.shape[0]} timed plans, {learn_s:.1f} s",
         "alg_GB_per_s_batched": round(nbytes.value / mb / 1e6, 1),
         "roofline_frac_batched": round(nbytes.value / mb / 1e6 / HBM_PEAK_GBS, 4),
         "per_query_alg_GB_per_s_batched": round(per_query / mb / 1e6, 1),
@@ -285,7 +318,8 @@ def run_batched(dev, reps=5):
         "identical_to_plain": same,
         "note": "plain = as-written programs, one fused launch each; batched = "
                 "the same queries with the static heuristic plan in one "
-                "persistent hipGraph launch",
+                "persistent hipGraph launch; batched_learned = the same launch "
+                "with the learned chooser's fused plans",
     }
 
 

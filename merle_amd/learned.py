@@ -106,9 +106,9 @@ def bitmap_cols(spec, bins, bounds=C4_BOUNDS):
 class Prepared:
     """A program bound to the device: vprg, row op, densities, candidates."""
 
-    def __init__(self, dev, sch, bins, spec):
+    def __init__(self, dev, sch, bins, spec, bounds=C4_BOUNDS):
         self.spec = spec
-        self.prg = to_vprg(bitmap_cols(spec, bins), sch.nrows, spec.instrs)
+        self.prg = to_vprg(bitmap_cols(spec, bins, bounds), sch.nrows, spec.instrs)
         self.op = to_rowop(spec, sch)
         self.dens = (C.c_double * 4)()
         check(dev.lib.mbx_col_density(dev.ctx, C.byref(self.prg), self.dens))
@@ -137,15 +137,20 @@ class Prepared:
         return timer.stop() if timer else None
 
 
-def collect(dev, sch, bins, nprog, plans_per_prog=3, seed=7, log=None):
-    """Times plans_per_prog random candidate plans of nprog programs:
+def collect(dev, sch, bins, nprog, plans_per_prog=3, seed=7, log=None,
+            specs=None, bounds=C4_BOUNDS):
+    """Times plans_per_prog random candidate plans of nprog programs (random
+    config-4 programs, or the given specs over bins of `bounds`):
     -> (features [n,16] float32, log runtime [n] float32)."""
     rng = np.random.default_rng(seed)
+    if specs is not None:
+        nprog = len(specs)
     out = dev.alloc(8 * 10_000, np.int64, 10_000)
     t = Timer(dev)
     X, Y = [], []
     for i in range(nprog):
-        p = Prepared(dev, sch, bins, random_program(rng))
+        spec = specs[i] if specs is not None else random_program(rng)
+        p = Prepared(dev, sch, bins, spec, bounds)
         picks = rng.choice(len(p.cands), size=min(plans_per_prog, len(p.cands)),
                            replace=False)
         for k in picks:
@@ -189,6 +194,22 @@ class CostModel:
                                            p.dens, p.spec.nr_grp, C.byref(best),
                                            C.byref(pred)))
         return best
+
+    def choose_fused(self, p):
+        """The cheapest fused candidate by the model (the batched launch
+        runs fused plans only)."""
+        cands = [c for c in p.cands if (c.flags & 3) == abi.PLAN_FUSED]
+        if not cands:
+            return p.heuristic
+        X = np.stack([p.features(self.dev, c) for c in cands]).astype(np.float32)
+        dx = self.dev.upload(X)
+        dy = self.dev.alloc(4 * len(cands), np.float32, len(cands))
+        check(self.dev.lib.mbx_mlp_forward(self.dev.ctx, self.m, dx.ptr, len(cands),
+                                           dy.ptr))
+        y = dy.host()
+        dx.free()
+        dy.free()
+        return cands[int(np.argmin(y))]
 
     def close(self):
         self.dev.lib.mbx_mlp_destroy(self.dev.ctx, self.m)
