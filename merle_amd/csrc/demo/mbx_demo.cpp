@@ -2,7 +2,7 @@
 // (ssb_demoall, main.c:6-12; ssb/fixedlayout.cu, ssb/lambdas.cuh:154-243)
 // written against the mybmpidx mirror.
 //
-//   mbx_demo <ssb_dir> [reps]
+//   mbx_demo <ssb_dir> [reps] [synth_rows]
 //
 // <ssb_dir> holds the binary column files of ssb-dbgen-lite -b (4-byte
 // header = element size in bytes, ssb-dbgen-lite.c:192-200).  The driver
@@ -14,13 +14,16 @@
 // col::from_bin2; edge bins -> candidate checks), timed per plan as the mean
 // of `reps` runs.  It prints the reference's five-block report
 // (drawFigs/drawFigs.py:15-46): index creation and the SSB block in full,
-// the WAH / RTScan, method and synthetic blocks with headers and n/a values
-// only.  SSB columns: Join = the bitmap-free scan (Sparse rows only, as
-// DOWORK), Ours = fused, PrgStg1 / Stg2 / Stg3 = the three materialise
-// stages; the plans this library does not build (Basefus, NfuStg1, Dedicat)
-// read 0.  Every plan's int64 sums must equal the scan's, or the driver
-// exits with status 1.
+// the synthetic block in full when synth_rows > 0 (seeded columns of that
+// many rows, see synth_suite), the WAH / RTScan and method blocks with
+// headers and n/a values only.  SSB columns: Join = the bitmap-free scan
+// (Sparse rows only, as DOWORK), Ours = fused under the library's static
+// plan, PrgStg1 / Stg2 / Stg3 = the three materialise stages; the plans
+// this library does not build (Basefus, NfuStg1, Dedicat) read 0.  Every
+// plan's int64 sums must equal the scan's, or the driver exits with
+// status 1.
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -293,6 +296,181 @@ Fixed fixed_layout(const Flight &f) {
   return x;
 }
 
+
+// The library's static plan of a program (column densities -> sparsest
+// column first; eager loads unless the early-out pays), made once outside
+// the timed loop.  "Ours" runs the fused query under it.
+bool static_plan(mbx_ctx *ctx, const mbx_vprg &ap, mbx_plan &plan) {
+  double dens[MBX_MAXCOLS];
+  if (mbx_col_density(ctx, &ap, dens) != MBX_OK ||
+      mbx_plan_heuristic(&ap, dens, &plan) != MBX_OK) {
+    std::fprintf(stderr, "plan: %s\n", mbx_last_error());
+    return false;
+  }
+  return true;
+}
+
+// Block 4 of the report: the synthetic suite of synth_demoall
+// (synth/cpuref.c:184-257; one row per case and selectivity as synth_bmp,
+// synth/synthdemo.cu:79-145, times it).  The query (synth/synthdemo.h:11-16):
+// COUNT(*) GROUP BY dim.attr2 over the rows with (a1 in [lo, hi) OR a2 in
+// [lo, hi)) AND dim.attr1[fk] in [lo, hi), under the perfect, many-OR and
+// candidate-check recipes, fused and as the three materialise stages.
+// Data per case: a1, a2 = Zipf(s) on [0, 65535) stored at the case's bit
+// width, fk = Zipf(s) on [0, N/100), dim.attr1[i] = i, dim.attr2 = uniform
+// bytes.  The reference reads files of unseeded numpy zipf mod 65535
+// (synth/onegen_zipf.py:5-10, batchgen.sh:12-25); these columns are seeded
+// and generated here.  [lo, lo + 60) is the first window whose probability
+// under the generator's pmf is at most the column selectivity of
+// synth/cpuref.c:191-195.  Returns the number of plans whose sums differ
+// from the scan's, or -1 on an error.
+template <class Timed>
+int synth_suite(mbx_ctx *ctx, uint32_t n, Timed &&timed) {
+  static const double kSkew[5] = {1.04, 1.08, 1.12, 1.16, 1.20};
+  static const double kColSel[11] = {.02, .04, .06, .09, .12, .18,
+                                     .25, .39, .51, .67, .82};
+  static const double kFulSel[11] = {1 / 512.0, 1 / 256.0, 1 / 128.0,
+                                     1 / 64.0,  1 / 32.0,  1 / 16.0,
+                                     1 / 8.0,   1 / 4.0,   .4, .6, .8};
+  constexpr uint32_t kVals = 65535, kWin = 60, kGroups = 256;
+  const uint32_t dimsz = n / 100;
+  if (dimsz == 0) {
+    std::fprintf(stderr, "synthetic suite: need at least 100 rows\n");
+    return -1;
+  }
+  const size_t nw = mbx_bitmap_stride_words(n);
+  void *poss = nullptr, *unc = nullptr, *rows = nullptr, *count = nullptr,
+       *out = nullptr, *want = nullptr, *dim1 = nullptr, *dim2 = nullptr;
+  if (mbx_alloc(ctx, nw * 4, &poss) || mbx_alloc(ctx, nw * 4, &unc) ||
+      mbx_alloc(ctx, (size_t)n * 4, &rows) || mbx_alloc(ctx, 256, &count) ||
+      mbx_alloc(ctx, 8 * kGroups, &out) || mbx_alloc(ctx, 8 * kGroups, &want) ||
+      mbx_alloc(ctx, (size_t)dimsz * 4, &dim1) || mbx_alloc(ctx, dimsz, &dim2)) {
+    std::fprintf(stderr, "alloc: %s\n", mbx_last_error());
+    return -1;
+  }
+  {
+    std::vector<uint32_t> iota(dimsz), u(dimsz);
+    for (uint32_t i = 0; i < dimsz; ++i) iota[i] = i;
+    mbx_gen_uniform_u32_host(u.data(), dimsz, 13458786, 0, 256);
+    std::vector<uint8_t> b(u.begin(), u.end());
+    mbx_memcpy_h2d(ctx, dim1, iota.data(), (size_t)dimsz * 4);
+    mbx_memcpy_h2d(ctx, dim2, b.data(), dimsz);
+  }
+  int bad = 0;
+  std::vector<int64_t> h_want(kGroups), h_got(kGroups);
+  std::vector<uint32_t> col(n);
+  for (int c = 0; c < 10; ++c) {
+    const double s = kSkew[c / 2];
+    const uint32_t bw = c % 2 ? 32 : 16;
+    // the case's columns
+    void *d[3] = {nullptr, nullptr, nullptr};   // a1, a2, fk
+    for (int k = 0; k < 3; ++k) {
+      const uint64_t seed = 0x5eed0000u + 16u * (uint64_t)c + (uint64_t)k;
+      mbx_gen_zipf_u32_host(col.data(), n, seed, k < 2 ? kVals : dimsz, s);
+      const size_t eb = k < 2 ? bw / 8 : 4;
+      if (mbx_alloc(ctx, (size_t)n * eb, &d[k])) {
+        std::fprintf(stderr, "alloc: %s\n", mbx_last_error());
+        return -1;
+      }
+      if (eb == 2) {
+        std::vector<uint16_t> h(col.begin(), col.end());
+        mbx_memcpy_h2d(ctx, d[k], h.data(), (size_t)n * 2);
+      } else {
+        mbx_memcpy_h2d(ctx, d[k], col.data(), (size_t)n * 4);
+      }
+    }
+    // window probabilities of the attribute pmf, (v+1)^-s / H
+    std::vector<double> pre(kVals + 1, 0.0);
+    for (uint32_t v = 0; v < kVals; ++v)
+      pre[v + 1] = pre[v] + std::pow((double)v + 1.0, -s);
+    const uint32_t *fk = static_cast<const uint32_t *>(d[2]);
+    for (int o = 1; o < 11; ++o) {
+      uint32_t lo = 0;
+      while (lo + kWin < kVals &&
+             (pre[lo + kWin] - pre[lo]) / pre[kVals] > kColSel[o])
+        ++lo;
+      const uint32_t hi = lo + kWin;
+      recipe r;
+      r.factsz = n;
+      for (int k = 0; k < 3; ++k) {
+        r.nbit[k] = k < 2 ? bw : 32;
+        r.min[k] = lo;
+        r.max[k] = hi;
+        r.dimsz[k] = k < 2 ? recipe::nodim : dimsz;
+        r.fk[k] = k < 2 ? nullptr : fk;
+        r.attr[k] = k < 2 ? d[k] : dim1;
+      }
+      vprg::instr ins3[MAXNINSTR];
+      ins3[0] = ins(vprg::ORM, 0, 0);
+      ins3[1] = ins(vprg::ORM, 0, 1);
+      ins3[2] = ins(vprg::ANDM, 0, 2);
+      mbx_rowop op;
+      std::memset(&op, 0, sizeof(op));
+      op.npreds = 3;
+      for (int k = 0; k < 3; ++k) {
+        op.preds[k].src.data = k < 2 ? d[k] : dim1;
+        op.preds[k].src.fk = k < 2 ? nullptr : fk;
+        op.preds[k].src.dimsz = k < 2 ? 0 : dimsz;
+        op.preds[k].src.elem_bytes = k < 2 ? bw / 8 : 4;
+        op.preds[k].lo = lo;
+        op.preds[k].hi = hi;
+      }
+      op.nkeys = 1;
+      key(op.keys[0], mbx_colref{dim2, fk, dimsz, 1}, MBX_KEY_IDENT, 1, 0, 1);
+      op.measure = MBX_MEAS_COUNT;
+      float t[4][4] = {};   // [plan][fused, stage 1, 2, 3]; join in t[3][0]
+      bool first = true;
+      for (int kind = 0; kind < 3; ++kind) {
+        vprg p = kind == 0 ? r.perfect(ins3)
+                           : kind == 1 ? r.many_or(ins3) : r.candchk(ins3);
+        const mbx_vprg ap = p.to_abi();
+        if (first) {
+          t[3][0] = timed([&] {
+            return mbx_prog_grpby(ctx, &ap, &op, kGroups,
+                                  static_cast<int64_t *>(want), MBX_PLAN_SCAN);
+          });
+          mbx_memcpy_d2h(ctx, h_want.data(), want, 8 * kGroups);
+          first = false;
+        }
+        mbx_plan plan;
+        if (!static_plan(ctx, ap, plan)) return -1;
+        t[kind][0] = timed([&] {
+          return mbx_prog_grpby_plan(ctx, &ap, &op, kGroups,
+                                     static_cast<int64_t *>(out), &plan);
+        });
+        mbx_memcpy_d2h(ctx, h_got.data(), out, 8 * kGroups);
+        bad += std::memcmp(h_want.data(), h_got.data(), 8 * kGroups) != 0;
+        uint32_t *pu = p.is_nochk() ? nullptr : static_cast<uint32_t *>(unc);
+        t[kind][1] = timed([&] {
+          return mbx_prog_eval(ctx, &ap, static_cast<uint32_t *>(poss), pu);
+        });
+        t[kind][2] = timed([&] {
+          return mbx_compact(ctx, static_cast<uint32_t *>(poss), pu, n,
+                             static_cast<uint32_t *>(rows),
+                             static_cast<uint32_t *>(count));
+        });
+        t[kind][3] = timed([&] {
+          return mbx_rows_grpby(ctx, static_cast<uint32_t *>(rows),
+                                static_cast<uint32_t *>(count), &ap, &op,
+                                kGroups, static_cast<int64_t *>(out), 0);
+        });
+        mbx_memcpy_d2h(ctx, h_got.data(), out, 8 * kGroups);
+        bad += std::memcmp(h_want.data(), h_got.data(), 8 * kGroups) != 0;
+        p.release();
+      }
+      std::printf("%.2f\t%u\t%.5f\t%.4f\t%.4f\t%.4f\t%.4f", s, bw, kFulSel[o],
+                  t[3][0], t[0][0], t[1][0], t[2][0]);
+      for (int kind = 0; kind < 3; ++kind)
+        std::printf("\t%.4f\t%.4f\t%.4f", t[kind][1], t[kind][2], t[kind][3]);
+      std::printf("\t0.0\n");
+      std::fflush(stdout);
+    }
+    for (void *q : d) mbx_free(ctx, q);
+  }
+  for (void *q : {poss, unc, rows, count, out, want, dim1, dim2}) mbx_free(ctx, q);
+  return bad;
+}
+
 double now_ms() {
   return std::chrono::duration<double, std::milli>(
              std::chrono::steady_clock::now().time_since_epoch())
@@ -303,11 +481,13 @@ double now_ms() {
 
 int main(int argc, char **argv) {
   if (argc < 2) {
-    std::fprintf(stderr, "usage: %s <ssb_dir> [reps]\n", argv[0]);
+    std::fprintf(stderr, "usage: %s <ssb_dir> [reps] [synth_rows]\n", argv[0]);
     return 2;
   }
   const std::string dir = argv[1];
   const int reps = argc > 2 ? std::atoi(argv[2]) : 20;
+  const uint32_t synth_rows =
+      argc > 3 ? (uint32_t)std::strtoul(argv[3], nullptr, 10) : 0;
   mbx_ctx *ctx = context();
   if (!ctx) {
     std::fprintf(stderr, "no device: %s\n", mbx_last_error());
@@ -429,8 +609,10 @@ int main(int argc, char **argv) {
         join = timed([&] {
           return mbx_prog_grpby(ctx, &ap, &op, g, want, MBX_PLAN_SCAN);
         });
+      mbx_plan plan;
+      if (!static_plan(ctx, ap, plan)) return 2;
       const float ours = timed([&] {
-        return mbx_prog_grpby(ctx, &ap, &op, g, out, MBX_PLAN_FUSED);
+        return mbx_prog_grpby_plan(ctx, &ap, &op, g, out, &plan);
       });
       mbx_memcpy_d2h(ctx, h_want.data(), want, 8 * g);
       mbx_memcpy_d2h(ctx, h_got.data(), out, 8 * g);
@@ -454,11 +636,18 @@ int main(int argc, char **argv) {
       std::printf("\t%.4f\t0\t0\t%.4f\t%.4f\t%.4f\t0", ours, s1, s2, s3);
     }
   }
-  // blocks 3 (method 1 / method 2) and 4 (synthetic) of the report: this
-  // driver runs the SSB part only, so they carry their headers alone
+  // block 3 (method 1 / method 2, ablation kernels this library does not
+  // build) carries its header alone; block 4 is the synthetic suite when
+  // synth_rows > 0
   std::printf("\n\nSeletiv\tMethod1\tMethod2\n\nSkew\tBitW\tSeletiv\tJoin\t"
               "Perfect\tManyOrs\tCandChk\tPftStg1\tPftStg2\tPftStg3\tOrsStg1\t"
               "OrsStg2\tOrsStg3\tChkStg1\tChkStg2\tChkStg3\tWAH\n");
+  std::fflush(stdout);
+  if (synth_rows) {
+    const int sb = synth_suite(ctx, synth_rows, timed);
+    if (sb < 0) return 2;
+    bad += sb;
+  }
   for (int d = 0; d < 2; ++d)
     for (int s = 0; s < NSRC; ++s)
       if (!bins[d][s].empty()) free_bitmap(bins[d][s][0]);

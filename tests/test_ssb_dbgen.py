@@ -197,16 +197,27 @@ def test_dbgen_flights_on_gpu(dev, ssb_host):
 def test_demo_driver_report(ssb_host, tmp_path_factory):
     """The C++ driver (merle_amd/lib/mbx_demo, the reference's ssb_demoall
     on the mybmpidx mirror) runs every flight's Sparse / Medium / Dense
-    programs, checks each plan against the scan plan (exit status 0) and
-    prints the five-block report with three SSB rows per flight."""
+    programs and the synthetic suite (perfect / many-OR / candidate-check
+    recipes on seeded Zipf columns), checks each plan against the scan plan
+    (exit status 0) and prints the five-block report with three SSB rows per
+    flight and one synthetic row per case and selectivity."""
     demo = os.path.join(ROOT, "merle_amd", "lib", "mbx_demo")
     if not os.path.exists(demo):
         pytest.skip("mbx_demo not built")
     d = _dbgen_dir(tmp_path_factory)
-    r = subprocess.run([demo, d, "3"], capture_output=True, text=True, timeout=300)
+    r = subprocess.run([demo, d, "3", "1000000"], capture_output=True, text=True,
+                       timeout=300)
     assert r.returncode == 0, r.stderr
     blocks = r.stdout.strip("\n").split("\n\n")
     assert len(blocks) == 5
+    # the synthetic block (synth/cpuref.c:207-256): 10 cases x 10 selectivities
+    syn = blocks[4].splitlines()
+    assert syn[0].split("\t")[:4] == ["Skew", "BitW", "Seletiv", "Join"]
+    assert len(syn) == 1 + 100
+    for row in syn[1:]:
+        f = row.split("\t")
+        assert len(f) == 17 and f[1] in ("16", "32")
+        assert all(float(x) > 0 for x in f[3:16])
     ssb_rows = blocks[2].splitlines()
     assert ssb_rows[0].split("\t")[:4] == ["Case", "BinType", "Join", "Ours"]
     assert len(ssb_rows) == 1 + 3 * len(ssbq.FLIGHTS)
