@@ -183,3 +183,42 @@ def test_config1_full(dev, config1):
         assert 0.005 < nsel / config1.nrows < 0.02
     for c in configs.C1_COLS:
         check(dev.lib.mbx_free(dev.ctx, bins[c][0]))
+
+
+def test_multipass_launch_shapes(dev):
+    """70M rows, past one resident pass: the eager conjunction form on 4
+    words per lane over the resident grid (group table in LDS) and on 2
+    words in even passes (table in HBM), two- and three-valued, next to the
+    early-out form; every result equals the oracle."""
+    n = 70_000_000
+    sch = configs.config1_device(dev, n, seed=5)
+    for c in sch.cols.values():
+        c.host = c.dev.host()
+    b = list(configs.C1_BOUNDS)
+    bins = {c: gpuq.create_bin(dev, sch, S.Src(c), b[:-1], b[1:])
+            for c in configs.C1_COLS}
+    conj_shapes = {((abi.ORM, 0), (abi.ANDM, 0)),
+                   ((abi.ORM, 0), (abi.ORM, 1), (abi.NOT, 1), (abi.AND, 0))}
+    picked = {}
+    for q in configs.config3_queries(160, seed=31):
+        shape = tuple((op, d) for op, d, _ in q.instrs[:4])
+        if not any(shape[:len(s)] == tuple(s) for s in conj_shapes):
+            continue
+        cols = configs.bin_cols(q, bins)
+        chk = any(c.leftmost is not None or c.rightmost is not None for c in cols)
+        kind = (chk, q.nr_grp > 4096)
+        if len(picked.setdefault(kind, [])) < 2:
+            picked[kind].append((q, cols))
+    assert {(False, False), (True, False)} <= set(picked)
+    for kind, qs in sorted(picked.items()):
+        for q, cols in qs:
+            want, _ = cpuref.query(q, sch)
+            for plan in (abi.PLAN_FUSED | abi.PLAN_EAGER, abi.PLAN_FUSED):
+                got = gpuq.run_query(dev, sch, q, cols, plan)
+                np.testing.assert_array_equal(got, want,
+                                              err_msg=f"{kind} {q.name} plan {plan}")
+    for c in configs.C1_COLS:
+        check(dev.lib.mbx_free(dev.ctx, bins[c][0]))
+    for c in sch.cols.values():
+        c.dev.free()
+        c.host = None
