@@ -38,57 +38,100 @@ static int check_bmp(const uint32_t *p) {
 
 // vprg validation follows the reference's limits (primitive.cuh:121,164-198):
 // registers r0..r2, column operands 0..3, at most 6 instructions.
-// Symbolic run of the program: each register holds 0 (empty), a
-// conjunction of column literals (bit c: column c, bit 4+c: NOT column c),
-// or "other".  True (and flneg set per column) when r0 ends as a non-empty
-// conjunction in which no column appears both plain and negated and every
-// column has bins -- then r0 = AND of the literals, word for word, in the
-// two-valued form and, with edge bins, in the three-valued one.
-static bool conj_literals(DevPrg &P) {
+// Symbolic run of the program.  Each register holds 0 (empty), a
+// conjunction of clauses, or "other".  A clause is an OR of plain columns
+// (a column mask) or NOT of one column.  ORM into an empty register opens a
+// clause and ORM / OR into a register that is one plain clause extends it;
+// ANDM and AND add clauses; NOT turns a one-column clause into its negation.
+// The program is a CNF of column literals when r0 ends as a non-empty
+// conjunction whose clauses use every memory column exactly once (each with
+// bins).  Then r0 = AND over clauses of (OR of the clause's bins),
+// complemented for negated clauses, word for word, in the two-valued form
+// and, with edge bins, in the three-valued one.
+namespace {
+struct Cnf {
   enum { kZero = 0, kConj = 1, kOther = 2 };
-  int kind[3] = {kZero, kZero, kZero};
-  uint32_t lit[3] = {0, 0, 0};
+  int kind = kZero;
+  uint32_t n = 0;
+  uint8_t mask[8] = {};
+  bool neg[8] = {};
+  bool add(uint8_t m, bool ng) {
+    for (uint32_t i = 0; i < n; ++i)
+      if (mask[i] == m && neg[i] == ng) return true;   // x & x = x
+    if (n == 8) return false;
+    mask[n] = m;
+    neg[n] = ng;
+    ++n;
+    return true;
+  }
+  bool one_plain() const { return kind == kConj && n == 1 && !neg[0]; }
+};
+}  // namespace
+
+static bool cnf_program(const DevPrg &P, Cnf &r0) {
+  Cnf r[3];
   for (uint32_t i = 0; i < P.nins; ++i) {
     const uint32_t op = P.ins[i][0], d = P.ins[i][1], s = P.ins[i][2];
+    Cnf &rd = r[d];
     switch (op) {
     case MBX_ORM:
-      if (kind[d] == kZero) { kind[d] = kConj; lit[d] = 1u << s; }
-      else { kind[d] = kOther; }
-      break;
-    case MBX_ANDM:
-      if (kind[d] == kConj) lit[d] |= 1u << s;
-      else if (kind[d] != kZero) kind[d] = kOther;   // 0 & x stays 0
-      break;
-    case MBX_NOT:
-      if (kind[s] == kConj && (lit[s] & (lit[s] - 1)) == 0) {
-        kind[d] = kConj;   // NOT of one literal
-        lit[d] = (lit[s] & 0xfu) ? lit[s] << 4 : lit[s] >> 4;
+      if (rd.kind == Cnf::kZero) {
+        rd = Cnf();
+        rd.kind = Cnf::kConj;
+        rd.add((uint8_t)(1u << s), false);
+      } else if (rd.one_plain()) {
+        rd.mask[0] |= (uint8_t)(1u << s);
       } else {
-        kind[d] = kOther;
+        rd.kind = Cnf::kOther;
       }
       break;
-    case MBX_AND:
-      if (kind[d] == kConj && kind[s] == kConj) lit[d] |= lit[s];
-      else if (kind[d] == kZero || kind[s] == kZero) { kind[d] = kZero; lit[d] = 0; }
-      else kind[d] = kOther;
+    case MBX_ANDM:   // 0 & x stays 0
+      if (rd.kind == Cnf::kConj && !rd.add((uint8_t)(1u << s), false))
+        rd.kind = Cnf::kOther;
       break;
-    default:   // OR: a disjunction
-      kind[d] = kOther;
+    case MBX_OR: {
+      const Cnf rs = r[s];
+      if (rs.kind == Cnf::kZero) break;
+      if (rd.kind == Cnf::kZero) rd = rs;
+      else if (rd.one_plain() && rs.one_plain()) rd.mask[0] |= rs.mask[0];
+      else rd.kind = Cnf::kOther;
       break;
     }
+    case MBX_AND: {
+      const Cnf rs = r[s];
+      if (rd.kind == Cnf::kZero || rs.kind == Cnf::kZero) {
+        rd = Cnf();
+      } else if (rd.kind == Cnf::kConj && rs.kind == Cnf::kConj) {
+        for (uint32_t k = 0; k < rs.n && rd.kind == Cnf::kConj; ++k)
+          if (!rd.add(rs.mask[k], rs.neg[k])) rd.kind = Cnf::kOther;
+      } else {
+        rd.kind = Cnf::kOther;
+      }
+      break;
+    }
+    default: {   // NOT of one literal
+      const Cnf rs = r[s];
+      if (rs.kind == Cnf::kConj && rs.n == 1 &&
+          (rs.mask[0] & (rs.mask[0] - 1)) == 0) {
+        rd = rs;
+        rd.neg[0] = !rs.neg[0];
+      } else {
+        rd.kind = Cnf::kOther;
+      }
+      break;
+    }
+    }
   }
-  if (kind[0] != kConj) return false;
-  const uint32_t pos = lit[0] & 0xfu, neg = lit[0] >> 4;
-  if (pos & neg) return false;
-  // the flat list holds exactly the columns of memory instructions; every
-  // one of them must be a literal of r0 (none may be read only elsewhere)
-  if ((pos | neg) != P.memcols) return false;
-  for (uint32_t c = 0; c < MBX_MAXCOLS; ++c) {
-    if (!((P.memcols >> c) & 1u)) continue;
-    if (P.end[c] <= P.beg[c]) return false;   // empty column: generic path
-    P.flneg[P.end[c] - 1] = (neg >> c) & 1u;
+  r0 = r[0];
+  if (r0.kind != Cnf::kConj || r0.n == 0) return false;
+  // the flat list holds exactly the columns of memory instructions; each
+  // must sit in exactly one clause of r0 (none read only elsewhere)
+  uint32_t seen = 0;
+  for (uint32_t k = 0; k < r0.n; ++k) {
+    if (seen & r0.mask[k]) return false;
+    seen |= r0.mask[k];
   }
-  return true;
+  return seen == P.memcols;
 }
 
 int make_dev_prg(const mbx_vprg *p, DevPrg *out) {
@@ -140,27 +183,49 @@ int make_dev_prg(const mbx_vprg *p, DevPrg *out) {
     if (mem) out->memcols |= 1u << in.src1;
   }
   out->nins = n;
-  for (uint32_t c = 0; c < MBX_MAXCOLS; ++c) {
+  // The flat operand list: column blocks (middle bins, then edge bins),
+  // clause by clause when the program is a CNF of column literals, else in
+  // column order; flend marks the last entry of each clause (each column
+  // when not a CNF).
+  Cnf cnf;
+  bool conj = cnf_program(*out, cnf);
+  for (uint32_t c = 0; c < MBX_MAXCOLS; ++c)   // an empty column: generic path
+    if (((out->memcols >> c) & 1u) && out->cols[c].nm == 0 &&
+        !out->cols[c].has_edge)
+      conj = false;
+  auto add_col = [&](uint32_t c) {
+    const DevCol &d = out->cols[c];
     out->beg[c] = (uint8_t)out->nfl;
-    if ((out->memcols >> c) & 1u) {
-      const DevCol &d = out->cols[c];
-      auto add = [&](const uint64_t *b, bool edge) {
-        out->fl[out->nfl] = b;
-        out->flcol[out->nfl] = (uint8_t)c;
-        out->fledge[out->nfl] = edge ? 1 : 0;
-        ++out->nfl;
-      };
-      for (uint32_t k = 0; k < d.nm; ++k) add(d.m[k], false);
-      if (d.l) add(d.l, true);
-      if (d.r) add(d.r, true);   // an empty column reads as all-zero
-    }
+    auto add = [&](const uint64_t *b, bool edge) {
+      out->fl[out->nfl] = b;
+      out->flcol[out->nfl] = (uint8_t)c;
+      out->fledge[out->nfl] = edge ? 1 : 0;
+      ++out->nfl;
+    };
+    for (uint32_t k = 0; k < d.nm; ++k) add(d.m[k], false);
+    if (d.l) add(d.l, true);
+    if (d.r) add(d.r, true);   // an empty column reads as all-zero
     out->end[c] = (uint8_t)out->nfl;
-    if (out->end[c] > out->beg[c]) out->flend[out->end[c] - 1] = 1;
+  };
+  for (uint32_t c = 0; c < MBX_MAXCOLS; ++c) out->beg[c] = out->end[c] = 0;
+  if (conj) {
+    for (uint32_t k = 0; k < cnf.n; ++k) {
+      for (uint32_t c = 0; c < MBX_MAXCOLS; ++c)
+        if ((cnf.mask[k] >> c) & 1u) add_col(c);
+      out->flend[out->nfl - 1] = 1;
+      out->flneg[out->nfl - 1] = cnf.neg[k] ? 1 : 0;
+    }
+  } else {
+    for (uint32_t c = 0; c < MBX_MAXCOLS; ++c) {
+      if ((out->memcols >> c) & 1u) add_col(c);
+      else out->beg[c] = out->end[c] = (uint8_t)out->nfl;
+      if (out->end[c] > out->beg[c]) out->flend[out->end[c] - 1] = 1;
+    }
   }
   out->nrows = p->factsz;
   out->nwords = (uint32_t)ceil_div(p->factsz, 64);
   out->chk = chk ? 1u : 0u;
-  out->conj = conj_literals(*out) ? 1u : 0u;
+  out->conj = conj ? 1u : 0u;
   return MBX_OK;
 }
 

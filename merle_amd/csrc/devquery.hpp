@@ -239,17 +239,20 @@ __device__ __forceinline__ void eval_words(const DevPrg &P, uint32_t w0,
       keep[k] = (1ull << (P.nrows & 63u)) - 1ull;
   }
   if (EAGER == 3) {
-    // EAGER 3: a conjunction of column literals (P.conj): r0 = AND over
-    // columns of the OR of their bins, complemented where flneg.  The flat
-    // list is walked in groups of kConjGroup, every load of a group in
-    // flight at once; per word the work is one OR per bitmap and one AND per
-    // column -- no program interpretation.  Three-valued (CHK), with c = OR
-    // of a column's middle bins and u = OR of its edge bins, the program's
-    // instructions reduce to
-    //   p = AND(plain literals: c | u) & AND(negated literals: ~c)
-    //   u = OR(all literals: u) & p
-    // (ORM into an empty register gives p = c | u, u = u; ANDM and AND give
-    // p &= p_s, u = (u | u_s) & p; NOT gives p = ~p_s | u_s = ~c, u = u_s).
+    // EAGER 3: a conjunction of clauses (P.conj): r0 = AND over clauses of
+    // the OR of their bins (a clause is an OR of columns, or NOT of one
+    // column; the flat list holds it contiguously, flend on its last entry),
+    // complemented where flneg.  The flat list is walked in groups of
+    // kConjGroup, every load of a group in flight at once; per word the work
+    // is one OR per bitmap and one AND per clause -- no program
+    // interpretation.  Three-valued (CHK), with c = OR of a clause's middle
+    // bins and u = OR of its edge bins, the instructions reduce to
+    //   p = AND(plain clauses: c | u) & AND(negated clauses: ~c)
+    //   u = OR(clauses: u & ~c) & p
+    // (ORM into an empty register gives p = c | u, u = u; ORM / OR onto a
+    // clause give p |= p_s, u = (u | u_s) & ~(c | c_s) for disjoint bins;
+    // ANDM and AND give p &= p_s, u = (u | u_s) & p; NOT gives
+    // p = ~p_s | u_s = ~c, u = u_s).
     uint32_t wa[WPL];
     bool any[WPL];
     uint64_t acc[WPL], cur[WPL], accu[WPL], curu[WPL];
@@ -297,11 +300,11 @@ __device__ __forceinline__ void eval_words(const DevPrg &P, uint32_t w0,
           for (int k = 0; k < WPL; ++k) {
             const uint64_t lit = CHK ? (cur[k] | (curu[k] & ~flip)) : cur[k];
             acc[k] &= lit ^ flip;
-            cur[k] = 0;
             if (CHK) {
-              accu[k] |= curu[k];
+              accu[k] |= curu[k] & ~cur[k];
               curu[k] = 0;
             }
+            cur[k] = 0;
           }
         }
       }

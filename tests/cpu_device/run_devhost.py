@@ -1,0 +1,89 @@
+"""Random programs through the host build of the device query functions
+(devhost.cpp) against the oracle.  Run as a child process (the library may
+be built with AddressSanitizer, which must be preloaded):
+
+    python run_devhost.py <libdevhost.so> <seeds>
+"""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.dirname(os.path.dirname(HERE)), os.path.dirname(HERE)]
+
+import datagen  # noqa: E402
+from merle_amd import abi, spec as S  # noqa: E402
+from oracle import cpuref  # noqa: E402
+
+L = C.CDLL(sys.argv[1])
+L.dh_alloc.restype = C.c_void_p
+L.dh_alloc.argtypes = [C.c_size_t]
+for f in (L.dh_scan, L.dh_fused):
+    f.argtypes = [C.POINTER(abi.Vprg), C.POINTER(abi.RowOp), C.c_uint32, C.c_void_p]
+L.dh_conj_words.restype = C.c_long
+L.dh_chk_conj_words.restype = C.c_long
+
+
+class HostArray:
+    """A column in a heap block of exactly its size (bounds-checked)."""
+
+    def __init__(self, a):
+        a = np.ascontiguousarray(a)
+        self.n, self.dtype = a.size, a.dtype
+        self.ptr = L.dh_alloc(a.nbytes)
+        C.memmove(self.ptr, a.ctypes.data, a.nbytes)
+
+
+def bitmap(words, n):
+    w64 = (n + 63) // 64
+    w = np.zeros(w64 * 2, np.uint32)
+    w[:min(len(words), w64 * 2)] = words[:w64 * 2]
+    return HostArray(w).ptr
+
+
+for seed in range(int(sys.argv[2])):
+    rng = np.random.default_rng(1000 + seed)
+    n = int(rng.choice([1, 63, 64, 65, 5000, 70001, 300007]))
+    schema = datagen.small_schema(n, seed)
+    for c in schema.cols.values():
+        c.dev = HostArray(c.host)
+    for it in range(6):
+        spec = datagen.random_spec(rng, schema, extra_groups=int(rng.integers(0, 3)))
+        want, _ = cpuref.query(spec, schema)
+        cols = []
+        for p in spec.preds:
+            col = None
+            if rng.random() < 0.6:   # equal-width bins, edges -> candidate check
+                hi_v = int(max(p.hi, 2)) * 2
+                width = max(1, hi_v // 12)
+                bounds = list(range(0, width * 13, width))
+                fk = schema[p.src.fk].host if p.src.fk else None
+                bins = cpuref.create_bin(schema[p.src.col].host, bounds[:-1], bounds[1:],
+                                         fk, schema[p.src.col].n if fk is not None else 0)
+                fb = S.from_bin(bounds, p.lo, p.hi)
+                if fb is not None:
+                    mid, left, right = fb
+                    col = S.BitmapCol([bitmap(bins[i], n) for i in mid],
+                                      None if left is None else bitmap(bins[left], n),
+                                      None if right is None else bitmap(bins[right], n))
+            if col is None:
+                fk = schema[p.src.fk].host if p.src.fk else None
+                ex = cpuref.create_join(schema[p.src.col].host, p.lo, min(p.hi, 2**32 - 1),
+                                        fk, schema[p.src.col].n if fk is not None else 0)
+                col = S.BitmapCol([bitmap(ex, n)])
+            cols.append(col)
+        prg = S.to_vprg(cols, n, spec.instrs)
+        op = S.to_rowop(spec, schema)
+        for name, f in (("fused", L.dh_fused), ("scan", L.dh_scan)):
+            out = np.zeros(spec.nr_grp, np.int64)
+            st = f(C.byref(prg), C.byref(op), spec.nr_grp, out.ctypes.data)
+            if st != 0:
+                print("STATUS", st, seed, it, name, spec)
+                sys.exit(1)
+            if not np.array_equal(out, want):
+                print("MISMATCH", seed, it, name, spec)
+                sys.exit(1)
+    print("seed", seed, "n", n, "ok; conjunction-form words",
+          L.dh_conj_words(), "three-valued", L.dh_chk_conj_words(), flush=True)
