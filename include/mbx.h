@@ -211,7 +211,9 @@ int mbx_alg_bytes(mbx_ctx *ctx, const mbx_vprg *prg, const mbx_rowop *op,
                   uint32_t nr_grp, uint64_t *bytes, uint64_t *selected);
 
 /* ------------------------------------------------------------ batching */
-/* One query of a batch. */
+/* No reference counterpart: the reference launches one query per kernel
+ * (ssb/lambdas.cuh:179-184, synth/synthdemo.cu:122-129).  One query of a
+ * batch: */
 typedef struct mbx_query {
   mbx_vprg prg;
   mbx_rowop op;
@@ -250,7 +252,7 @@ int mbx_default_tune(mbx_tune *t);
 
 
 /* ------------------------------------------------------------- serving */
-/* Replays a request trace (arrival times in seconds, sorted) against
+/* No reference counterpart.  Replays a request trace (arrival times in seconds, sorted) against
  * resident data: memory-based admission (each request's int64 output is
  * allocated from the pool when it fits mem_budget), continuous batching of
  * admitted requests onto nstreams slots, up to max_batch requests per
@@ -280,6 +282,8 @@ int mbx_serve_replay(mbx_ctx *ctx, const mbx_query *reqs, uint32_t nq,
                      mbx_serve_stats *stats);
 
 /* --------------------------------------------------------------- timing */
+/* Event pairs on the context's stream, as DOWORK times its launches with
+ * cudaEventRecord / cudaEventElapsedTime (ssb/lambdas.cuh:158-240). */
 typedef struct mbx_timer mbx_timer;
 int mbx_timer_create(mbx_ctx *ctx, mbx_timer **t);
 int mbx_timer_start(mbx_ctx *ctx, mbx_timer *t);
@@ -293,7 +297,9 @@ int mbx_copy_kernel(mbx_ctx *ctx, void *dst, const void *src, size_t bytes);
 
 
 /* ------------------------------------------------------------ plans */
-/* A plan of one query: the column order of a conjunctive program
+/* No reference counterpart beyond the static bin choice of col::from_bin2
+ * (bmpidx_create.cu:165-174); these pick evaluation order and fused vs
+ * materialised execution per query.  A plan of one query: the column order of a conjunctive program
  * (ORM r0 <- c ; ANDM r0 <- c ; ...) and the execution plan (MBX_PLAN_*).
  * Every plan of a query selects the same rows and gives the same sums. */
 #define MBX_NFEAT 16
@@ -325,7 +331,8 @@ int mbx_prog_grpby_plan(mbx_ctx *ctx, const mbx_vprg *prg, const mbx_rowop *op,
                         uint32_t nr_grp, int64_t *d_out, const mbx_plan *plan);
 
 /* ------------------------------------------------------ cost-model MLP */
-/* y = W2 relu(W1 x + b1) + b2 in fp32; parameters flat W1|b1|W2|b2.
+/* No reference counterpart.  y = W2 relu(W1 x + b1) + b2 in fp32;
+ * parameters flat W1|b1|W2|b2.
  * Forward, backward and Adam run as HIP kernels on the device. */
 int mbx_mlp_create(mbx_ctx *ctx, int din, int dh, int dout, uint64_t seed,
                    mbx_mlp **out);
