@@ -744,14 +744,6 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t x,
 }
 
 // ------------------------------------------------------------ tile pipeline
-// Orders this wave's LDS writes before its own later LDS reads: the LDS
-// executes a wave's instructions in issue order, so only the compiler must
-// be kept from moving accesses across this point.
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // One wave's share: lane l holds the words w0 + l + k * ws (k < WPL) below
 // wend -- WPL chunks of 64 consecutive words, ws apart.  Program -> popcount
