@@ -7,10 +7,12 @@
 // reads row 64g+l of every group (coalesced 4-byte loads), so a wave64
 // ballot of a per-row predicate is exactly one 64-row bitmap word: the two
 // reference uint32 words of those rows, in little-endian order.  The wave
-// then holds 16 words = 128 bytes per bin, lane j the line of bin j.  The
-// lines go out through LDS, transposed so that each 16-byte store
-// instruction writes 8 whole lines (8 lanes per line) rather than 16-byte
-// pieces of 64 lines: 8 times fewer write requests to L2.
+// then holds 16 words = 128 bytes per bin, lane j the line of bin j.  With
+// more than 8 bins the lines go out through LDS, transposed so that each
+// 16-byte store instruction writes 8 whole lines (8 lanes per line) rather
+// than 16-byte pieces of 64 lines: 8 times fewer write requests to L2.  With
+// up to 8 bins each lane stores its own line and the kernel keeps its LDS
+// (and occupancy) for the gathers of the foreign-key path.
 //
 // Consecutive bins (maxs[j] == mins[j+1], the reference's bin tables,
 // ssb/bmpidx-create.cu:8-30) take the radix path: each row gets a bin index,
@@ -105,7 +107,7 @@ __device__ __forceinline__ void store_lines(const BinArgs &a, u64x2 *tr,
   }
   wave_lds_sync();
   const uint32_t part = lane & 7u;
-#pragma unroll
+#pragma unroll 1
   for (uint32_t t = 0; t < 8; ++t) {
     const uint32_t line = t * 8 + (lane >> 3);
     if (line < n) {
@@ -118,10 +120,28 @@ __device__ __forceinline__ void store_lines(const BinArgs &a, u64x2 *tr,
   wave_lds_sync();   // the next chunk rewrites tr
 }
 
-template <bool RADIX, bool SORTED>
+// One lane writes a full 128-byte line (16 words of 64 rows).
+__device__ __forceinline__ void store_line(uint32_t *dst, const uint64_t *w) {
+  u64x2 *d = reinterpret_cast<u64x2 *>(dst);
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    u64x2 x;
+    x.x = w[2 * i];
+    x.y = w[2 * i + 1];
+    d[i] = x;   // plain stores: L2 merges the 8 parts into one full-line write
+  }
+}
+
+// Transpose buffer of the TR kernels (more than 8 bins): 8 KB per wave, as
+// dynamic LDS of kTransposeLds bytes.  The kernels for up to 8 bins have
+// neither the buffer nor its registers (76 against 104 VGPRs: 6 waves per
+// SIMD against 4 for the latency-bound foreign-key gathers).
+extern __shared__ u64x2 s_tr_dyn[];
+constexpr size_t kTransposeLds = 4 * 64 * 8 * sizeof(u64x2);
+
+template <bool RADIX, bool SORTED, bool TR>
 __global__ __launch_bounds__(256) void bins_kernel(BinArgs a) {
   __shared__ uint64_t s_bounds[SORTED ? kMaxSortedBounds : 1];
-  __shared__ u64x2 s_tr[4][64 * 8];
   if (SORTED) {
     for (uint32_t i = threadIdx.x; i <= a.nb; i += blockDim.x)
       s_bounds[i] = a.mins[i];
@@ -182,7 +202,13 @@ __global__ __launch_bounds__(256) void bins_kernel(BinArgs a) {
           }
         }
       }
-      store_lines(a, s_tr[threadIdx.x >> 6], w, cbase, inchunk, blk);
+      if (TR)
+        store_lines(a, s_tr_dyn + (threadIdx.x >> 6) * 64 * 8, w, cbase,
+                    inchunk, blk);
+      else if (lane < inchunk)
+        store_line(a.out + (size_t)(cbase + lane) * a.stride_words +
+                       (size_t)blk * 32,
+                   w);
     }
   }
 }
@@ -270,15 +296,19 @@ static int launch_bins(mbx_ctx *ctx, const mbx_colref *src, uint32_t nrows,
   const size_t cap = (size_t)ctx->num_cus * 16;
   if (wgs > cap) wgs = cap;
   if (wgs == 0) wgs = 1;
-  if (a.mode == kGeneric)
-    hipLaunchKernelGGL((bins_kernel<false, false>), dim3((unsigned)wgs),
-                       dim3(256), 0, ctx->stream, a);
-  else if (a.mode == kSorted)
-    hipLaunchKernelGGL((bins_kernel<true, true>), dim3((unsigned)wgs),
-                       dim3(256), 0, ctx->stream, a);
-  else
-    hipLaunchKernelGGL((bins_kernel<true, false>), dim3((unsigned)wgs),
-                       dim3(256), 0, ctx->stream, a);
+  const bool tr = nb > 8;
+  const size_t lds = tr ? kTransposeLds : 0;
+  const dim3 g((unsigned)wgs), b(256);
+  if (a.mode == kGeneric) {
+    if (tr) hipLaunchKernelGGL((bins_kernel<false, false, true>), g, b, lds, ctx->stream, a);
+    else hipLaunchKernelGGL((bins_kernel<false, false, false>), g, b, lds, ctx->stream, a);
+  } else if (a.mode == kSorted) {
+    if (tr) hipLaunchKernelGGL((bins_kernel<true, true, true>), g, b, lds, ctx->stream, a);
+    else hipLaunchKernelGGL((bins_kernel<true, true, false>), g, b, lds, ctx->stream, a);
+  } else {
+    if (tr) hipLaunchKernelGGL((bins_kernel<true, false, true>), g, b, lds, ctx->stream, a);
+    else hipLaunchKernelGGL((bins_kernel<true, false, false>), g, b, lds, ctx->stream, a);
+  }
   hipError_t e = hipGetLastError();
   if (tmp) {
     // the bounds must outlive the kernel: wait before returning them
