@@ -429,8 +429,8 @@ def run_learned(dev, nprog=20000, neval=200):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--rows", type=int, default=None,
                     help="fact rows (default: config 2's 120M)")
     ap.add_argument("--no-extras", action="store_true",
