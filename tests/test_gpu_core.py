@@ -22,7 +22,9 @@ SIZES = [1, 31, 32, 33, 63, 64, 65, 1000, 1023, 1024, 1025, 4099, 65537,
 @pytest.mark.parametrize("dt", [np.uint8, np.uint16, np.uint32, np.uint64])
 def test_bins_all_modes(dev, n, dt):
     """Byte-identical bitmaps vs ref_create_bin for every bin mode: width-1,
-    equal-width, sorted (uneven) and arbitrary (overlapping) bins."""
+    equal-width, sorted (uneven) and arbitrary (overlapping) bins, each with
+    more than 8 bins (lines stored through the LDS transpose) and with up to
+    8 (one lane per line)."""
     rng = np.random.default_rng(n + np.dtype(dt).itemsize)
     v = rng.integers(0, 300, n).astype(dt)
     d = dev.upload(v)
@@ -31,7 +33,10 @@ def test_bins_all_modes(dev, n, dt):
     sets = [(list(range(10, 110)), list(range(11, 111))),          # unit, 100 bins
             (list(range(0, 300, 30)), list(range(30, 330, 30))),   # equal
             (list(uneven[:-1]), list(uneven[1:])),                 # sorted
-            (list(rng.integers(0, 150, 7)), list(rng.integers(100, 300, 7)))]
+            (list(rng.integers(0, 150, 7)), list(rng.integers(100, 300, 7))),
+            (list(range(40, 45)), list(range(41, 46))),            # unit, 5 bins
+            (list(rng.integers(0, 150, 70)),                       # arbitrary, 70
+             list(rng.integers(100, 300, 70)))]
     for mins, maxs in sets:
         nb = len(mins)
         outs = (C.c_void_p * nb)()
