@@ -104,9 +104,22 @@ def query_cols(spec, idx):
     return cols
 
 
-def cpu_baseline(dev, sch, specs, target_s=12.0):
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(dev, sch, specs, target_s=5.0, reps=3):
     """The oracle (raw-column scan, OpenMP) on the host cores, on the first
-    R fact rows; R is sized from a short probe to ~target_s of CPU work."""
+    R fact rows, R sized from a short probe to ~target_s of CPU work per
+    pass; the median of `reps` passes on all threads, then one pass on one
+    thread over a sample sized the same way."""
     from merle_amd import spec as S
     from oracle import cpuref
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -120,29 +133,46 @@ def cpu_baseline(dev, sch, specs, target_s=12.0):
     for k in fact:
         host.add(k, sch[k].dev.host(probe))
     host.nrows = probe
-    t0 = time.perf_counter()
-    for s in specs:
-        cpuref.query(s, host, nthreads=threads, row0=0, row1=probe)
-    per_row = (time.perf_counter() - t0) / (probe * len(specs))
-    R = int(min(n, max(probe, target_s / (per_row * len(specs)))))
-    R -= R % 64
+
+    def rows_for(nthreads):
+        t0 = time.perf_counter()
+        for s in specs:
+            cpuref.query(s, host, nthreads=nthreads, row0=0, row1=min(probe, 1_000_000))
+        per_row = (time.perf_counter() - t0) / (min(probe, 1_000_000) * len(specs))
+        r = int(min(n, max(64, target_s / (per_row * len(specs)))))
+        return r - r % 64
+
+    def run(nthreads, r):
+        res = []
+        t0 = time.perf_counter()
+        for s in specs:
+            out, _ = cpuref.query(s, host, nthreads=nthreads, row0=0, row1=r)
+            res.append(out)
+        return time.perf_counter() - t0, res
+
+    R = max(probe, rows_for(threads))
     if R > probe:
         for k in fact:
             host.cols[k].host = sch[k].dev.host(R)
             host.cols[k].n = R
     host.nrows = R
-    results = []
-    t0 = time.perf_counter()
-    for s in specs:
-        out, _ = cpuref.query(s, host, nthreads=threads, row0=0, row1=R)
-        results.append(out)
-    dt = time.perf_counter() - t0
+    passes = [run(threads, R) for _ in range(reps)]
+    dt = float(np.median([p[0] for p in passes]))
+    results = passes[0][1]
+    R1 = min(R, rows_for(1))
+    dt1, _ = run(1, R1)
     rate = R * len(specs) / dt / 1e9
     return dict(value=round(rate, 4), unit="G rows/s", cores=threads,
                 kind="port",
                 sample=f"the 12 config-2 queries over the first {R:,} of "
                        f"{n:,} fact rows (oracle raw-column scan, "
-                       f"{threads} OpenMP threads, {dt:.1f} s)"), R, results
+                       f"{threads} OpenMP threads, median of {reps} passes "
+                       f"of {dt:.1f} s)",
+                one_core={"value": round(R1 * len(specs) / dt1 / 1e9, 4),
+                          "unit": "G rows/s",
+                          "sample": f"the same queries over the first "
+                                    f"{R1:,} rows, one thread, {dt1:.1f} s"},
+                cpu_model=_cpu_model(), nproc=os.cpu_count()), R, results
 
 
 def make_bins(dev, sch, bounds_of):
