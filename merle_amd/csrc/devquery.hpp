@@ -308,6 +308,21 @@ __device__ __forceinline__ void eval_words(const DevPrg &P, uint32_t w0,
           }
         }
       }
+      if (g + kConjGroup < P.nfl) {
+        // Early-out between groups: a word whose running AND is empty in
+        // every lane skips the later groups' loads (uniform branch), and a
+        // lane whose word is empty loads the first live lane's address, so
+        // a 128-byte line with no live word is not fetched.
+#pragma unroll
+        for (int k = 0; k < WPL; ++k) {
+          const bool live = acc[k] != 0;
+          const uint64_t m = __ballot(live);
+          any[k] = m != 0;
+          const uint32_t w = w0 + ws * k;
+          const uint32_t wf = __shfl(w, any[k] ? (int)__builtin_ctzll(m) : 0, 64);
+          wa[k] = live ? w : wf;
+        }
+      }
     }
 #pragma unroll
     for (int k = 0; k < WPL; ++k) {
