@@ -179,9 +179,13 @@ enum mbx_flags {
   MBX_PLAN_SCAN = 2,         /* grpby on every row (primitive.cuh:16-48) */
   MBX_PLAN_MASK = 3,
   MBX_ACCUMULATE = 16,       /* add into d_out instead of zeroing it first */
-  MBX_PLAN_EAGER = 32        /* fused/materialise: load every bitmap operand
+  MBX_PLAN_EAGER = 32,       /* fused/materialise: load every bitmap operand
                                 up front instead of skipping ANDM loads for
                                 empty words (same bits, fewer round trips) */
+  MBX_PLAN_DIRECT = 64       /* fused, ablation: no compaction -- each lane
+                                walks the set bits of its own words and runs
+                                the row op on them (abla::method1,
+                                primitive_abla.cuh:62-108) */
 };
 
 /* Program + group-by in one call.  d_out[nr_grp] int64 on the device.

@@ -21,6 +21,7 @@ MEAS_COUNT, MEAS_COL, MEAS_MUL, MEAS_SUB = range(4)
 PLAN_FUSED, PLAN_MATERIALISE, PLAN_SCAN = 0, 1, 2
 ACCUMULATE = 16
 PLAN_EAGER = 32
+PLAN_DIRECT = 64
 
 
 class ColRef(C.Structure):

@@ -14,8 +14,8 @@
 // col::from_bin2; edge bins -> candidate checks), timed per plan as the mean
 // of `reps` runs.  It prints the reference's five-block report
 // (drawFigs/drawFigs.py:15-46): index creation and the SSB block in full,
-// the synthetic block in full when synth_rows > 0 (seeded columns of that
-// many rows, see synth_suite), the WAH / RTScan and method blocks with
+// the method and synthetic blocks in full when synth_rows > 0 (seeded
+// columns of that many rows, see synth_suite), the WAH / RTScan block with
 // headers and n/a values only.  SSB columns: Join = the bitmap-free scan
 // (Sparse rows only, as DOWORK), Ours = fused under the library's static
 // plan, PrgStg1 / Stg2 / Stg3 = the three materialise stages; the plans
@@ -310,136 +310,210 @@ bool static_plan(mbx_ctx *ctx, const mbx_vprg &ap, mbx_plan &plan) {
   return true;
 }
 
-// Block 4 of the report: the synthetic suite of synth_demoall
-// (synth/cpuref.c:184-257; one row per case and selectivity as synth_bmp,
-// synth/synthdemo.cu:79-145, times it).  The query (synth/synthdemo.h:11-16):
-// COUNT(*) GROUP BY dim.attr2 over the rows with (a1 in [lo, hi) OR a2 in
-// [lo, hi)) AND dim.attr1[fk] in [lo, hi), under the perfect, many-OR and
-// candidate-check recipes, fused and as the three materialise stages.
-// Data per case: a1, a2 = Zipf(s) on [0, 65535) stored at the case's bit
-// width, fk = Zipf(s) on [0, N/100), dim.attr1[i] = i, dim.attr2 = uniform
-// bytes.  The reference reads files of unseeded numpy zipf mod 65535
-// (synth/onegen_zipf.py:5-10, batchgen.sh:12-25); these columns are seeded
-// and generated here.  [lo, lo + 60) is the first window whose probability
-// under the generator's pmf is at most the column selectivity of
-// synth/cpuref.c:191-195.  Returns the number of plans whose sums differ
-// from the scan's, or -1 on an error.
+// ------------------------------------------------ the synthetic suite
+// synth_demoall (synth/cpuref.c:184-257).  The query (synth/synthdemo.h:
+// 11-16): COUNT(*) GROUP BY dim.attr2 over the rows with (a1 in [lo, hi) OR
+// a2 in [lo, hi)) AND dim.attr1[fk] in [lo, hi).  Data per case: a1, a2 =
+// Zipf(s) on [0, 65535) stored at the case's bit width, fk = Zipf(s) on
+// [0, N/100), dim.attr1[i] = i, dim.attr2 = uniform bytes.  The reference
+// reads files of unseeded numpy zipf mod 65535 (synth/onegen_zipf.py:5-10,
+// batchgen.sh:12-25); these columns are seeded and generated here.
+// [lo, lo + 60) is the first window whose probability under the generator's
+// pmf is at most the column selectivity of synth/cpuref.c:191-195.
+constexpr uint32_t kSynthVals = 65535, kSynthWin = 60, kSynthGroups = 256;
+const double kSynthColSel[11] = {.02, .04, .06, .09, .12, .18,
+                                 .25, .39, .51, .67, .82};
+const double kSynthFulSel[11] = {1 / 512.0, 1 / 256.0, 1 / 128.0, 1 / 64.0,
+                                 1 / 32.0,  1 / 16.0,  1 / 8.0,   1 / 4.0,
+                                 .4,        .6,        .8};
+
+struct SynthCase {
+  uint32_t n = 0, dimsz = 0, bw = 0;
+  double s = 0;
+  void *col[3] = {nullptr, nullptr, nullptr};   // a1, a2, fk
+  void *dim1 = nullptr, *dim2 = nullptr;
+  std::vector<double> pre;                       // prefix sums of the pmf
+};
+
+void synth_free(mbx_ctx *ctx, SynthCase &c) {
+  for (void *&q : c.col) if (q) mbx_free(ctx, q), q = nullptr;
+  if (c.dim1) mbx_free(ctx, c.dim1), c.dim1 = nullptr;
+  if (c.dim2) mbx_free(ctx, c.dim2), c.dim2 = nullptr;
+}
+
+bool synth_case(mbx_ctx *ctx, uint32_t n, double s, uint32_t bw, int idx,
+                SynthCase &c) {
+  c.n = n;
+  c.dimsz = n / 100;
+  c.bw = bw;
+  c.s = s;
+  std::vector<uint32_t> h(n);
+  for (int k = 0; k < 3; ++k) {
+    const uint64_t seed = 0x5eed0000u + 16u * (uint64_t)idx + (uint64_t)k;
+    mbx_gen_zipf_u32_host(h.data(), n, seed, k < 2 ? kSynthVals : c.dimsz, s);
+    const size_t eb = k < 2 ? bw / 8 : 4;
+    if (mbx_alloc(ctx, (size_t)n * eb, &c.col[k])) return false;
+    if (eb == 2) {
+      std::vector<uint16_t> h16(h.begin(), h.end());
+      mbx_memcpy_h2d(ctx, c.col[k], h16.data(), (size_t)n * 2);
+    } else {
+      mbx_memcpy_h2d(ctx, c.col[k], h.data(), (size_t)n * 4);
+    }
+  }
+  std::vector<uint32_t> iota(c.dimsz), u(c.dimsz);
+  for (uint32_t i = 0; i < c.dimsz; ++i) iota[i] = i;
+  mbx_gen_uniform_u32_host(u.data(), c.dimsz, 13458786, 0, 256);
+  std::vector<uint8_t> b(u.begin(), u.end());
+  if (mbx_alloc(ctx, (size_t)c.dimsz * 4, &c.dim1) ||
+      mbx_alloc(ctx, c.dimsz, &c.dim2))
+    return false;
+  mbx_memcpy_h2d(ctx, c.dim1, iota.data(), (size_t)c.dimsz * 4);
+  mbx_memcpy_h2d(ctx, c.dim2, b.data(), c.dimsz);
+  c.pre.assign(kSynthVals + 1, 0.0);
+  for (uint32_t v = 0; v < kSynthVals; ++v)
+    c.pre[v + 1] = c.pre[v] + std::pow((double)v + 1.0, -s);
+  return true;
+}
+
+// The query of selectivity index o: its recipe inputs and row op.
+void synth_query(const SynthCase &c, int o, recipe &r, mbx_rowop &op) {
+  uint32_t lo = 0;
+  while (lo + kSynthWin < kSynthVals &&
+         (c.pre[lo + kSynthWin] - c.pre[lo]) / c.pre[kSynthVals] > kSynthColSel[o])
+    ++lo;
+  const uint32_t hi = lo + kSynthWin;
+  const uint32_t *fk = static_cast<const uint32_t *>(c.col[2]);
+  r = recipe();
+  r.factsz = c.n;
+  std::memset(&op, 0, sizeof(op));
+  op.npreds = 3;
+  for (int k = 0; k < 3; ++k) {
+    r.nbit[k] = k < 2 ? c.bw : 32;
+    r.min[k] = lo;
+    r.max[k] = hi;
+    r.dimsz[k] = k < 2 ? recipe::nodim : c.dimsz;
+    r.fk[k] = k < 2 ? nullptr : fk;
+    r.attr[k] = k < 2 ? c.col[k] : c.dim1;
+    op.preds[k].src.data = k < 2 ? c.col[k] : c.dim1;
+    op.preds[k].src.fk = k < 2 ? nullptr : fk;
+    op.preds[k].src.dimsz = k < 2 ? 0 : c.dimsz;
+    op.preds[k].src.elem_bytes = k < 2 ? c.bw / 8 : 4;
+    op.preds[k].lo = lo;
+    op.preds[k].hi = hi;
+  }
+  op.nkeys = 1;
+  key(op.keys[0], mbx_colref{c.dim2, fk, c.dimsz, 1}, MBX_KEY_IDENT, 1, 0, 1);
+  op.measure = MBX_MEAS_COUNT;
+}
+
+void synth_instrs(vprg::instr (&ins3)[MAXNINSTR]) {
+  ins3[0] = ins(vprg::ORM, 0, 0);
+  ins3[1] = ins(vprg::ORM, 0, 1);
+  ins3[2] = ins(vprg::ANDM, 0, 2);
+}
+
+// Block 3: method 1 against method 2 (synth_method, synth/synthdemo.cu:
+// 147-204) on the skew-1.04, 32-bit case, the perfect recipe and the seven
+// smallest selectivities.  Method 1 is the fused plan without compaction
+// (MBX_PLAN_DIRECT), method 2 the fused plan.  Returns mismatches, -1 on an
+// error.
+template <class Timed>
+int synth_methods(mbx_ctx *ctx, uint32_t n, Timed &&timed) {
+  SynthCase c;
+  if (!synth_case(ctx, n, 1.04, 32, 1, c)) {
+    std::fprintf(stderr, "synthetic data: %s\n", mbx_last_error());
+    synth_free(ctx, c);
+    return -1;
+  }
+  void *d1 = nullptr, *d2 = nullptr;
+  if (mbx_alloc(ctx, 8 * kSynthGroups, &d1) || mbx_alloc(ctx, 8 * kSynthGroups, &d2)) {
+    synth_free(ctx, c);
+    return -1;
+  }
+  int bad = 0;
+  std::vector<int64_t> h1(kSynthGroups), h2(kSynthGroups);
+  vprg::instr ins3[MAXNINSTR];
+  synth_instrs(ins3);
+  for (int o = 0; o < 7; ++o) {
+    recipe r;
+    mbx_rowop op;
+    synth_query(c, o, r, op);
+    vprg p = r.perfect(ins3);
+    const mbx_vprg ap = p.to_abi();
+    const float m1 = timed([&] {
+      return mbx_prog_grpby(ctx, &ap, &op, kSynthGroups, static_cast<int64_t *>(d1),
+                            MBX_PLAN_FUSED | MBX_PLAN_DIRECT);
+    });
+    const float m2 = timed([&] {
+      return mbx_prog_grpby(ctx, &ap, &op, kSynthGroups, static_cast<int64_t *>(d2),
+                            MBX_PLAN_FUSED);
+    });
+    mbx_memcpy_d2h(ctx, h1.data(), d1, 8 * kSynthGroups);
+    mbx_memcpy_d2h(ctx, h2.data(), d2, 8 * kSynthGroups);
+    bad += std::memcmp(h1.data(), h2.data(), 8 * kSynthGroups) != 0;
+    p.release();
+    std::printf("%.5f\t%.4f\t%.4f\n", kSynthFulSel[o], m1, m2);
+    std::fflush(stdout);
+  }
+  mbx_free(ctx, d1);
+  mbx_free(ctx, d2);
+  synth_free(ctx, c);
+  return bad;
+}
+
+// Block 4: one row per case and selectivity as synth_bmp times it
+// (synth/synthdemo.cu:79-145): the scan, the perfect, many-OR and
+// candidate-check recipes fused, and each recipe's three materialise
+// stages.  Returns the number of plans whose sums differ from the scan's,
+// or -1 on an error.
 template <class Timed>
 int synth_suite(mbx_ctx *ctx, uint32_t n, Timed &&timed) {
   static const double kSkew[5] = {1.04, 1.08, 1.12, 1.16, 1.20};
-  static const double kColSel[11] = {.02, .04, .06, .09, .12, .18,
-                                     .25, .39, .51, .67, .82};
-  static const double kFulSel[11] = {1 / 512.0, 1 / 256.0, 1 / 128.0,
-                                     1 / 64.0,  1 / 32.0,  1 / 16.0,
-                                     1 / 8.0,   1 / 4.0,   .4, .6, .8};
-  constexpr uint32_t kVals = 65535, kWin = 60, kGroups = 256;
-  const uint32_t dimsz = n / 100;
-  if (dimsz == 0) {
-    std::fprintf(stderr, "synthetic suite: need at least 100 rows\n");
-    return -1;
-  }
   const size_t nw = mbx_bitmap_stride_words(n);
   void *poss = nullptr, *unc = nullptr, *rows = nullptr, *count = nullptr,
-       *out = nullptr, *want = nullptr, *dim1 = nullptr, *dim2 = nullptr;
+       *out = nullptr, *want = nullptr;
   if (mbx_alloc(ctx, nw * 4, &poss) || mbx_alloc(ctx, nw * 4, &unc) ||
       mbx_alloc(ctx, (size_t)n * 4, &rows) || mbx_alloc(ctx, 256, &count) ||
-      mbx_alloc(ctx, 8 * kGroups, &out) || mbx_alloc(ctx, 8 * kGroups, &want) ||
-      mbx_alloc(ctx, (size_t)dimsz * 4, &dim1) || mbx_alloc(ctx, dimsz, &dim2)) {
+      mbx_alloc(ctx, 8 * kSynthGroups, &out) ||
+      mbx_alloc(ctx, 8 * kSynthGroups, &want)) {
     std::fprintf(stderr, "alloc: %s\n", mbx_last_error());
     return -1;
   }
-  {
-    std::vector<uint32_t> iota(dimsz), u(dimsz);
-    for (uint32_t i = 0; i < dimsz; ++i) iota[i] = i;
-    mbx_gen_uniform_u32_host(u.data(), dimsz, 13458786, 0, 256);
-    std::vector<uint8_t> b(u.begin(), u.end());
-    mbx_memcpy_h2d(ctx, dim1, iota.data(), (size_t)dimsz * 4);
-    mbx_memcpy_h2d(ctx, dim2, b.data(), dimsz);
-  }
   int bad = 0;
-  std::vector<int64_t> h_want(kGroups), h_got(kGroups);
-  std::vector<uint32_t> col(n);
-  for (int c = 0; c < 10; ++c) {
-    const double s = kSkew[c / 2];
-    const uint32_t bw = c % 2 ? 32 : 16;
-    // the case's columns
-    void *d[3] = {nullptr, nullptr, nullptr};   // a1, a2, fk
-    for (int k = 0; k < 3; ++k) {
-      const uint64_t seed = 0x5eed0000u + 16u * (uint64_t)c + (uint64_t)k;
-      mbx_gen_zipf_u32_host(col.data(), n, seed, k < 2 ? kVals : dimsz, s);
-      const size_t eb = k < 2 ? bw / 8 : 4;
-      if (mbx_alloc(ctx, (size_t)n * eb, &d[k])) {
-        std::fprintf(stderr, "alloc: %s\n", mbx_last_error());
-        return -1;
-      }
-      if (eb == 2) {
-        std::vector<uint16_t> h(col.begin(), col.end());
-        mbx_memcpy_h2d(ctx, d[k], h.data(), (size_t)n * 2);
-      } else {
-        mbx_memcpy_h2d(ctx, d[k], col.data(), (size_t)n * 4);
-      }
+  std::vector<int64_t> h_want(kSynthGroups), h_got(kSynthGroups);
+  vprg::instr ins3[MAXNINSTR];
+  synth_instrs(ins3);
+  for (int ci = 0; ci < 10; ++ci) {
+    SynthCase c;
+    if (!synth_case(ctx, n, kSkew[ci / 2], ci % 2 ? 32 : 16, ci, c)) {
+      std::fprintf(stderr, "synthetic data: %s\n", mbx_last_error());
+      synth_free(ctx, c);
+      return -1;
     }
-    // window probabilities of the attribute pmf, (v+1)^-s / H
-    std::vector<double> pre(kVals + 1, 0.0);
-    for (uint32_t v = 0; v < kVals; ++v)
-      pre[v + 1] = pre[v] + std::pow((double)v + 1.0, -s);
-    const uint32_t *fk = static_cast<const uint32_t *>(d[2]);
     for (int o = 1; o < 11; ++o) {
-      uint32_t lo = 0;
-      while (lo + kWin < kVals &&
-             (pre[lo + kWin] - pre[lo]) / pre[kVals] > kColSel[o])
-        ++lo;
-      const uint32_t hi = lo + kWin;
       recipe r;
-      r.factsz = n;
-      for (int k = 0; k < 3; ++k) {
-        r.nbit[k] = k < 2 ? bw : 32;
-        r.min[k] = lo;
-        r.max[k] = hi;
-        r.dimsz[k] = k < 2 ? recipe::nodim : dimsz;
-        r.fk[k] = k < 2 ? nullptr : fk;
-        r.attr[k] = k < 2 ? d[k] : dim1;
-      }
-      vprg::instr ins3[MAXNINSTR];
-      ins3[0] = ins(vprg::ORM, 0, 0);
-      ins3[1] = ins(vprg::ORM, 0, 1);
-      ins3[2] = ins(vprg::ANDM, 0, 2);
       mbx_rowop op;
-      std::memset(&op, 0, sizeof(op));
-      op.npreds = 3;
-      for (int k = 0; k < 3; ++k) {
-        op.preds[k].src.data = k < 2 ? d[k] : dim1;
-        op.preds[k].src.fk = k < 2 ? nullptr : fk;
-        op.preds[k].src.dimsz = k < 2 ? 0 : dimsz;
-        op.preds[k].src.elem_bytes = k < 2 ? bw / 8 : 4;
-        op.preds[k].lo = lo;
-        op.preds[k].hi = hi;
-      }
-      op.nkeys = 1;
-      key(op.keys[0], mbx_colref{dim2, fk, dimsz, 1}, MBX_KEY_IDENT, 1, 0, 1);
-      op.measure = MBX_MEAS_COUNT;
-      float t[4][4] = {};   // [plan][fused, stage 1, 2, 3]; join in t[3][0]
-      bool first = true;
+      synth_query(c, o, r, op);
+      float t[4][4] = {};   // [recipe][fused, stage 1, 2, 3]; scan in t[3][0]
       for (int kind = 0; kind < 3; ++kind) {
         vprg p = kind == 0 ? r.perfect(ins3)
                            : kind == 1 ? r.many_or(ins3) : r.candchk(ins3);
         const mbx_vprg ap = p.to_abi();
-        if (first) {
+        if (kind == 0) {
           t[3][0] = timed([&] {
-            return mbx_prog_grpby(ctx, &ap, &op, kGroups,
+            return mbx_prog_grpby(ctx, &ap, &op, kSynthGroups,
                                   static_cast<int64_t *>(want), MBX_PLAN_SCAN);
           });
-          mbx_memcpy_d2h(ctx, h_want.data(), want, 8 * kGroups);
-          first = false;
+          mbx_memcpy_d2h(ctx, h_want.data(), want, 8 * kSynthGroups);
         }
         mbx_plan plan;
         if (!static_plan(ctx, ap, plan)) return -1;
         t[kind][0] = timed([&] {
-          return mbx_prog_grpby_plan(ctx, &ap, &op, kGroups,
+          return mbx_prog_grpby_plan(ctx, &ap, &op, kSynthGroups,
                                      static_cast<int64_t *>(out), &plan);
         });
-        mbx_memcpy_d2h(ctx, h_got.data(), out, 8 * kGroups);
-        bad += std::memcmp(h_want.data(), h_got.data(), 8 * kGroups) != 0;
+        mbx_memcpy_d2h(ctx, h_got.data(), out, 8 * kSynthGroups);
+        bad += std::memcmp(h_want.data(), h_got.data(), 8 * kSynthGroups) != 0;
         uint32_t *pu = p.is_nochk() ? nullptr : static_cast<uint32_t *>(unc);
         t[kind][1] = timed([&] {
           return mbx_prog_eval(ctx, &ap, static_cast<uint32_t *>(poss), pu);
@@ -452,22 +526,22 @@ int synth_suite(mbx_ctx *ctx, uint32_t n, Timed &&timed) {
         t[kind][3] = timed([&] {
           return mbx_rows_grpby(ctx, static_cast<uint32_t *>(rows),
                                 static_cast<uint32_t *>(count), &ap, &op,
-                                kGroups, static_cast<int64_t *>(out), 0);
+                                kSynthGroups, static_cast<int64_t *>(out), 0);
         });
-        mbx_memcpy_d2h(ctx, h_got.data(), out, 8 * kGroups);
-        bad += std::memcmp(h_want.data(), h_got.data(), 8 * kGroups) != 0;
+        mbx_memcpy_d2h(ctx, h_got.data(), out, 8 * kSynthGroups);
+        bad += std::memcmp(h_want.data(), h_got.data(), 8 * kSynthGroups) != 0;
         p.release();
       }
-      std::printf("%.2f\t%u\t%.5f\t%.4f\t%.4f\t%.4f\t%.4f", s, bw, kFulSel[o],
-                  t[3][0], t[0][0], t[1][0], t[2][0]);
+      std::printf("%.2f\t%u\t%.5f\t%.4f\t%.4f\t%.4f\t%.4f", c.s, c.bw,
+                  kSynthFulSel[o], t[3][0], t[0][0], t[1][0], t[2][0]);
       for (int kind = 0; kind < 3; ++kind)
         std::printf("\t%.4f\t%.4f\t%.4f", t[kind][1], t[kind][2], t[kind][3]);
       std::printf("\t0.0\n");
       std::fflush(stdout);
     }
-    for (void *q : d) mbx_free(ctx, q);
+    synth_free(ctx, c);
   }
-  for (void *q : {poss, unc, rows, count, out, want, dim1, dim2}) mbx_free(ctx, q);
+  for (void *q : {poss, unc, rows, count, out, want}) mbx_free(ctx, q);
   return bad;
 }
 
@@ -636,12 +710,18 @@ int main(int argc, char **argv) {
       std::printf("\t%.4f\t0\t0\t%.4f\t%.4f\t%.4f\t0", ours, s1, s2, s3);
     }
   }
-  // block 3 (method 1 / method 2, ablation kernels this library does not
-  // build) carries its header alone; block 4 is the synthetic suite when
-  // synth_rows > 0
-  std::printf("\n\nSeletiv\tMethod1\tMethod2\n\nSkew\tBitW\tSeletiv\tJoin\t"
-              "Perfect\tManyOrs\tCandChk\tPftStg1\tPftStg2\tPftStg3\tOrsStg1\t"
-              "OrsStg2\tOrsStg3\tChkStg1\tChkStg2\tChkStg3\tWAH\n");
+  // blocks 3 (method 1 / method 2) and 4 (the synthetic suite) when
+  // synth_rows > 0, else their headers alone
+  std::printf("\n\nSeletiv\tMethod1\tMethod2\n");
+  std::fflush(stdout);
+  if (synth_rows) {
+    const int mb = synth_methods(ctx, synth_rows, timed);
+    if (mb < 0) return 2;
+    bad += mb;
+  }
+  std::printf("\nSkew\tBitW\tSeletiv\tJoin\tPerfect\tManyOrs\tCandChk\t"
+              "PftStg1\tPftStg2\tPftStg3\tOrsStg1\tOrsStg2\tOrsStg3\tChkStg1\t"
+              "ChkStg2\tChkStg3\tWAH\n");
   std::fflush(stdout);
   if (synth_rows) {
     const int sb = synth_suite(ctx, synth_rows, timed);

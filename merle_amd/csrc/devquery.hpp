@@ -770,7 +770,7 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t x,
 // rounds: a dense range takes more rounds, each using all lanes.
 constexpr uint32_t kWaveList = kListCap / 4;
 
-template <bool CHK, bool ACCT, int WPL, int EAGER, int E = 2>
+template <bool CHK, bool ACCT, int WPL, int EAGER, int E = 2, bool DIRECT = false>
 __device__ __forceinline__ void run_wave_range(const DevPrg &P, const DevOp &O,
                                                uint32_t nr_grp, uint32_t w0,
                                                uint32_t ws, uint32_t wend,
@@ -780,6 +780,29 @@ __device__ __forceinline__ void run_wave_range(const DevPrg &P, const DevOp &O,
   const uint32_t lane = lane_id();
   uint64_t p[WPL], u[WPL];
   eval_words<CHK, ACCT, EAGER, WPL>(P, w0 + lane, ws, wend, p, u, acct);
+  if (DIRECT) {
+    // The ablation without compaction (abla::method1, primitive_abla.cuh:
+    // 85-98): each lane walks the set bits of its own words, re-checks an
+    // uncertain row and adds the row straight into the sink; the loop
+    // diverges across lanes, so there is no wave aggregation.
+#pragma unroll
+    for (int k = 0; k < WPL; ++k) {
+      uint64_t m = p[k];
+      const uint32_t row0 = (w0 + ws * k + lane) * 64u;
+      while (m) {
+        const uint32_t b = (uint32_t)__builtin_ctzll(m);
+        m &= m - 1;
+        const uint32_t row = row0 + b;
+        if (CHK && ((u[k] >> b) & 1ull) && !row_passes<ACCT>(P, O, row, acct))
+          continue;
+        uint32_t key;
+        int64_t val;
+        if (row_keyval<ACCT>(O, row, nr_grp, key, val, acct))
+          sink_add(sink, key, val, nr_grp);
+      }
+    }
+    return;
+  }
   uint32_t cnt = 0;
 #pragma unroll
   for (int k = 0; k < WPL; ++k) cnt += __popcll(p[k]);

@@ -31,7 +31,7 @@ struct QArgs {
 // bitmap (DRAM-page friendly), not R scattered ranges.  The workgroup keeps
 // its group table in LDS until the end; the table then goes out through the
 // replicated flush (flush_groups).
-template <bool CHK, bool LDSG, bool ACCT, int WPL, int EAGER>
+template <bool CHK, bool LDSG, bool ACCT, int WPL, int EAGER, bool DIRECT = false>
 __global__ __launch_bounds__(256) void fused_kernel(QArgs a) {
   __shared__ uint32_t s_list[4][kWaveList];
   GroupSink sink;
@@ -44,9 +44,9 @@ __global__ __launch_bounds__(256) void fused_kernel(QArgs a) {
   const uint32_t ws = gridDim.x * 4 * 64;   // words between a wave's chunks
   for (uint32_t w0 = (blockIdx.x * 4 + wv) * 64; w0 < a.prg.nwords;
        w0 += ws * WPL)
-    run_wave_range<CHK, ACCT, WPL, EAGER>(a.prg, a.op, a.nr_grp, w0, ws,
-                                          a.prg.nwords, sink, s_list[wv],
-                                          a.acct);
+    run_wave_range<CHK, ACCT, WPL, EAGER, 2, DIRECT>(a.prg, a.op, a.nr_grp, w0,
+                                                     ws, a.prg.nwords, sink,
+                                                     s_list[wv], a.acct);
   if (LDSG) flush_groups(a.sp, a.out, a.nr_grp, blockIdx.x, gridDim.x);
 }
 
@@ -338,6 +338,20 @@ static void fused_conj(mbx_ctx *ctx, const QArgs &a, size_t lds) {
   else fused_even<C, L, false, 3, 2>(ctx, a, lds);
 }
 
+// The no-compaction ablation: early-out evaluation, 2 words per lane, the
+// resident grid.
+template <bool C, bool L>
+static void fused_direct(mbx_ctx *ctx, const QArgs &a, size_t lds) {
+  const void *fn =
+      reinterpret_cast<const void *>(&fused_kernel<C, L, false, 2, 0, true>);
+  const unsigned per_cu = occupancy(fn, lds);
+  const uint64_t waves = ceil_div(ceil_div(a.prg.nwords, 64), 2);
+  const uint64_t resident = (uint64_t)ctx->num_cus * per_cu * 4;
+  const unsigned grid = (unsigned)ceil_div(waves < resident ? waves : resident, 4);
+  hipLaunchKernelGGL((fused_kernel<C, L, false, 2, 0, true>), dim3(grid),
+                     dim3(256), lds, ctx->stream, a);
+}
+
 template <bool C, bool L>
 static void fused_modes(mbx_ctx *ctx, const QArgs &a, size_t lds, int mode) {
   if (mode == 3) fused_conj<C, L>(ctx, a, lds);
@@ -364,6 +378,11 @@ int launch_fused(mbx_ctx *ctx, const QueryPlan &q, const Acct *acct) {
   if (acct) {
     if (chk) fused_launch<true, false, true, 0>(ctx, a, lds);
     else fused_launch<false, false, true, 0>(ctx, a, lds);
+  } else if (q.direct) {
+    if (chk && q.lds) fused_direct<true, true>(ctx, a, lds);
+    else if (chk) fused_direct<true, false>(ctx, a, lds);
+    else if (q.lds) fused_direct<false, true>(ctx, a, lds);
+    else fused_direct<false, false>(ctx, a, lds);
   } else if (chk) {
     if (q.lds) fused_modes<true, true>(ctx, a, lds, mode);
     else fused_modes<true, false>(ctx, a, lds, mode);
@@ -492,6 +511,7 @@ int mbx_prog_grpby(mbx_ctx *ctx, const mbx_vprg *prg, const mbx_rowop *op,
     if (st) return st;
   }
   q.eager = (flags & MBX_PLAN_EAGER) != 0;
+  q.direct = (flags & MBX_PLAN_DIRECT) != 0;
   switch (flags & MBX_PLAN_MASK) {
   case MBX_PLAN_FUSED: return launch_fused(ctx, q, nullptr);
   case MBX_PLAN_MATERIALISE: return run_materialise(ctx, q);

@@ -13,6 +13,7 @@ struct QueryPlan {
   uint32_t nr_grp = 0;
   bool lds = false;   // group table fits in LDS
   bool eager = false; // all operand loads up front (no ANDM early-out)
+  bool direct = false; // no compaction: lanes walk their own set bits
   // replica tables + counters of the LDS-table flush (zero between
   // launches); null = the context's own.  Launches that may overlap on the
   // device need their own.

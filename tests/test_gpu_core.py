@@ -82,9 +82,10 @@ def test_bins_fk_projection(dev, n):
 @pytest.mark.parametrize("seed", range(16))
 def test_random_queries_all_plans(dev, seed):
     """Random programs (AND/OR/NOT/ANDM/ORM, exact and edge bins), group
-    counts from 1 to >4096 (LDS table and HBM atomics): fused (early-out
-    and eager, which takes the conjunction form for two- and three-valued
-    conjunctions) == materialise == scan == oracle, bit-exact int64."""
+    counts from 1 to >4096 (LDS table and HBM atomics): fused (early-out,
+    eager -- the conjunction form for two- and three-valued conjunctions --
+    and the no-compaction ablation) == materialise == scan == oracle,
+    bit-exact int64."""
     rng = np.random.default_rng(1000 + seed)
     n = int(rng.choice([1, 63, 64, 65, 5000, 70001, 300007]))
     schema = gpuq.upload_schema(dev, datagen.small_schema(n, seed))
@@ -93,6 +94,7 @@ def test_random_queries_all_plans(dev, seed):
         want, _ = cpuref.query(spec, schema)
         cols, allocs = gpuq.plan_cols(dev, schema, spec, rng)
         for plan in (abi.PLAN_FUSED, abi.PLAN_FUSED | abi.PLAN_EAGER,
+                     abi.PLAN_FUSED | abi.PLAN_DIRECT,
                      abi.PLAN_MATERIALISE, abi.PLAN_SCAN):
             got = gpuq.run_query(dev, schema, spec, cols, plan)
             np.testing.assert_array_equal(got, want, err_msg=f"plan {plan} {spec}")
@@ -112,6 +114,7 @@ def test_dense_and_empty(dev):
         want, _ = cpuref.query(spec, schema)
         cols, allocs = gpuq.plan_cols(dev, schema, spec, rng, binned_frac=0)
         for plan in (abi.PLAN_FUSED, abi.PLAN_FUSED | abi.PLAN_EAGER,
+                     abi.PLAN_FUSED | abi.PLAN_DIRECT,
                      abi.PLAN_MATERIALISE, abi.PLAN_SCAN):
             np.testing.assert_array_equal(gpuq.run_query(dev, schema, spec, cols, plan), want)
         gpuq.free_all(dev, allocs)
@@ -179,6 +182,7 @@ def test_config1_full(dev, config1):
                                     None if l is None else ptrs[l],
                                     None if r is None else ptrs[r]))
         for plan in (abi.PLAN_FUSED, abi.PLAN_FUSED | abi.PLAN_EAGER,
+                     abi.PLAN_FUSED | abi.PLAN_DIRECT,
                      abi.PLAN_MATERIALISE, abi.PLAN_SCAN):
             got = gpuq.run_query(dev, config1, spec, cols, plan)
             np.testing.assert_array_equal(got, want, err_msg=f"{spec.name} plan {plan}")

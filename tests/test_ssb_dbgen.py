@@ -197,10 +197,10 @@ def test_dbgen_flights_on_gpu(dev, ssb_host):
 def test_demo_driver_report(ssb_host, tmp_path_factory):
     """The C++ driver (merle_amd/lib/mbx_demo, the reference's ssb_demoall
     on the mybmpidx mirror) runs every flight's Sparse / Medium / Dense
-    programs and the synthetic suite (perfect / many-OR / candidate-check
-    recipes on seeded Zipf columns), checks each plan against the scan plan
-    (exit status 0) and prints the five-block report with three SSB rows per
-    flight and one synthetic row per case and selectivity."""
+    programs, method 1 (no compaction) against method 2 and the synthetic
+    suite (perfect / many-OR / candidate-check recipes on seeded Zipf
+    columns), checks each plan against the scan plan or the other method
+    (exit status 0) and prints the five-block report."""
     demo = os.path.join(ROOT, "merle_amd", "lib", "mbx_demo")
     if not os.path.exists(demo):
         pytest.skip("mbx_demo not built")
@@ -210,6 +210,13 @@ def test_demo_driver_report(ssb_host, tmp_path_factory):
     assert r.returncode == 0, r.stderr
     blocks = r.stdout.strip("\n").split("\n\n")
     assert len(blocks) == 5
+    # method 1 / method 2 (synth/cpuref.c:194-205): 7 selectivities
+    meth = blocks[3].splitlines()
+    assert meth[0].split("\t") == ["Seletiv", "Method1", "Method2"]
+    assert len(meth) == 1 + 7
+    for row in meth[1:]:
+        f = row.split("\t")
+        assert len(f) == 3 and float(f[1]) > 0 and float(f[2]) > 0
     # the synthetic block (synth/cpuref.c:207-256): 10 cases x 10 selectivities
     syn = blocks[4].splitlines()
     assert syn[0].split("\t")[:4] == ["Skew", "BitW", "Seletiv", "Join"]
