@@ -231,3 +231,35 @@ def test_multipass_launch_shapes(dev):
     for c in sch.cols.values():
         c.dev.free()
         c.host = None
+
+
+def test_accumulate_and_group_count(dev):
+    """MBX_ACCUMULATE adds into the output instead of zeroing it (a second
+    run doubles every sum), for the LDS table and the HBM atomics; a query
+    with no groups is refused with a status."""
+    n = 200_003
+    rng = np.random.default_rng(5)
+    schema = gpuq.upload_schema(dev, datagen.small_schema(n, 5))
+    for extra in (0, 5000):   # LDS table, then HBM atomics
+        spec = datagen.random_spec(rng, schema, extra_groups=extra)
+        want, _ = cpuref.query(spec, schema)
+        cols, allocs = gpuq.plan_cols(dev, schema, spec, rng)
+        prg = S.to_vprg(cols, n, spec.instrs)
+        op = S.to_rowop(spec, schema)
+        out = dev.alloc(8 * spec.nr_grp, np.int64, spec.nr_grp)
+        for plan in (abi.PLAN_FUSED, abi.PLAN_FUSED | abi.PLAN_EAGER,
+                     abi.PLAN_MATERIALISE, abi.PLAN_SCAN):
+            check(dev.lib.mbx_prog_grpby(dev.ctx, C.byref(prg), C.byref(op),
+                                         spec.nr_grp, out.ptr, plan))
+            check(dev.lib.mbx_prog_grpby(dev.ctx, C.byref(prg), C.byref(op),
+                                         spec.nr_grp, out.ptr,
+                                         plan | abi.ACCUMULATE))
+            dev.sync()
+            np.testing.assert_array_equal(out.host(), 2 * want,
+                                          err_msg=f"plan {plan} groups {spec.nr_grp}")
+        assert dev.lib.mbx_prog_grpby(dev.ctx, C.byref(prg), C.byref(op), 0,
+                                      out.ptr, abi.PLAN_FUSED) == abi.ERR_INVALID
+        out.free()
+        gpuq.free_all(dev, allocs)
+    for c in schema.cols.values():
+        c.dev.free()
