@@ -6,7 +6,7 @@ column), seeded random bitmap programs, their candidate plans timed on the
 GPU, and a 16 -> 64 -> 1 fp32 MLP (the library's HIP kernels) trained on the
 log runtimes.  The chooser scores every candidate plan of a query and runs the
 cheapest; the static heuristic (sparsest column first, fused) is the
-fallback and the judge: every plan returns identical sums.
+fallback and the yardstick: every plan returns identical sums.
 """
 import ctypes as C
 import math
