@@ -329,8 +329,9 @@ static void fused_even(mbx_ctx *ctx, const QArgs &a, size_t lds) {
 // the table in HBM, 2 words in even passes, which spreads the HBM atomics
 // over the whole kernel (9 % faster than 4 words on 437,500 groups).
 // Three-valued (candidate-check) conjunctions follow the same rule: on 60
-// config-4 programs, 4 words on the resident grid took 2.75 ms, 2 words in
-// even passes 2.87-2.89, 2 words on the resident grid 3.04, 1 word 3.51.
+// random programs over a 120M-row, 8-column Zipf table, 4 words on the
+// resident grid took 2.75 ms, 2 words in even passes 2.87-2.89, 2 words on
+// the resident grid 3.04, 1 word 3.51.
 template <bool C, bool L>
 static void fused_conj(mbx_ctx *ctx, const QArgs &a, size_t lds) {
   if (fused_try<C, L, false, 3, 2>(ctx, a, lds, false)) return;
