@@ -22,6 +22,7 @@
 // instead of one ballot per bin.  Any other bin set takes the generic path:
 // one ballot per bin, as the reference does.
 #include <cstring>
+#include <type_traits>
 
 #include "devfuncs.hpp"
 
@@ -49,6 +50,7 @@ struct BinArgs {
   uint32_t pad;
   const uint64_t *mins;  // device, nb (kGeneric) / bounds nb+1 (kSorted)
   const uint64_t *maxs;  // device, nb (kGeneric)
+  const void *tab;       // device, dimsz+1 bin indexes (foreign-key radix path)
   uint32_t *out;
   size_t stride_words;
 };
@@ -85,6 +87,30 @@ __device__ __forceinline__ uint32_t bin_index(const BinArgs &a, uint64_t v,
     if (np <= a.nb + 1 && s_bounds[np - 1] <= v) pos = np;
   }
   return (pos >= 1 && pos <= a.nb) ? pos - 1 : kNone;
+}
+
+// Bin index of each dimension row, for a foreign-key column on the radix
+// path: entry k < dimsz holds row k's bin, entry dimsz the bin of INT64_MAX
+// (what a key past the dimension reads as), ~0 for no bin.  The fact pass
+// then reads one 1- or 2-byte entry per row: the table of a 1.4M-row
+// dimension (1.4 or 2.8 MB) stays in an XCD's 4 MB L2, where its 4-byte
+// attribute column (5.6 MB) did not.
+template <bool SORTED, typename T>
+__global__ __launch_bounds__(256) void dim_index_kernel(BinArgs a, T *tab) {
+  __shared__ uint64_t s_bounds[SORTED ? kMaxSortedBounds : 1];
+  if (SORTED) {
+    for (uint32_t i = threadIdx.x; i <= a.nb; i += blockDim.x)
+      s_bounds[i] = a.mins[i];
+    __syncthreads();
+  }
+  const size_t n = (size_t)a.src.dimsz + 1;
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (size_t)gridDim.x * blockDim.x) {
+    const uint64_t v = k < a.src.dimsz ? ld_elem(a.src.data, a.src.eb, k, true)
+                                       : (uint64_t)INT64_MAX;
+    const uint32_t i = bin_index(a, v, s_bounds);
+    tab[k] = i == kNone ? (T)~(T)0 : (T)i;
+  }
 }
 
 typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
@@ -139,8 +165,11 @@ __device__ __forceinline__ void store_line(uint32_t *dst, const uint64_t *w) {
 extern __shared__ u64x2 s_tr_dyn[];
 constexpr size_t kTransposeLds = 4 * 64 * 8 * sizeof(u64x2);
 
-template <bool RADIX, bool SORTED, bool TR>
+// TAB: 0 = values through bin_value; 1 / 2 = bin indexes from a.tab of
+// uint8 / uint16 entries, looked up by foreign key (radix path only).
+template <bool RADIX, bool SORTED, bool TR, int TAB = 0>
 __global__ __launch_bounds__(256) void bins_kernel(BinArgs a) {
+  static_assert(TAB == 0 || (RADIX && !SORTED), "table path is radix-only");
   __shared__ uint64_t s_bounds[SORTED ? kMaxSortedBounds : 1];
   if (SORTED) {
     for (uint32_t i = threadIdx.x; i <= a.nb; i += blockDim.x)
@@ -156,20 +185,35 @@ __global__ __launch_bounds__(256) void bins_kernel(BinArgs a) {
     const size_t row0 = (size_t)blk * 1024;
     uint64_t v[16];
     uint32_t ok = 0;
-#pragma unroll
-    for (int g = 0; g < 16; ++g) {
-      const size_t row = row0 + 64 * g + lane;
-      v[g] = 0;
-      if (row < a.nrows) {
-        v[g] = bin_value(a.src, row);
-        ok |= 1u << g;
-      }
-    }
     uint32_t idx[16];
-    if (RADIX) {
+    if constexpr (TAB != 0) {
+      using T = typename std::conditional<TAB == 1, uint8_t, uint16_t>::type;
+      const T *tab = static_cast<const T *>(a.tab);
 #pragma unroll
-      for (int g = 0; g < 16; ++g)
-        idx[g] = (ok >> g & 1u) ? bin_index(a, v[g], s_bounds) : kNone;
+      for (int g = 0; g < 16; ++g) {
+        const size_t row = row0 + 64 * g + lane;
+        idx[g] = kNone;
+        if (row < a.nrows) {
+          const uint32_t k = min(ld_stream(a.src.fk + row), a.src.dimsz);
+          const uint32_t t = tab[k];
+          if (t != (uint32_t)(T)~(T)0) idx[g] = t;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int g = 0; g < 16; ++g) {
+        const size_t row = row0 + 64 * g + lane;
+        v[g] = 0;
+        if (row < a.nrows) {
+          v[g] = bin_value(a.src, row);
+          ok |= 1u << g;
+        }
+      }
+      if (RADIX) {
+#pragma unroll
+        for (int g = 0; g < 16; ++g)
+          idx[g] = (ok >> g & 1u) ? bin_index(a, v[g], s_bounds) : kNone;
+      }
     }
     for (uint32_t c = 0; c < nchunks; ++c) {
       const uint32_t cbase = c * 64;
@@ -292,6 +336,37 @@ static int launch_bins(mbx_ctx *ctx, const mbx_colref *src, uint32_t nrows,
       return hip_fail(e, "create_bin: bounds upload");
     }
   }
+  // Foreign-key radix path: a bin-index table over the dimension first.
+  int tab = 0;
+  void *d_tab = nullptr;
+  if (a.src.fk && a.mode != kGeneric && nb < 0xffff &&
+      a.src.dimsz < 0x80000000u) {
+    tab = nb < 0xff ? 1 : 2;
+    st = ctx->pool->alloc(((size_t)a.src.dimsz + 1) * tab, &d_tab);
+    if (st) {
+      if (tmp) ctx->pool->free(tmp);
+      return st;
+    }
+    a.tab = d_tab;
+    size_t tg = ceil_div((size_t)a.src.dimsz + 1, 256);
+    if (tg > (size_t)ctx->num_cus * 8) tg = (size_t)ctx->num_cus * 8;
+    const dim3 g((unsigned)tg), b(256);
+    if (a.mode == kSorted) {
+      if (tab == 1)
+        hipLaunchKernelGGL((dim_index_kernel<true, uint8_t>), g, b, 0, ctx->stream, a,
+                           static_cast<uint8_t *>(d_tab));
+      else
+        hipLaunchKernelGGL((dim_index_kernel<true, uint16_t>), g, b, 0, ctx->stream, a,
+                           static_cast<uint16_t *>(d_tab));
+    } else {
+      if (tab == 1)
+        hipLaunchKernelGGL((dim_index_kernel<false, uint8_t>), g, b, 0, ctx->stream, a,
+                           static_cast<uint8_t *>(d_tab));
+      else
+        hipLaunchKernelGGL((dim_index_kernel<false, uint16_t>), g, b, 0, ctx->stream, a,
+                           static_cast<uint16_t *>(d_tab));
+    }
+  }
   size_t wgs = ceil_div(a.nblocks, 4);
   const size_t cap = (size_t)ctx->num_cus * 16;
   if (wgs > cap) wgs = cap;
@@ -299,7 +374,13 @@ static int launch_bins(mbx_ctx *ctx, const mbx_colref *src, uint32_t nrows,
   const bool tr = nb > 8;
   const size_t lds = tr ? kTransposeLds : 0;
   const dim3 g((unsigned)wgs), b(256);
-  if (a.mode == kGeneric) {
+  if (tab == 1) {
+    if (tr) hipLaunchKernelGGL((bins_kernel<true, false, true, 1>), g, b, lds, ctx->stream, a);
+    else hipLaunchKernelGGL((bins_kernel<true, false, false, 1>), g, b, lds, ctx->stream, a);
+  } else if (tab == 2) {
+    if (tr) hipLaunchKernelGGL((bins_kernel<true, false, true, 2>), g, b, lds, ctx->stream, a);
+    else hipLaunchKernelGGL((bins_kernel<true, false, false, 2>), g, b, lds, ctx->stream, a);
+  } else if (a.mode == kGeneric) {
     if (tr) hipLaunchKernelGGL((bins_kernel<false, false, true>), g, b, lds, ctx->stream, a);
     else hipLaunchKernelGGL((bins_kernel<false, false, false>), g, b, lds, ctx->stream, a);
   } else if (a.mode == kSorted) {
@@ -310,10 +391,11 @@ static int launch_bins(mbx_ctx *ctx, const mbx_colref *src, uint32_t nrows,
     else hipLaunchKernelGGL((bins_kernel<true, false, false>), g, b, lds, ctx->stream, a);
   }
   hipError_t e = hipGetLastError();
-  if (tmp) {
-    // the bounds must outlive the kernel: wait before returning them
+  if (tmp || d_tab) {
+    // the bounds and the table must outlive the kernels: wait first
     hipError_t e2 = hipStreamSynchronize(ctx->stream);
-    ctx->pool->free(tmp);
+    if (tmp) ctx->pool->free(tmp);
+    if (d_tab) ctx->pool->free(d_tab);
     if (e == hipSuccess) e = e2;
   }
   if (e != hipSuccess) return hip_fail(e, "create_bin: launch");

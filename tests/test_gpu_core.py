@@ -79,6 +79,42 @@ def test_bins_fk_projection(dev, n):
     np.testing.assert_array_equal(got, cpuref.create_join(fk, 100, 900))
 
 
+@pytest.mark.parametrize("n", [1, 4099, 300001])
+@pytest.mark.parametrize("dt", [np.uint16, np.uint32, np.uint64])
+def test_bins_fk_radix_table(dev, n, dt):
+    """FK projection on the radix path, through the dimension's bin-index
+    table (uint8 entries below 255 bins, uint16 from 255): width-1,
+    equal-width and sorted bins, 5 to 400 bins, keys past the dimension
+    (INT64_MAX: caught by a last bin reaching 2^64-1, else by none)."""
+    rng = np.random.default_rng(n * 7 + np.dtype(dt).itemsize)
+    dimsz = 3000
+    dim = rng.integers(0, 2000, dimsz).astype(dt)
+    fk = rng.integers(0, 3300, n).astype(np.uint32)
+    dd, df = dev.upload(dim), dev.upload(fk)
+    ref = abi.ColRef(dd.ptr, df.ptr, dimsz, np.dtype(dt).itemsize)
+    top = (1 << 64) - 1
+    uneven = [int(x) for x in np.unique(rng.integers(0, 2000, 500))]
+    sets = [(list(range(100, 105)), list(range(101, 106))),        # unit, 5
+            (list(range(0, 254)), list(range(1, 255))),           # unit, 254
+            (list(range(0, 255)), list(range(1, 256))),           # unit, 255
+            (list(range(0, 300)), list(range(1, 301))),           # unit, 300
+            (list(range(0, 2000, 50)), list(range(50, 2050, 50))),  # equal, 40
+            (uneven[:30], uneven[1:30] + [top]),                  # sorted, to 2^64-1
+            (uneven[:400], uneven[1:401])]                        # sorted, 400
+    for mins, maxs in sets:
+        nb = len(mins)
+        outs = (C.c_void_p * nb)()
+        check(dev.lib.mbx_create_bin(dev.ctx, C.byref(ref), n,
+                                     (C.c_uint64 * nb)(*mins),
+                                     (C.c_uint64 * nb)(*maxs), nb, outs, None))
+        got = gpuq.bitmaps_host(dev, list(outs), n)
+        np.testing.assert_array_equal(
+            got, cpuref.create_bin(dim, mins, maxs, fk, dimsz), err_msg=f"{nb} bins")
+        check(dev.lib.mbx_free(dev.ctx, outs[0]))
+    dd.free()
+    df.free()
+
+
 @pytest.mark.parametrize("seed", range(16))
 def test_random_queries_all_plans(dev, seed):
     """Random programs (AND/OR/NOT/ANDM/ORM, exact and edge bins), group
