@@ -47,7 +47,7 @@ struct BinArgs {
   uint64_t b0, width;    // kUnit / kEqual
   double inv_width;      // kEqual
   uint32_t search_top;   // kSorted: largest power of two <= nb+1
-  uint32_t pad;
+  uint32_t tab16;        // table bytes / 16 (table copied into LDS)
   const uint64_t *mins;  // device, nb (kGeneric) / bounds nb+1 (kSorted)
   const uint64_t *maxs;  // device, nb (kGeneric)
   const void *tab;       // device, dimsz+1 bin indexes (foreign-key radix path)
@@ -140,7 +140,10 @@ __device__ __forceinline__ void store_lines(const BinArgs &a, u64x2 *tr,
       const u64x2 x = tr[line * 8 + (part ^ (line & 7u))];
       u64x2 *dst = reinterpret_cast<u64x2 *>(
           a.out + (size_t)(cbase + line) * a.stride_words + (size_t)blk * 32);
-      dst[part] = x;
+      // streaming store: the lines are not read back soon, and under
+      // normal stores their 15 GB (1,000 bins) evicted the foreign-key bin
+      // table from L2 (2.6 GB of extra HBM fetch)
+      __builtin_nontemporal_store(x, dst + part);
     }
   }
   wave_lds_sync();   // the next chunk rewrites tr
@@ -165,15 +168,27 @@ __device__ __forceinline__ void store_line(uint32_t *dst, const uint64_t *w) {
 extern __shared__ u64x2 s_tr_dyn[];
 constexpr size_t kTransposeLds = 4 * 64 * 8 * sizeof(u64x2);
 
+// Largest bin-index table copied into each workgroup's LDS (a 40,000-row
+// dimension's uint8 table): random byte lookups there cost no L2 request.
+constexpr size_t kLdsTableMax = 64 * 1024;
+
 // TAB: 0 = values through bin_value; 1 / 2 = bin indexes from a.tab of
 // uint8 / uint16 entries, looked up by foreign key (radix path only).
-template <bool RADIX, bool SORTED, bool TR, int TAB = 0>
+// LT: the table is first copied into dynamic LDS, after the transpose buffer.
+template <bool RADIX, bool SORTED, bool TR, int TAB = 0, bool LT = false>
 __global__ __launch_bounds__(256) void bins_kernel(BinArgs a) {
   static_assert(TAB == 0 || (RADIX && !SORTED), "table path is radix-only");
+  static_assert(!LT || (TAB != 0 && !TR), "LDS table: table path, no transpose");
   __shared__ uint64_t s_bounds[SORTED ? kMaxSortedBounds : 1];
   if (SORTED) {
     for (uint32_t i = threadIdx.x; i <= a.nb; i += blockDim.x)
       s_bounds[i] = a.mins[i];
+    __syncthreads();
+  }
+  u64x2 *s_tab = s_tr_dyn + (TR ? kTransposeLds / sizeof(u64x2) : 0);
+  if constexpr (LT) {
+    const u64x2 *src = static_cast<const u64x2 *>(a.tab);
+    for (uint32_t i = threadIdx.x; i < a.tab16; i += blockDim.x) s_tab[i] = src[i];
     __syncthreads();
   }
   const uint32_t lane = lane_id();
@@ -188,7 +203,8 @@ __global__ __launch_bounds__(256) void bins_kernel(BinArgs a) {
     uint32_t idx[16];
     if constexpr (TAB != 0) {
       using T = typename std::conditional<TAB == 1, uint8_t, uint16_t>::type;
-      const T *tab = static_cast<const T *>(a.tab);
+      const T *tab = LT ? reinterpret_cast<const T *>(s_tab)
+                        : static_cast<const T *>(a.tab);
 #pragma unroll
       for (int g = 0; g < 16; ++g) {
         const size_t row = row0 + 64 * g + lane;
@@ -342,7 +358,9 @@ static int launch_bins(mbx_ctx *ctx, const mbx_colref *src, uint32_t nrows,
   if (a.src.fk && a.mode != kGeneric && nb < 0xffff &&
       a.src.dimsz < 0x80000000u) {
     tab = nb < 0xff ? 1 : 2;
-    st = ctx->pool->alloc(((size_t)a.src.dimsz + 1) * tab, &d_tab);
+    const size_t tab_bytes = ((((size_t)a.src.dimsz + 1) * tab) + 15) & ~(size_t)15;
+    a.tab16 = (uint32_t)(tab_bytes / 16);
+    st = ctx->pool->alloc(tab_bytes, &d_tab);
     if (st) {
       if (tmp) ctx->pool->free(tmp);
       return st;
@@ -367,14 +385,32 @@ static int launch_bins(mbx_ctx *ctx, const mbx_colref *src, uint32_t nrows,
                            static_cast<uint16_t *>(d_tab));
     }
   }
+  const bool tr = nb > 8;
+  size_t lds = tr ? kTransposeLds : 0;
+  size_t cap = (size_t)ctx->num_cus * 16;
+  // A small table goes into LDS when the workgroup's LDS stays within the
+  // device's per-block limit; the grid then holds only the resident
+  // workgroups, so each copies the table once.  Not beside the transpose
+  // buffer: at 72 KB per workgroup the write-bound TR kernels lose half
+  // their waves (40,000-row dimension, 250 bins: 1.43 -> 1.97 ms).
+  const bool lt = tab && !tr && (size_t)a.tab16 * 16 <= kLdsTableMax &&
+                  (size_t)a.tab16 * 16 <= ctx->lds_per_block;
+  if (lt) {
+    lds = (size_t)a.tab16 * 16;
+    size_t per_cu = ctx->lds_per_cu / lds;
+    if (per_cu > 8) per_cu = 8;
+    if (per_cu == 0) per_cu = 1;
+    cap = (size_t)ctx->num_cus * per_cu;
+  }
   size_t wgs = ceil_div(a.nblocks, 4);
-  const size_t cap = (size_t)ctx->num_cus * 16;
   if (wgs > cap) wgs = cap;
   if (wgs == 0) wgs = 1;
-  const bool tr = nb > 8;
-  const size_t lds = tr ? kTransposeLds : 0;
   const dim3 g((unsigned)wgs), b(256);
-  if (tab == 1) {
+  if (lt && tab == 1) {
+    hipLaunchKernelGGL((bins_kernel<true, false, false, 1, true>), g, b, lds, ctx->stream, a);
+  } else if (lt) {
+    hipLaunchKernelGGL((bins_kernel<true, false, false, 2, true>), g, b, lds, ctx->stream, a);
+  } else if (tab == 1) {
     if (tr) hipLaunchKernelGGL((bins_kernel<true, false, true, 1>), g, b, lds, ctx->stream, a);
     else hipLaunchKernelGGL((bins_kernel<true, false, false, 1>), g, b, lds, ctx->stream, a);
   } else if (tab == 2) {

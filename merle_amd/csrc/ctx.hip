@@ -88,6 +88,7 @@ int mbx_init(int device, size_t pool_bytes, mbx_ctx **out) {
     c->lds_per_cu = prop.maxSharedMemoryPerMultiProcessor
                         ? prop.maxSharedMemoryPerMultiProcessor
                         : 160 * 1024;
+    if (prop.sharedMemPerBlock) c->lds_per_block = prop.sharedMemPerBlock;
   }
   e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
   if (e != hipSuccess) {

@@ -73,6 +73,7 @@ struct mbx_ctx {
   hipStream_t stream = nullptr;
   int num_cus = 256;
   size_t lds_per_cu = 160 * 1024;
+  size_t lds_per_block = 64 * 1024;
   mbx::Pool *pool = nullptr;
   // scratch reused by the plans (one query at a time per context)
   uint32_t *d_counter = nullptr;   // 4 uint32 work/counter slots

@@ -81,15 +81,17 @@ def test_bins_fk_projection(dev, n):
 
 @pytest.mark.parametrize("n", [1, 4099, 300001])
 @pytest.mark.parametrize("dt", [np.uint16, np.uint32, np.uint64])
-def test_bins_fk_radix_table(dev, n, dt):
+@pytest.mark.parametrize("dimsz", [3000, 40000, 70000])
+def test_bins_fk_radix_table(dev, n, dt, dimsz):
     """FK projection on the radix path, through the dimension's bin-index
-    table (uint8 entries below 255 bins, uint16 from 255): width-1,
-    equal-width and sorted bins, 5 to 400 bins, keys past the dimension
-    (INT64_MAX: caught by a last bin reaching 2^64-1, else by none)."""
-    rng = np.random.default_rng(n * 7 + np.dtype(dt).itemsize)
-    dimsz = 3000
+    table (uint8 entries below 255 bins, uint16 from 255), read from LDS
+    (tables up to 64 KB: 3,000 rows, and 40,000 rows of uint8) or from L2:
+    width-1, equal-width and sorted bins, 5 to 400 bins, keys past the
+    dimension (INT64_MAX: caught by a last bin reaching 2^64-1, else by
+    none)."""
+    rng = np.random.default_rng(n * 7 + np.dtype(dt).itemsize + dimsz)
     dim = rng.integers(0, 2000, dimsz).astype(dt)
-    fk = rng.integers(0, 3300, n).astype(np.uint32)
+    fk = rng.integers(0, dimsz + dimsz // 10, n).astype(np.uint32)
     dd, df = dev.upload(dim), dev.upload(fk)
     ref = abi.ColRef(dd.ptr, df.ptr, dimsz, np.dtype(dt).itemsize)
     top = (1 << 64) - 1
