@@ -352,19 +352,19 @@ static int launch_bins(mbx_ctx *ctx, const mbx_colref *src, uint32_t nrows,
       return hip_fail(e, "create_bin: bounds upload");
     }
   }
-  // Foreign-key radix path: a bin-index table over the dimension first.
+  // Foreign-key radix path: a bin-index table over the dimension first
+  // (without room for it in the pool, the values are read directly).
   int tab = 0;
   void *d_tab = nullptr;
   if (a.src.fk && a.mode != kGeneric && nb < 0xffff &&
       a.src.dimsz < 0x80000000u) {
-    tab = nb < 0xff ? 1 : 2;
-    const size_t tab_bytes = ((((size_t)a.src.dimsz + 1) * tab) + 15) & ~(size_t)15;
+    const int eb = nb < 0xff ? 1 : 2;
+    const size_t tab_bytes = ((((size_t)a.src.dimsz + 1) * eb) + 15) & ~(size_t)15;
+    if (ctx->pool->alloc(tab_bytes, &d_tab) != MBX_OK) d_tab = nullptr;
+    else tab = eb;
     a.tab16 = (uint32_t)(tab_bytes / 16);
-    st = ctx->pool->alloc(tab_bytes, &d_tab);
-    if (st) {
-      if (tmp) ctx->pool->free(tmp);
-      return st;
-    }
+  }
+  if (tab) {
     a.tab = d_tab;
     size_t tg = ceil_div((size_t)a.src.dimsz + 1, 256);
     if (tg > (size_t)ctx->num_cus * 8) tg = (size_t)ctx->num_cus * 8;

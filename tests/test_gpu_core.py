@@ -117,6 +117,38 @@ def test_bins_fk_radix_table(dev, n, dt, dimsz):
     df.free()
 
 
+def test_bins_fk_table_pool_full(dev):
+    """A context whose fixed pool has no room for the dimension's bin table
+    (2.5 MiB pool, 2.1 MiB of it context scratch; 1,000,000-row dimension)
+    reads the dimension values directly and gives the same bitmaps."""
+    rng = np.random.default_rng(17)
+    n, dimsz = 300001, 1_000_000
+    dim = rng.integers(0, 64, dimsz).astype(np.uint32)
+    fk = rng.integers(0, dimsz + 5000, n).astype(np.uint32)
+    dd, df = dev.upload(dim), dev.upload(fk)
+    ctx = C.c_void_p()
+    check(dev.lib.mbx_init(0, 5 << 19, C.byref(ctx)))
+    try:
+        for nb in (5, 64):
+            mins, maxs = list(range(nb)), list(range(1, nb + 1))
+            sw = stride_words(n)
+            slab = dev.alloc(sw * 4 * nb)
+            ref = abi.ColRef(dd.ptr, df.ptr, dimsz, 4)
+            check(dev.lib.mbx_create_bin_into(ctx, C.byref(ref), n,
+                                              (C.c_uint64 * nb)(*mins),
+                                              (C.c_uint64 * nb)(*maxs), nb,
+                                              slab.ptr, sw))
+            check(dev.lib.mbx_sync(ctx))
+            got = gpuq.bitmaps_host(dev, [slab.ptr + j * sw * 4 for j in range(nb)], n)
+            np.testing.assert_array_equal(
+                got, cpuref.create_bin(dim, mins, maxs, fk, dimsz), err_msg=f"{nb} bins")
+            slab.free()
+    finally:
+        dev.lib.mbx_destroy(ctx)
+    dd.free()
+    df.free()
+
+
 @pytest.mark.parametrize("seed", range(16))
 def test_random_queries_all_plans(dev, seed):
     """Random programs (AND/OR/NOT/ANDM/ORM, exact and edge bins), group
