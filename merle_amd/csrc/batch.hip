@@ -10,8 +10,12 @@
 // drains its own list through its own counter.  For each item a workgroup
 // zeroes its LDS group table, runs the same wave pipeline as the
 // single-query kernel (program -> compaction -> row op) over the range, and
-// flushes the table to the query's int64 output.  Output zeroing, counter
-// reset and the launch are captured once into a hipGraph (fill kernels only:
+// flushes the table to the query's int64 output.  Queries come in two
+// classes, each drained by its own persistent kernel: eager conjunctions
+// with a small LDS table (most queries), whose kernel holds only that
+// evaluation form and sink (93 VGPRs: 5 waves per SIMD), and the rest (every
+// form, LDS or HBM sink: 120 VGPRs, 4 waves).  Output zeroing, counter reset
+// and the two launches are captured once into a hipGraph (fill kernels only:
 // no memset nodes) and replayed per run.
 #include <algorithm>
 #include <cstring>
@@ -38,16 +42,19 @@ __device__ __forceinline__ void batch_item(const BatchQ &q, uint32_t w_begin,
   // the item's 64-word chunks, interleaved over the workgroup's four waves
   // (three-valued programs take half as many per pass: twice the registers)
   constexpr int W = CHK ? kBWPL / 2 : kBWPL;
-  const uint32_t wv = threadIdx.x >> 6;
+  const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   for (uint32_t w0 = w_begin + wv * 64; w0 < w_end; w0 += 4 * 64 * W)
     run_wave_range<CHK, false, W, EAGER>(q.prg, q.op, q.nr_grp, w0, 4 * 64,
                                          w_end, sink, s_list, nullptr);
 }
 
+// CONJ: the class-0 kernel (eager conjunctions, LDS table); else every form.
+template <bool CONJ>
 __global__ __launch_bounds__(256) void batch_kernel(BatchArgs a) {
   __shared__ uint32_t s_list[4][kWaveList];
   __shared__ uint32_t s_item;
-  const uint32_t wv = threadIdx.x >> 6;
+  // uniform, so the row-list addresses stay scalar (4 VGPRs fewer)
+  const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint32_t x = blockIdx.x % a.nx;
   // this XCD's row ranges x, x + nx, ...: nq items each
   const uint32_t nranges = a.max_items > x ? (a.max_items - x + a.nx - 1) / a.nx : 0;
@@ -65,9 +72,9 @@ __global__ __launch_bounds__(256) void batch_kernel(BatchArgs a) {
       continue;
     }
     GroupSink sink;
-    sink.lds = q.lds != 0;
+    sink.lds = CONJ || q.lds != 0;
     sink.out = q.out;
-    if (q.lds)
+    if (CONJ || q.lds)
       for (uint32_t i = threadIdx.x; i < q.nr_grp; i += 256) s_grp_dyn[i] = 0;
     __syncthreads();
     const uint32_t w_begin = r * q.words_per_item;
@@ -76,7 +83,10 @@ __global__ __launch_bounds__(256) void batch_kernel(BatchArgs a) {
     // three-valued) and for two-valued programs with small operand lists
     // (straight into registers); the rest take the early-out form (same
     // bits, fewer registers for the whole kernel)
-    if (q.chk) {
+    if (CONJ) {
+      if (q.chk) batch_item<true, 3>(q, w_begin, w_end, sink, s_list[wv]);
+      else batch_item<false, 3>(q, w_begin, w_end, sink, s_list[wv]);
+    } else if (q.chk) {
       if (q.eager && q.prg.conj)
         batch_item<true, 3>(q, w_begin, w_end, sink, s_list[wv]);
       else
@@ -88,7 +98,7 @@ __global__ __launch_bounds__(256) void batch_kernel(BatchArgs a) {
     } else {
       batch_item<false, 0>(q, w_begin, w_end, sink, s_list[wv]);
     }
-    if (q.lds) {
+    if (CONJ || q.lds) {
       __syncthreads();
       for (uint32_t i = threadIdx.x; i < q.nr_grp; i += 256) {
         const unsigned long long v = s_grp_dyn[i];
@@ -114,9 +124,8 @@ using namespace mbx;
 struct mbx_batch {
   BatchQ *d_qs = nullptr;     // pool memory: descriptors then the counters
   uint32_t *d_counter = nullptr;
-  uint32_t nq = 0, total_items = 0, max_items = 0;
-  size_t lds_bytes = 0;
-  unsigned grid = 0;
+  uint32_t nq = 0;
+  mbx::BatchShape shape{};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   mbx_tune tune{};
@@ -124,40 +133,50 @@ struct mbx_batch {
 
 namespace mbx {
 
-int batch_launch(hipStream_t s, const BatchQ *d_qs, uint32_t nq,
-                 uint32_t max_items, uint32_t *d_counter, size_t lds_bytes,
-                 unsigned grid) {
-  hipLaunchKernelGGL(zero_outputs_kernel, dim3(std::min(nq, 1024u)),
-                     dim3(256), 0, s, d_qs, nq);
-  MBX_HIP(hipGetLastError());
-  int st = launch_fill_u32(s, d_counter, 0, kXcdLists * kXcdCounterStride);
-  if (st) return st;
-  BatchArgs a;
-  a.qs = d_qs;
-  a.nq = nq;
-  a.max_items = max_items;
-  a.nx = grid < kXcdLists ? grid : kXcdLists;
-  a.counter = d_counter;
-  hipLaunchKernelGGL(batch_kernel, dim3(grid), dim3(256), lds_bytes, s, a);
-  MBX_HIP(hipGetLastError());
-  return MBX_OK;
-}
-
-unsigned batch_grid(mbx_ctx *ctx, const mbx_tune &t, uint32_t total_items) {
+static unsigned batch_grid(mbx_ctx *ctx, const mbx_tune &t,
+                           uint32_t total_items) {
   uint64_t g = std::min<uint64_t>((uint64_t)ctx->num_cus * t.wg_per_cu, total_items);
   if (g > kXcdLists) g -= g % kXcdLists;   // whole rounds over the XCDs
   return (unsigned)std::max<uint64_t>(1, g);
 }
 
+int batch_launch(mbx_ctx *ctx, hipStream_t s, const BatchQ *d_qs,
+                 const BatchShape &sh, uint32_t *d_counter, const mbx_tune &t) {
+  const uint32_t nq = sh.nq[0] + sh.nq[1];
+  hipLaunchKernelGGL(zero_outputs_kernel, dim3(std::min(nq, 1024u)),
+                     dim3(256), 0, s, d_qs, nq);
+  MBX_HIP(hipGetLastError());
+  int st = launch_fill_u32(s, d_counter, 0, kCounterBytes / 4);
+  if (st) return st;
+  for (int c = 0; c < 2; ++c) {
+    if (sh.nq[c] == 0) continue;
+    const unsigned grid = batch_grid(ctx, t, sh.items[c]);
+    BatchArgs a;
+    a.qs = d_qs + (c ? sh.nq[0] : 0);
+    a.nq = sh.nq[c];
+    a.max_items = sh.max_items[c];
+    a.nx = grid < kXcdLists ? grid : kXcdLists;
+    a.counter = d_counter + c * kXcdLists * kXcdCounterStride;
+    if (c == 0)
+      hipLaunchKernelGGL(batch_kernel<true>, dim3(grid), dim3(256),
+                         sh.lds_bytes[0], s, a);
+    else
+      hipLaunchKernelGGL(batch_kernel<false>, dim3(grid), dim3(256),
+                         sh.lds_bytes[1], s, a);
+    MBX_HIP(hipGetLastError());
+  }
+  return MBX_OK;
+}
+
 static int enqueue_batch(mbx_ctx *ctx, mbx_batch *b) {
-  return batch_launch(ctx->stream, b->d_qs, b->nq, b->max_items,
-                      b->d_counter, b->lds_bytes, b->grid);
+  return batch_launch(ctx, ctx->stream, b->d_qs, b->shape, b->d_counter,
+                      b->tune);
 }
 
 int batch_prepare(mbx_ctx *ctx, const mbx_query *qs, uint32_t nq,
-                  const mbx_tune &t, BatchQ *h, uint32_t *total_items,
-                  size_t *lds_bytes, uint32_t *max_items) {
-  uint32_t items = 0, max_lds_grp = 0, maxi = 0;
+                  const mbx_tune &t, BatchQ *h, BatchShape *sh) {
+  std::memset(sh, 0, sizeof(*sh));
+  std::vector<BatchQ> cls[2];
   for (uint32_t i = 0; i < nq; ++i) {
     QueryPlan p;
     int st = prepare(ctx, &qs[i].prg, &qs[i].op, qs[i].nr_grp, qs[i].d_out, &p);
@@ -165,25 +184,26 @@ int batch_prepare(mbx_ctx *ctx, const mbx_query *qs, uint32_t nq,
       set_error("batch query " + std::to_string(i) + ": " + mbx_last_error());
       return st;
     }
-    BatchQ &b = h[i];
+    BatchQ b;
     std::memset(&b, 0, sizeof(b));
     b.prg = p.prg;
     b.op = p.op;
     b.out = p.out;
     b.nr_grp = p.nr_grp;
     b.lds = p.nr_grp <= t.lds_groups ? 1u : 0u;
-    if (b.lds) max_lds_grp = std::max(max_lds_grp, p.nr_grp);
     b.chk = p.prg.chk;
     b.eager = (qs[i].flags & MBX_PLAN_EAGER) ? 1u : 0u;
     b.words_per_item = t.tile_words;
     b.nitems = (uint32_t)ceil_div(p.prg.nwords, b.words_per_item);
-    b.item0 = items;
-    items += b.nitems;
-    maxi = std::max(maxi, b.nitems);
+    const int c = (b.eager && p.prg.conj && b.lds && p.nr_grp <= kConjLdsGroups) ? 0 : 1;
+    sh->items[c] += b.nitems;
+    sh->max_items[c] = std::max(sh->max_items[c], b.nitems);
+    if (b.lds) sh->lds_bytes[c] = std::max<size_t>(sh->lds_bytes[c], (size_t)p.nr_grp * 8);
+    cls[c].push_back(b);
   }
-  *total_items = items;
-  if (max_items) *max_items = maxi;
-  *lds_bytes = (size_t)max_lds_grp * 8;
+  for (int c = 0; c < 2; ++c) sh->nq[c] = (uint32_t)cls[c].size();
+  std::copy(cls[0].begin(), cls[0].end(), h);
+  std::copy(cls[1].begin(), cls[1].end(), h + cls[0].size());
   return MBX_OK;
 }
 
@@ -216,18 +236,13 @@ static int build_batch(mbx_ctx *ctx, const mbx_query *qs, uint32_t nq,
   int st = valid_tune(&t);
   if (st) return st;
   std::vector<BatchQ> h(nq);
-  uint32_t items = 0;
-  size_t lds_bytes = 0;
-  uint32_t max_items = 0;
-  st = batch_prepare(ctx, qs, nq, t, h.data(), &items, &lds_bytes, &max_items);
+  BatchShape shape;
+  st = batch_prepare(ctx, qs, nq, t, h.data(), &shape);
   if (st) return st;
   auto *b = new mbx_batch;
   b->nq = nq;
-  b->total_items = items;
-  b->max_items = max_items;
+  b->shape = shape;
   b->tune = t;
-  b->lds_bytes = lds_bytes;
-  b->grid = batch_grid(ctx, t, items);
   void *mem = nullptr;
   const size_t qbytes = ceil_div(sizeof(BatchQ) * nq, 256) * 256;
   st = ctx->pool->alloc(qbytes + kCounterBytes, &mem);
@@ -312,7 +327,7 @@ int mbx_autotune(mbx_ctx *ctx, const mbx_query *qs, uint32_t nq,
                  mbx_tune *best, float *best_ms) {
   if (!ctx || !qs || !best) return fail(MBX_ERR_INVALID, "autotune: null");
   static const uint32_t tiles[] = {512, 1024, 2048, 4096, 8192};
-  static const uint32_t wgs[] = {2, 4, 8};
+  static const uint32_t wgs[] = {2, 4, 5, 8};
   static const uint32_t ldsg[] = {1024, 4096};
   hipEvent_t e0, e1;
   MBX_HIP(hipEventCreate(&e0));

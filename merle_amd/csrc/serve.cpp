@@ -221,20 +221,17 @@ int mbx_serve_replay(mbx_ctx *ctx, const mbx_query *reqs, uint32_t nq,
           if (!st) st = launch_fused(ctx, qp, nullptr);
         }
         ctx->stream = keep;
-        uint32_t items = 0, max_items = 0;
-        size_t lds = 0;
+        BatchShape shape;
         const uint32_t kb = nsmall;
         if (!st && kb)
-          st = batch_prepare(ctx, tmp.data(), kb, tune, sl.h, &items, &lds,
-                             &max_items);
+          st = batch_prepare(ctx, tmp.data(), kb, tune, sl.h, &shape);
         if (!st && kb) {
           hipError_t e = hipMemcpyAsync(sl.d, sl.h, sizeof(BatchQ) * kb,
                                         hipMemcpyHostToDevice, sl.s);
           if (e != hipSuccess) st = hip_fail(e, "serve: descriptors");
         }
         if (!st && kb)
-          st = batch_launch(sl.s, sl.d, kb, max_items, sl.counter, lds,
-                            batch_grid(ctx, tune, items));
+          st = batch_launch(ctx, sl.s, sl.d, shape, sl.counter, tune);
       }
       if (!st) {
         hipError_t e = hipEventRecord(sl.done, sl.s);

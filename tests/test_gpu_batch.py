@@ -57,11 +57,21 @@ def plain_results(dev, qs, specs):
     return res
 
 
-def test_batch_equals_plain_and_oracle(dev, tables):
+@pytest.mark.parametrize("eager", [False, True])
+def test_batch_equals_plain_and_oracle(dev, tables, eager):
+    """Early-out queries all run in the general kernel; with eager loads the
+    conjunctions with up to 3,000 groups run in the conjunction kernel and
+    the rest (larger LDS tables, HBM tables) in the general one, in the same
+    graph."""
     sch, bins = tables
-    specs = configs.config3_queries(64, seed=11)
+    specs = configs.config3_queries(64, seed=12)
     qs, outs = make_queries(dev, sch, bins, specs)
     plain = plain_results(dev, qs, specs)
+    if eager:
+        for i in range(len(specs)):
+            qs[i].flags = abi.PLAN_EAGER
+        assert any(s.nr_grp > 4096 for s in specs)
+        assert any(3000 < s.nr_grp <= 4096 for s in specs)
     for i in range(0, len(specs), 8):           # oracle on a subset
         want, _ = cpuref.query(specs[i], sch)
         np.testing.assert_array_equal(plain[i], want, err_msg=specs[i].name)
