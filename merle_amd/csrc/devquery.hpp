@@ -769,6 +769,13 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t x,
 // reference's method 1 / method 2 switch (primitive.cuh:373-397) becomes
 // rounds: a dense range takes more rounds, each using all lanes.
 constexpr uint32_t kWaveList = kListCap / 4;
+// A wave range with at least this many candidate rows per word on average
+// (a quarter of its rows) skips the row list.  The list is filled round by
+// round, and in a dense range only the two or three lanes whose rows fall
+// in the round write it, one row at a time: for ranges where most rows are
+// uncertain (an edge bin over a skewed column's head) that serial fill cost
+// more than the row op itself.
+constexpr uint32_t kDenseRowsPerWord = 16;
 
 template <bool CHK, bool ACCT, int WPL, int EAGER, int E = 2, bool DIRECT = false>
 __device__ __forceinline__ void run_wave_range(const DevPrg &P, const DevOp &O,
@@ -809,6 +816,50 @@ __device__ __forceinline__ void run_wave_range(const DevPrg &P, const DevOp &O,
   const uint32_t incl = wave_incl_scan(cnt);
   const uint32_t total = __shfl(incl, 63, 64);
   const uint32_t off = incl - cnt;
+  if (total >= kDenseRowsPerWord * 64u * WPL) {
+    // Dense: word by word (E at a time), lane b takes bit b, so a column's
+    // loads for a word's rows are one coalesced 256-B access.  Where the
+    // wave's words fit its list space they wait there (read back as
+    // broadcasts), so they hold no registers during the row op.
+    constexpr int NV = CHK ? 2 : 1;
+    constexpr bool kStage = NV * WPL * 64 * 8 <= (int)(kWaveList * 4);
+    uint64_t *sw = reinterpret_cast<uint64_t *>(s_list);
+    if constexpr (kStage) {
+#pragma unroll
+      for (int k = 0; k < WPL; ++k) {
+        sw[(NV * k) * 64 + lane] = p[k];
+        if (CHK) sw[(NV * k + 1) * 64 + lane] = u[k];
+      }
+      wave_lds_sync();
+    }
+#pragma unroll
+    for (int k = 0; k < WPL; ++k) {
+      const uint32_t wk = w0 + ws * k;   // lane s holds word wk + s
+      for (uint32_t s = 0; s < 64u; s += E) {
+        uint32_t ent[E];
+        bool valid[E];
+        bool anyv = false;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const uint32_t src = s + (uint32_t)e;
+          const uint64_t pw = kStage ? sw[(NV * k) * 64 + src]
+                                     : __shfl(p[k], (int)src, 64);
+          valid[e] = (pw >> lane) & 1ull;
+          ent[e] = (wk + src) * 64u + lane;
+          if (CHK) {
+            const uint64_t uw = kStage ? sw[(NV * k + 1) * 64 + src]
+                                       : __shfl(u[k], (int)src, 64);
+            if ((uw >> lane) & 1ull) ent[e] |= 0x80000000u;
+          }
+          anyv = anyv || valid[e];
+        }
+        if (__ballot(anyv) == 0) continue;
+        process_entries<CHK, ACCT, E>(P, O, nr_grp, ent, valid, sink, acct);
+      }
+    }
+    if (kStage) wave_lds_sync();   // the list space is rewritten next range
+    return;
+  }
   for (uint32_t base = 0; base < total; base += kWaveList) {
     const uint32_t end = base + kWaveList;
     if (cnt && off < end && off + cnt > base) {

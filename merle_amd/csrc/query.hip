@@ -33,7 +33,7 @@ struct QArgs {
 // replicated flush (flush_groups).
 template <bool CHK, bool LDSG, bool ACCT, int WPL, int EAGER, bool DIRECT = false>
 __global__ __launch_bounds__(256) void fused_kernel(QArgs a) {
-  __shared__ uint32_t s_list[4][kWaveList];
+  __shared__ __align__(16) uint32_t s_list[4][kWaveList];
   GroupSink sink;
   sink.lds = LDSG;
   sink.out = a.out;

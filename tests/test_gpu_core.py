@@ -149,6 +149,65 @@ def test_bins_fk_table_pool_full(dev):
     df.free()
 
 
+@pytest.mark.parametrize("n", [70001, 1_000_003])
+def test_dense_candidate_ranges(dev, n):
+    """Ranges where most rows are candidates take the word-by-word row op
+    instead of the row list: two-valued (75% x 80% of rows) and three-valued
+    (an edge bin over the head of the Zipf column a32 leaves most rows
+    uncertain; AND NOT of a misaligned range), plain and through a foreign
+    key, with LDS and HBM group tables, every plan == oracle."""
+    rng = np.random.default_rng(n)
+    schema = gpuq.upload_schema(dev, datagen.small_schema(n, 77))
+    a16 = gpuq.exact_bitmap(dev, schema, S.Pred(S.Src("a16"), 0, 3000))
+    b32 = gpuq.exact_bitmap(dev, schema, S.Pred(S.Src("b32"), 0, 40))
+    b_a32 = list(range(0, 1001, 250))
+    a32 = gpuq.create_bin(dev, schema, S.Src("a32"), b_a32[:-1], b_a32[1:])
+    b_a8 = list(range(0, 257, 8))
+    a8 = gpuq.create_bin(dev, schema, S.Src("a8"), b_a8[:-1], b_a8[1:])
+    b_d32 = list(range(0, 61, 7))
+    d32 = gpuq.create_bin(dev, schema, S.Src("d32", "fk"), b_d32[:-1], b_d32[1:])
+
+    def binned(ptrs, bounds, lo, hi):
+        mid, l, r = S.from_bin(bounds, lo, hi)
+        return S.BitmapCol([ptrs[i] for i in mid], None if l is None else ptrs[l],
+                           None if r is None else ptrs[r])
+
+    O, A, M, N_ = abi.ORM, abi.ANDM, abi.AND, abi.NOT
+    cases = [
+        ([S.Pred(S.Src("a16"), 0, 3000), S.Pred(S.Src("b32"), 0, 40)],
+         [S.BitmapCol([a16]), S.BitmapCol([b32])], [(O, 0, 0), (A, 0, 1)]),
+        ([S.Pred(S.Src("a32"), 1, 500), S.Pred(S.Src("a8"), 10, 20)],
+         [binned(a32, b_a32, 1, 500), binned(a8, b_a8, 10, 20)],
+         [(O, 0, 0), (O, 1, 1), (N_, 1, 1), (M, 0, 1)]),
+        ([S.Pred(S.Src("a32"), 3, 700), S.Pred(S.Src("d32", "fk"), 2, 30)],
+         [binned(a32, b_a32, 3, 700), binned(d32, b_d32, 2, 30)],
+         [(O, 0, 0), (A, 0, 1)]),
+    ]
+    for preds, cols, ins in cases:
+        for extra in (0, 5000):
+            keys, total = datagen.random_keys(rng)
+            m, ma, mb = datagen.random_measure(rng)
+            spec = S.QuerySpec(preds, ins, keys, total + extra, m, ma, mb)
+            want, nsel = cpuref.query(spec, schema)
+            assert nsel > n // 8, spec   # candidates dense enough to skip the row list
+            for plan in (abi.PLAN_FUSED, abi.PLAN_FUSED | abi.PLAN_EAGER,
+                         abi.PLAN_MATERIALISE, abi.PLAN_SCAN):
+                got = gpuq.run_query(dev, schema, spec, cols, plan)
+                np.testing.assert_array_equal(got, want, err_msg=f"plan {plan} {spec}")
+            # the accounting kernel walks the same ranges: its count of
+            # selected rows is the oracle's
+            prg = S.to_vprg(cols, schema.nrows, spec.instrs)
+            op = S.to_rowop(spec, schema)
+            nb, ns = C.c_uint64(), C.c_uint64()
+            check(dev.lib.mbx_alg_bytes(dev.ctx, C.byref(prg), C.byref(op), spec.nr_grp,
+                                        C.byref(nb), C.byref(ns)))
+            assert ns.value == nsel, spec
+    for ptr in (a16, b32, a32[0], a8[0], d32[0]):
+        check(dev.lib.mbx_free(dev.ctx, ptr))
+    for c in schema.cols.values():
+        c.dev.free()
+
+
 @pytest.mark.parametrize("seed", range(16))
 def test_random_queries_all_plans(dev, seed):
     """Random programs (AND/OR/NOT/ANDM/ORM, exact and edge bins), group
