@@ -183,6 +183,28 @@ int make_dev_prg(const mbx_vprg *p, DevPrg *out) {
     if (mem) out->memcols |= 1u << in.src1;
   }
   out->nins = n;
+  // Truth table of the program over the exact column predicates, with the
+  // row re-check's semantics (registers start false; ANDM / ORM read
+  // column src1, AND / OR / NOT registers): an uncertain row is then one
+  // lookup instead of an interpretation of the instruction list.
+  uint32_t truth = 0;
+  for (uint32_t m = 0; m < (1u << MBX_MAXCOLS); ++m) {
+    bool r[3] = {false, false, false};
+    for (uint32_t i = 0; i < n; ++i) {
+      const uint8_t op = out->ins[i][0], d = out->ins[i][1], s = out->ins[i][2];
+      bool v;
+      if (op == MBX_AND) v = r[d] && r[s];
+      else if (op == MBX_OR) v = r[d] || r[s];
+      else if (op == MBX_NOT) v = !r[s];
+      else {
+        const bool es = (m >> s) & 1u;
+        v = op == MBX_ANDM ? (r[d] && es) : (r[d] || es);
+      }
+      r[d] = v;
+    }
+    if (r[0]) truth |= 1u << m;
+  }
+  out->truth = truth;
   // The flat operand list: column blocks (middle bins, then edge bins),
   // clause by clause when the program is a CNF of column literals, else in
   // column order; flend marks the last entry of each clause (each column

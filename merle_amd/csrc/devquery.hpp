@@ -470,8 +470,9 @@ __device__ __forceinline__ uint64_t src_value(const DevSrc &s, uint32_t idx,
   return ok ? elem_at<ACCT>(s.data, s.eb, idx, acct) : 0;
 }
 
-// Re-check of an uncertain row: the program's instruction list over the
-// exact predicates of its columns (the op's chk branch, ssb/lambdas.cuh:27-33).
+// Re-check of an uncertain row: the program over the exact predicates of
+// its columns (the op's chk branch, ssb/lambdas.cuh:27-33), looked up in the
+// program's truth table (bit c of the index = column c's predicate).
 template <bool ACCT>
 __device__ __forceinline__ bool row_passes(const DevPrg &P, const DevOp &O,
                                            uint32_t row, const Acct *acct) {
@@ -479,31 +480,15 @@ __device__ __forceinline__ bool row_passes(const DevPrg &P, const DevOp &O,
 #pragma unroll
   for (int c = 0; c < MBX_MAXCOLS; ++c)
     if (c < (int)O.npreds) ix[c] = src_index<ACCT>(O.pred[c], row, acct);
-  bool e[MBX_MAXCOLS] = {false, false, false, false};
+  uint32_t m = 0;
 #pragma unroll
   for (int c = 0; c < MBX_MAXCOLS; ++c)
     if (c < (int)O.npreds) {
       bool ok;
       const uint64_t v = src_value<ACCT>(O.pred[c], ix[c], ok, acct);
-      e[c] = ok && v >= O.lo[c] && v < O.hi[c];
+      m |= (ok && v >= O.lo[c] && v < O.hi[c]) ? 1u << c : 0u;
     }
-  bool r0 = false, r1 = false, r2 = false;
-  for (uint32_t i = 0; i < P.nins; ++i) {
-    const uint32_t op = P.ins[i][0], d = P.ins[i][1], s = P.ins[i][2];
-    const bool rd = d == 0 ? r0 : (d == 1 ? r1 : r2);
-    bool v;
-    if (op == MBX_AND || op == MBX_OR || op == MBX_NOT) {
-      const bool rs = s == 0 ? r0 : (s == 1 ? r1 : r2);
-      v = op == MBX_AND ? (rd && rs) : (op == MBX_OR ? (rd || rs) : !rs);
-    } else {
-      const bool es = s == 0 ? e[0] : (s == 1 ? e[1] : (s == 2 ? e[2] : e[3]));
-      v = op == MBX_ANDM ? (rd && es) : (rd || es);
-    }
-    r0 = d == 0 ? v : r0;
-    r1 = d == 1 ? v : r1;
-    r2 = d == 2 ? v : r2;
-  }
-  return r0;
+  return (P.truth >> m) & 1u;
 }
 
 // {value, group} of E selected rows at once (primitive.cuh:10-15); ok[e] is

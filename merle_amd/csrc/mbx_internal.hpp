@@ -124,6 +124,9 @@ struct DevPrg {
   // columns c of (OR of c's bins), complemented where flneg
   // (e.g. ORM r0<-a; ANDM r0<-b; ORM r1<-c; NOT r1; AND r0,r1 = a & b & ~c)
   uint32_t conj;
+  // the program as a boolean function of its columns' exact predicates:
+  // bit m is r0 when column c's predicate is bit c of m (at most 4 columns)
+  uint32_t truth;
 };
 struct DevSrc {
   const void *data;
