@@ -251,6 +251,15 @@ int make_dev_prg(const mbx_vprg *p, DevPrg *out) {
   return MBX_OK;
 }
 
+void key_div_magic(uint32_t d, uint64_t *m, uint8_t *sh1, uint8_t *sh2) {
+  uint32_t l = 0;
+  while ((1ull << l) < d) ++l;
+  const unsigned __int128 num = (unsigned __int128)((1ull << l) - d) << 64;
+  *m = (uint64_t)(num / d) + 1;
+  *sh1 = (uint8_t)(l < 1 ? l : 1);
+  *sh2 = (uint8_t)(l > 1 ? l - 1 : 0);
+}
+
 int make_dev_op(const mbx_rowop *op, uint32_t npreds_hint, DevOp *out) {
   if (!op || !out) return fail(MBX_ERR_INVALID, "rowop: null");
   std::memset(out, 0, sizeof(*out));
@@ -280,6 +289,7 @@ int make_dev_op(const mbx_rowop *op, uint32_t npreds_hint, DevOp *out) {
     d.src = to_dev(k.src);
     d.kind = k.kind;
     d.div = k.div;
+    if (k.kind == MBX_KEY_DIV) key_div_magic(k.div, &d.magic, &d.sh1, &d.sh2);
     d.base = k.base;
     d.stride = k.stride;
     d.nb = k.nbounds;

@@ -491,6 +491,13 @@ __device__ __forceinline__ bool row_passes(const DevPrg &P, const DevOp &O,
   return (P.truth >> m) & 1u;
 }
 
+// x / kp.div with key_div_magic's multiplier and shifts: exact for every
+// 64-bit x, a few multiplies instead of the 64-bit division sequence.
+__device__ __forceinline__ uint64_t key_div(const DevKey &kp, uint64_t x) {
+  const uint64_t t = __umul64hi(kp.magic, x);
+  return (t + ((x - t) >> kp.sh1)) >> kp.sh2;
+}
+
 // {value, group} of E selected rows at once (primitive.cuh:10-15); ok[e] is
 // false for a dropped row (fk out of range or key outside [0, nr_grp)).
 template <bool ACCT, int E>
@@ -547,7 +554,7 @@ __device__ __forceinline__ void rows_keyval(const DevOp &O, const uint32_t *row,
         const uint64_t x = v[e][i];
         uint64_t f = x;
         if (kp.kind == MBX_KEY_DIV) {
-          f = x / kp.div;
+          f = key_div(kp, x);
         } else if (kp.kind == MBX_KEY_BUCKET) {
           f = 0;
           for (uint32_t j = 0; j < kp.nb; ++j) f += kp.bounds[j] <= x;

@@ -138,7 +138,9 @@ struct DevKey {
   DevSrc src;
   uint32_t kind, div;
   int64_t base, stride;
-  uint32_t nb, pad;
+  uint32_t nb;
+  uint8_t sh1, sh2, pad[2];   // KEY_DIV: shifts of the division by div
+  uint64_t magic;             // KEY_DIV: multiplier of the division by div
   uint64_t bounds[MBX_MAXBOUNDS];
 };
 struct DevOp {
@@ -151,6 +153,12 @@ struct DevOp {
 
 // Validation + conversion; returns MBX_OK or an error (with message).
 int make_dev_prg(const mbx_vprg *p, DevPrg *out);
+// Division of any 64-bit x by an invariant d >= 1 as
+//   t = mulhi64(m, x);  q = (t + ((x - t) >> sh1)) >> sh2
+// (Granlund and Montgomery, "Division by Invariant Integers using
+// Multiplication", PLDI 1994, figure 4.1, N = 64): l = ceil(log2 d),
+// m = floor(2^64 (2^l - d) / d) + 1, sh1 = min(l, 1), sh2 = max(l - 1, 0).
+void key_div_magic(uint32_t d, uint64_t *m, uint8_t *sh1, uint8_t *sh2);
 int make_dev_op(const mbx_rowop *op, uint32_t npreds_hint, DevOp *out);
 int check_src(const mbx_colref &c, const char *what);
 

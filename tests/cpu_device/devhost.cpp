@@ -6,7 +6,8 @@
 //     registers, and the conjunction form (two- and three-valued) -- which
 //     must agree bit for bit (status 97-99 otherwise), then the row op and
 //     group-by on the selected rows;
-//   dh_scan: the bitmap-free scan (every row, exact predicates).
+//   dh_scan: the bitmap-free scan (every row, exact predicates);
+//   dh_key_div: the key division by an invariant divisor.
 #include "../../merle_amd/csrc/devquery.hpp"
 thread_local host_dim3 threadIdx, blockIdx, blockDim, gridDim;
 unsigned long long mbx::s_grp_dyn[1 << 16];
@@ -16,6 +17,13 @@ extern "C" {
 long dh_conj_words() { return g_conj; }
 long dh_chk_conj_words() { return g_chk_conj; }
 void *dh_alloc(size_t n) { return malloc(n ? n : 1); }
+// x / d through the kernels' key division (key_div_magic + key_div)
+uint64_t dh_key_div(uint32_t d, uint64_t x) {
+  DevKey k{};
+  k.div = d;
+  key_div_magic(d, &k.magic, &k.sh1, &k.sh2);
+  return key_div(k, x);
+}
 void dh_free(void *p) { free(p); }
 int dh_scan(const mbx_vprg *prg, const mbx_rowop *op, uint32_t nr_grp, int64_t *out) {
   DevPrg P;

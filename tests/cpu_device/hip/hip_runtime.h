@@ -29,6 +29,9 @@ extern thread_local host_dim3 threadIdx, blockIdx, blockDim, gridDim;
 template <class T> T host_ntload(const T *p) { return *p; }
 #define __builtin_nontemporal_load(p) host_ntload(p)
 inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
+inline unsigned long long __umul64hi(unsigned long long a, unsigned long long b) {
+  return (unsigned long long)(((unsigned __int128)a * b) >> 64);
+}
 inline unsigned __shfl_up(unsigned v, int, int) { return v; }
 inline unsigned __shfl(unsigned v, int, int) { return v; }
 inline void __syncthreads() {}

@@ -24,6 +24,29 @@ for f in (L.dh_scan, L.dh_fused):
     f.argtypes = [C.POINTER(abi.Vprg), C.POINTER(abi.RowOp), C.c_uint32, C.c_void_p]
 L.dh_conj_words.restype = C.c_long
 L.dh_chk_conj_words.restype = C.c_long
+L.dh_key_div.restype = C.c_uint64
+L.dh_key_div.argtypes = [C.c_uint32, C.c_uint64]
+
+# The key division by multiplication equals integer division for divisors
+# 1..300, powers of two and their neighbours, random 32-bit divisors, and
+# dividends at every boundary of 32- and 64-bit ranges plus random ones.
+rng0 = np.random.default_rng(5)
+divs = list(range(1, 301)) + [1 << k for k in range(32)] + \
+    [(1 << k) + 1 for k in range(1, 32)] + [(1 << k) - 1 for k in range(2, 33)] + \
+    [int(x) for x in rng0.integers(1, 1 << 32, 200)]
+ndiv = 0
+for d in divs:
+    xs = [0, 1, d - 1, d, d + 1, 2 * d - 1, 2 * d, (1 << 32) - 1, 1 << 32,
+          (1 << 63) - 1, 1 << 63, (1 << 64) - 1, (1 << 64) - d] + \
+        [int(x) for x in rng0.integers(0, 1 << 63, 8, dtype=np.uint64)] + \
+        [int(x) for x in rng0.integers(0, 1 << 32, 8)]
+    for x in xs:
+        got = L.dh_key_div(d, x)
+        if got != x // d:
+            print(f"key division wrong: {x} / {d} = {got}, want {x // d}")
+            sys.exit(96)
+        ndiv += 1
+print(f"key division: {ndiv} quotients exact")
 
 
 class HostArray:
