@@ -50,7 +50,8 @@ __device__ __forceinline__ void batch_item(const BatchQ &q, uint32_t w_begin,
 
 // CONJ: the class-0 kernel (eager conjunctions, LDS table); else every form.
 template <bool CONJ>
-__global__ __launch_bounds__(256) void batch_kernel(BatchArgs a) {
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(CONJ ? 5 : 1, 8))) void batch_kernel(BatchArgs a) {
   __shared__ __align__(16) uint32_t s_list[4][kWaveList];
   __shared__ uint32_t s_item;
   // uniform, so the row-list addresses stay scalar (4 VGPRs fewer)

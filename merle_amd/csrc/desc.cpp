@@ -244,6 +244,24 @@ int make_dev_prg(const mbx_vprg *p, DevPrg *out) {
       if (out->end[c] > out->beg[c]) out->flend[out->end[c] - 1] = 1;
     }
   }
+  // Columns the re-check of a candidate row need not load: those no memory
+  // instruction reads (the program does not depend on them), and in a
+  // conjunction the columns of clauses without edge bins, which every
+  // candidate row satisfies -- a plain clause is its exact OR, true (its
+  // columns are set true), a negated one its column's exact predicate,
+  // false.  Each column sits in one clause, so the others are unaffected.
+  uint32_t known = ~out->memcols & ((1u << MBX_MAXCOLS) - 1u), known_val = 0;
+  if (conj)
+    for (uint32_t k = 0; k < cnf.n; ++k) {
+      bool edge = false;
+      for (uint32_t c = 0; c < MBX_MAXCOLS; ++c)
+        if ((cnf.mask[k] >> c) & 1u) edge = edge || out->cols[c].has_edge;
+      if (edge) continue;
+      known |= cnf.mask[k];
+      if (!cnf.neg[k]) known_val |= cnf.mask[k];
+    }
+  out->known = known;
+  out->known_val = known_val;
   out->nrows = p->factsz;
   out->nwords = (uint32_t)ceil_div(p->factsz, 64);
   out->chk = chk ? 1u : 0u;

@@ -472,18 +472,25 @@ __device__ __forceinline__ uint64_t src_value(const DevSrc &s, uint32_t idx,
 
 // Re-check of an uncertain row: the program over the exact predicates of
 // its columns (the op's chk branch, ssb/lambdas.cuh:27-33), looked up in the
-// program's truth table (bit c of the index = column c's predicate).
-template <bool ACCT>
+// program's truth table (bit c of the index = column c's predicate).  For
+// a candidate row (CAND: a row the bitmaps left in P or U) the columns in
+// P.known are not loaded: their value is P.known_val (no bit at or above
+// npreds: a program with edge bins reads only columns below npreds, which
+// the launch checks).  The scan plan re-checks every row and loads every
+// column.
+template <bool ACCT, bool CAND = true>
 __device__ __forceinline__ bool row_passes(const DevPrg &P, const DevOp &O,
                                            uint32_t row, const Acct *acct) {
+  const uint32_t known = CAND ? P.known : 0u;
   uint32_t ix[MBX_MAXCOLS];
 #pragma unroll
   for (int c = 0; c < MBX_MAXCOLS; ++c)
-    if (c < (int)O.npreds) ix[c] = src_index<ACCT>(O.pred[c], row, acct);
-  uint32_t m = 0;
+    if (c < (int)O.npreds && !((known >> c) & 1u))
+      ix[c] = src_index<ACCT>(O.pred[c], row, acct);
+  uint32_t m = CAND ? P.known_val : 0u;
 #pragma unroll
   for (int c = 0; c < MBX_MAXCOLS; ++c)
-    if (c < (int)O.npreds) {
+    if (c < (int)O.npreds && !((known >> c) & 1u)) {
       bool ok;
       const uint64_t v = src_value<ACCT>(O.pred[c], ix[c], ok, acct);
       m |= (ok && v >= O.lo[c] && v < O.hi[c]) ? 1u << c : 0u;

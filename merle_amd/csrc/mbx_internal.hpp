@@ -127,6 +127,9 @@ struct DevPrg {
   // the program as a boolean function of its columns' exact predicates:
   // bit m is r0 when column c's predicate is bit c of m (at most 4 columns)
   uint32_t truth;
+  // columns whose predicate the re-check of a candidate row need not load
+  // (bit c), and the value it takes for them (known_val bit c)
+  uint32_t known, known_val;
 };
 struct DevSrc {
   const void *data;

@@ -63,7 +63,7 @@ __global__ __launch_bounds__(256) void scan_kernel(QArgs a) {
   const uint32_t n = a.prg.nrows;
   for (uint32_t row = blockIdx.x * 256 + threadIdx.x; row < n;
        row += gridDim.x * 256) {
-    if (!row_passes<false>(a.prg, a.op, row, nullptr)) continue;
+    if (!row_passes<false, false>(a.prg, a.op, row, nullptr)) continue;
     uint32_t key;
     int64_t val;
     if (row_keyval<false>(a.op, row, a.nr_grp, key, val, nullptr))

@@ -33,7 +33,7 @@ int dh_scan(const mbx_vprg *prg, const mbx_rowop *op, uint32_t nr_grp, int64_t *
   st = make_dev_op(op, 0, &O);
   if (st) return st;
   for (uint32_t row = 0; row < P.nrows; ++row) {
-    if (!row_passes<false>(P, O, row, nullptr)) continue;
+    if (!row_passes<false, false>(P, O, row, nullptr)) continue;
     uint32_t key;
     int64_t val;
     if (row_keyval<false>(O, row, nr_grp, key, val, nullptr)) out[key] += val;
