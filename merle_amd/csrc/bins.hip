@@ -163,8 +163,8 @@ __device__ __forceinline__ void store_line(uint32_t *dst, const uint64_t *w) {
 
 // Transpose buffer of the TR kernels (more than 8 bins): 8 KB per wave, as
 // dynamic LDS of kTransposeLds bytes.  The kernels for up to 8 bins have
-// neither the buffer nor its registers (76 against 104 VGPRs: 6 waves per
-// SIMD against 4 for the latency-bound foreign-key gathers).
+// neither the buffer nor its registers (76 against 83 VGPRs: 6 waves per
+// SIMD against 5 for the latency-bound foreign-key gathers).
 extern __shared__ u64x2 s_tr_dyn[];
 constexpr size_t kTransposeLds = 4 * 64 * 8 * sizeof(u64x2);
 
