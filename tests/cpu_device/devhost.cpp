@@ -7,7 +7,8 @@
 //     must agree bit for bit (status 97-99 otherwise), then the row op and
 //     group-by on the selected rows;
 //   dh_scan: the bitmap-free scan (every row, exact predicates);
-//   dh_key_div: the key division by an invariant divisor.
+//   dh_key_div: the key division by an invariant divisor;
+//   dh_prg_masks: the re-check's truth table and unloaded columns.
 #include "../../merle_amd/csrc/devquery.hpp"
 thread_local host_dim3 threadIdx, blockIdx, blockDim, gridDim;
 unsigned long long mbx::s_grp_dyn[1 << 16];
@@ -17,6 +18,18 @@ extern "C" {
 long dh_conj_words() { return g_conj; }
 long dh_chk_conj_words() { return g_chk_conj; }
 void *dh_alloc(size_t n) { return malloc(n ? n : 1); }
+// The host-built re-check tables of a program: truth table, columns a
+// candidate row need not load and their value, conjunction flag.
+int dh_prg_masks(const mbx_vprg *prg, uint32_t *out) {
+  DevPrg P;
+  int st = make_dev_prg(prg, &P);
+  if (st) return st;
+  out[0] = P.truth;
+  out[1] = P.known;
+  out[2] = P.known_val;
+  out[3] = P.conj;
+  return 0;
+}
 // x / d through the kernels' key division (key_div_magic + key_div)
 uint64_t dh_key_div(uint32_t d, uint64_t x) {
   DevKey k{};

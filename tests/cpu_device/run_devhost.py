@@ -48,6 +48,73 @@ for d in divs:
         ndiv += 1
 print(f"key division: {ndiv} quotients exact")
 
+# The re-check of a candidate row looks the program up in a truth table and
+# does not load the columns the host marks as known.  Over 256 rows (row
+# m + 16e: column c's exact predicate is bit c of m; for a column with edge
+# bins, bit c of e puts the row in its edge bin) the oracle evaluates each
+# random program three-valued (P, U): the truth table must equal the
+# program's two-valued result on the 16 predicate patterns, and for every
+# row in P the result with the known columns substituted must equal it.
+L.dh_prg_masks.argtypes = [C.POINTER(abi.Vprg), C.c_void_p]
+dummy = L.dh_alloc(256)
+rows = np.arange(256)
+mpat, epat = rows % 16, rows // 16
+
+
+def bits(mask):
+    w = np.zeros(8, np.uint32)
+    for r in np.nonzero(mask)[0]:
+        w[r // 32] |= np.uint32(1 << (r % 32))
+    return w
+
+
+def row_bits(words):
+    return np.array([(int(words[r // 32]) >> (r % 32)) & 1 for r in range(256)], bool)
+
+
+CNF_SHAPES = [[(abi.ORM, 0, 0), (abi.ANDM, 0, 1), (abi.ANDM, 0, 2)],
+              [(abi.ORM, 0, 0), (abi.ORM, 0, 1), (abi.ANDM, 0, 2)],
+              [(abi.ORM, 0, 0), (abi.ORM, 1, 1), (abi.NOT, 1, 1), (abi.AND, 0, 1)],
+              [(abi.ORM, 0, 0), (abi.ORM, 1, 1), (abi.NOT, 1, 1), (abi.AND, 0, 1),
+               (abi.ANDM, 0, 2), (abi.ANDM, 0, 3)]]
+nprog = nconj = 0
+for it in range(600):
+    ins = CNF_SHAPES[it % 4] if it % 2 else datagen.random_program(rng0, int(rng0.integers(1, 5)))
+    edge = [bool(rng0.random() < 0.5) for _ in range(4)]
+    ref_cols = []
+    prg = abi.Vprg()
+    for c in range(4):
+        pred = ((mpat >> c) & 1).astype(bool)
+        inedge = ((epat >> c) & 1).astype(bool) & edge[c]
+        mid = bits(pred & ~inedge)
+        ref_cols.append(([mid], bits(inedge) if edge[c] else None, None))
+        prg.cols[c].middle[0] = dummy
+        if edge[c]:
+            prg.cols[c].leftmost = dummy
+    prg.factsz = 256
+    for i, (o, d, s_) in enumerate(ins):
+        prg.instrs[i] = abi.Instr(o, d, s_, 0)
+    out4 = (C.c_uint32 * 4)()
+    if L.dh_prg_masks(C.byref(prg), out4) != 0:
+        print("dh_prg_masks failed", ins)
+        sys.exit(95)
+    truth, known, kval, conj = list(out4)
+    exact = row_bits(cpuref.vprg_eval(cpuref.vprg_of(
+        [([bits((mpat >> c) & 1 == 1)], None, None) for c in range(4)], 256, ins), 0)[0])
+    for m in range(16):
+        if ((truth >> m) & 1) != exact[m]:
+            print(f"truth table wrong: {ins} m={m}")
+            sys.exit(94)
+    poss, _ = cpuref.vprg_eval(cpuref.vprg_of(ref_cols, 256, ins), 1)
+    for r in np.nonzero(row_bits(poss))[0]:
+        m = int(mpat[r])
+        if ((truth >> m) & 1) != ((truth >> ((m & ~known) | kval)) & 1):
+            print(f"unloaded columns change a candidate row: {ins} edges={edge} row={r}")
+            sys.exit(93)
+    nprog += 1
+    nconj += conj
+print(f"re-check tables: {nprog} programs ({nconj} conjunctions) agree with the oracle")
+
 
 class HostArray:
     """A column in a heap block of exactly its size (bounds-checked)."""
