@@ -348,9 +348,26 @@ def test_multipass_launch_shapes(dev):
         if len(picked.setdefault(kind, [])) < 2:
             picked[kind].append((q, cols))
     assert {(False, False), (True, False)} <= set(picked)
-    for kind, qs in sorted(picked.items()):
+    # dense candidates at this size (the word-by-word row op with the words
+    # kept in registers: three-valued at 4 words per lane, two-valued at 6):
+    # an edge bin over the Zipf head of p0 (half the rows uncertain), and a
+    # program that is not a conjunction, (p0 & p1) | p2, two-valued
+    O, A, R = abi.ORM, abi.ANDM, abi.OR
+    dense = [S.QuerySpec([S.Pred(S.Src("p0"), 3, 100), S.Pred(S.Src("p1"), 0, 100)],
+                         [(O, 0, 0), (A, 0, 1)],
+                         [S.Key(S.Src("p2"), abi.KEY_DIV, 100, 0)], 10,
+                         abi.MEAS_COUNT, None, None, "dense-chk"),
+             S.QuerySpec([S.Pred(S.Src("p0"), 0, 25), S.Pred(S.Src("p1"), 0, 25),
+                          S.Pred(S.Src("p2"), 0, 100)],
+                         [(O, 0, 0), (A, 0, 1), (O, 1, 2), (R, 0, 1)],
+                         [S.Key(S.Src("p1"), abi.KEY_DIV, 250, 0)], 4,
+                         abi.MEAS_COUNT, None, None, "dense-or")]
+    picked["dense"] = [(q, configs.bin_cols(q, bins)) for q in dense]
+    for kind, qs in sorted(picked.items(), key=str):
         for q, cols in qs:
-            want, _ = cpuref.query(q, sch)
+            want, nsel = cpuref.query(q, sch)
+            if kind == "dense":
+                assert nsel > n // 5, q.name
             for plan in (abi.PLAN_FUSED | abi.PLAN_EAGER, abi.PLAN_FUSED):
                 got = gpuq.run_query(dev, sch, q, cols, plan)
                 np.testing.assert_array_equal(got, want,
