@@ -104,7 +104,12 @@ size_t mbx_bitmap_stride_words(uint32_t nrows);
  * bitmaps, bit j set iff mins[j] <= value < maxs[j]; a foreign key >= dimsz
  * reads as INT64_MAX (:26-27).  One slab is allocated from the pool;
  * out[j] = out[0] + j*stride; free only out[0].  *single_bytes (optional)
- * gets ceil(N/32)*4 as the reference returns.  Host arrays mins/maxs/out. */
+ * gets ceil(N/32)*4 as the reference returns.  Host arrays mins/maxs/out.
+ * The call waits for its kernels when it uses temporary pool memory: bounds
+ * on the device (bins of unequal widths or not consecutive), or the bin
+ * index of every dimension row for a foreign-key column with consecutive
+ * bins (dimsz+1 entries of 1 byte, 2 from 255 bins; read directly instead
+ * when the pool has no room). */
 int mbx_create_bin(mbx_ctx *ctx, const mbx_colref *src, uint32_t nrows,
                    const uint64_t *mins, const uint64_t *maxs, size_t nbins,
                    uint32_t **out, size_t *single_bytes);
