@@ -48,34 +48,6 @@ def profiled_traffic():
     return None
 
 
-def build_index(dev, sch, attrs):
-    """One bitmap per attribute value, projected through the fact FK
-    (create_bin with fk).  Slabs are allocated first, so the timed region
-    holds only the creation kernels (the reference times cudaMalloc too,
-    ssb/bmpidx-create.cu:64-68).  Returns {attr: [device ptrs]}, ms, bytes."""
-    from merle_amd import spec as S
-    from merle_amd.configs import C2_ATTRS
-    from merle_amd.host import Timer, check, stride_words
-    idx, ms, nbytes = {}, 0.0, 0
-    sw = stride_words(sch.nrows)
-    t = Timer(dev)
-    for a in attrs:
-        card = C2_ATTRS[a][2]
-        ref = S.colref(sch, S.Src(a, C2_ATTRS[a][1]))
-        slab = dev.alloc(sw * 4 * card)
-        dev.sync()
-        t.start()
-        check(dev.lib.mbx_create_bin_into(dev.ctx, C.byref(ref), sch.nrows,
-                                          (C.c_uint64 * card)(*range(card)),
-                                          (C.c_uint64 * card)(*range(1, card + 1)),
-                                          card, slab.ptr, sw))
-        ms += t.stop()
-        nbytes += sch.nrows * 4 + card * sch.nrows // 8
-        idx[a] = [slab.ptr + j * sw * 4 for j in range(card)]
-    t.close()
-    return idx, ms, nbytes
-
-
 def copy_calibration(nbytes, reps):
     """Plain device copy of nbytes, reps times (for byte-counter scaling)."""
     from merle_amd.host import Device, Timer, check
@@ -91,17 +63,6 @@ def copy_calibration(nbytes, reps):
                       "GB_per_s": round(2 * nbytes * reps / ms / 1e6, 1)}))
     dev.close()
     return 0
-
-
-def query_cols(spec, idx):
-    from merle_amd import spec as S
-    cols = []
-    for p in spec.preds:
-        card = len(idx[p.src.col])
-        mid, l, r = S.from_bin(list(range(card + 1)), p.lo, p.hi)
-        assert l is None and r is None, "config-2 bins are exact"
-        cols.append(S.BitmapCol([idx[p.src.col][i] for i in mid]))
-    return cols
 
 
 def _cpu_model():
@@ -175,56 +136,6 @@ def cpu_baseline(dev, sch, specs, target_s=5.0, reps=3):
                 cpu_model=_cpu_model(), nproc=os.cpu_count()), R, results
 
 
-def make_bins(dev, sch, bounds_of):
-    """One create_bin_into slab per column -> {col: [bin ptrs]}, [slabs]."""
-    from merle_amd import abi
-    from merle_amd.host import check, stride_words
-    sw = stride_words(sch.nrows)
-    out, slabs = {}, []
-    for col, b in bounds_of.items():
-        nb = len(b) - 1
-        slab = dev.alloc(sw * 4 * nb)
-        ref = abi.ColRef(sch[col].dev.ptr, None, 0, 4)
-        check(dev.lib.mbx_create_bin_into(dev.ctx, C.byref(ref), sch.nrows,
-                                          (C.c_uint64 * nb)(*b[:-1]),
-                                          (C.c_uint64 * nb)(*b[1:]), nb,
-                                          slab.ptr, sw))
-        out[col] = [slab.ptr + j * sw * 4 for j in range(nb)]
-        slabs.append(slab)
-    dev.sync()
-    return out, slabs
-
-
-def query_array(sch, specs, bins, bounds):
-    from merle_amd import abi, configs, spec as S
-    qs = (abi.Query * len(specs))()
-    for i, s in enumerate(specs):
-        qs[i].prg = S.to_vprg(configs.bin_cols(s, bins, bounds), sch.nrows,
-                              s.instrs)
-        qs[i].op = S.to_rowop(s, sch)
-        qs[i].nr_grp = s.nr_grp
-    return qs
-
-
-def plan_queries(dev, qs):
-    """A copy of the query array with the static heuristic plan applied to
-    every query (sparsest column first; eager loads unless the early-out
-    pays), as the batched and serving paths run them."""
-    from merle_amd import abi
-    from merle_amd.host import check
-    out = (abi.Query * len(qs))()
-    C.memmove(out, qs, C.sizeof(out))
-    for i in range(len(qs)):
-        dens = (C.c_double * 4)()
-        check(dev.lib.mbx_col_density(dev.ctx, C.byref(qs[i].prg), dens))
-        plan = abi.Plan()
-        check(dev.lib.mbx_plan_heuristic(C.byref(qs[i].prg), dens, C.byref(plan)))
-        check(dev.lib.mbx_plan_apply(C.byref(qs[i].prg), C.byref(plan),
-                                     C.byref(out[i].prg)))
-        out[i].flags = plan.flags & abi.PLAN_EAGER
-    return out
-
-
 def run_batched(dev, reps=5, train_programs=1024):
     """Config 3: 256 queries over the config-1 tables.  Plain = one fused
     launch per query, back to back on one stream (the reference's
@@ -235,9 +146,9 @@ def run_batched(dev, reps=5, train_programs=1024):
     from merle_amd.host import Timer, check
     n = configs.C1_ROWS
     sch = configs.config1_device(dev, n)
-    bins, slabs = make_bins(dev, sch, {c: configs.C1_BOUNDS for c in configs.C1_COLS})
+    bins, slabs = configs.build_bins(dev, sch, {c: configs.C1_BOUNDS for c in configs.C1_COLS})
     specs = configs.config3_queries(256)
-    qs = query_array(sch, specs, bins, configs.C1_BOUNDS)
+    qs = configs.query_array(sch, specs, bins, configs.C1_BOUNDS)
     offs = np.cumsum([0] + [q.nr_grp for q in specs])
     plain_out = dev.alloc(8 * int(offs[-1]), np.int64, int(offs[-1]))
     batch_out = dev.alloc(8 * int(offs[-1]), np.int64, int(offs[-1]))
@@ -245,7 +156,7 @@ def run_batched(dev, reps=5, train_programs=1024):
         qs[i].d_out = batch_out.ptr + 8 * int(offs[i])
     t = Timer(dev)
 
-    pq = plan_queries(dev, qs)
+    pq = configs.heuristic_queries(dev, qs)
 
     def plain(arr):
         check(dev.lib.mbx_fill_u32(dev.ctx, plain_out.ptr, 0, 2 * int(offs[-1])))
@@ -353,22 +264,26 @@ def run_batched(dev, reps=5, train_programs=1024):
     }
 
 
-def run_serving(dev, nreq=1000, rates=(2000.0, 20000.0), budget=256 << 20):
+def run_serving(dev, nreq=1000, rates=(2000.0, 20000.0), budget=256 << 20,
+                tight=256 << 10):
     """Config 5: 600M rows x 10 Zipf columns + 640 bucket bitmaps resident;
-    Poisson arrivals replayed against the serving loop at a moderate and at a
+    Poisson arrivals replayed against the server at a moderate and at a
     saturating rate, once with continuous batching (2 streams, up to 4
     requests per dispatch, large requests as their own fused launches) and
-    once one request at a time; every request's result must be identical."""
+    once one request at a time; then once more at the saturating rate with a
+    budget of the server's fixed memory plus `tight` bytes, so admission
+    binds.  Every request's result must be identical in every replay."""
     from merle_amd import abi, configs
     from merle_amd.host import check
     n = configs.C5_ROWS
     t0 = time.time()
     sch = configs.config5_device(dev, n)
-    bins, slabs = make_bins(dev, sch, configs.C5_BOUNDS)
+    bins, slabs = configs.build_bins(dev, sch, configs.C5_BOUNDS)
     log(f"config-5 tables and {sum(len(v) for v in bins.values())} bitmaps "
         f"ready in {time.time() - t0:.1f} s")
     specs = configs.config5_queries(nreq)
-    qs = plan_queries(dev, query_array(sch, specs, bins, configs.C5_BOUNDS))
+    qs = configs.heuristic_queries(
+        dev, configs.query_array(sch, specs, bins, configs.C5_BOUNDS))
     tot = sum(q.nr_grp for q in specs)
     # algorithmic bytes from a sample of the requests (the accounting kernel
     # is slow), scaled to all of them
@@ -377,49 +292,68 @@ def run_serving(dev, nreq=1000, rates=(2000.0, 20000.0), budget=256 << 20):
     check(dev.lib.mbx_alg_bytes_batch(dev.ctx, qs, ns, C.byref(nbytes),
                                       C.byref(nsel)))
     est = nbytes.value * nreq / ns
-    out, same = {}, True
+
+    def replay(arrivals, cfg):
+        res = np.zeros(tot, np.int64)
+        lat = np.zeros(nreq)
+        st = abi.ServeStats()
+        check(dev.lib.mbx_serve_replay(
+            dev.ctx, qs, nreq, arrivals.ctypes.data_as(C.POINTER(C.c_double)),
+            C.byref(cfg), res.ctypes.data,
+            lat.ctypes.data_as(C.POINTER(C.c_double)), C.byref(st)))
+        gbs = est / st.wall_s / 1e9
+        return res, {
+            "completed": st.completed, "rejected": st.rejected,
+            "wall_s": round(st.wall_s, 4),
+            "G_rows_per_s": round(n * st.completed / st.wall_s / 1e9, 1),
+            "p50_ms": round(st.p50_s * 1e3, 3),
+            "p99_ms": round(st.p99_s * 1e3, 3),
+            "mean_ms": round(st.mean_s * 1e3, 3),
+            "mean_batch": round(st.mean_batch, 2),
+            "admission_waits": st.admission_waits,
+            "peak_mem_MB": round(st.peak_mem / 1e6, 3),
+            "fixed_mem_MB": round(st.fixed_mem / 1e6, 3),
+            "alg_GB_per_s": round(gbs, 1),
+            "roofline_frac": round(gbs / HBM_PEAK_GBS, 4),
+        }
+
+    batched_cfg = abi.ServeCfg(2, 4, budget, 0, 0, abi.Tune(8192, 4, 4096, 0))
+    plain_cfg = abi.ServeCfg(1, 1, budget, 1, 0, abi.Tune())
+    out, same, ref = {}, True, None
     for rate in rates:
         rng = np.random.default_rng(5)
         arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
         arrivals -= arrivals[0]
-        res_by_mode, block = {}, {}
-        for mode in ("batched", "one_at_a_time"):
-            res = np.zeros(tot, np.int64)
-            lat = np.zeros(nreq)
-            st = abi.ServeStats()
-            if mode == "batched":
-                cfg = abi.ServeCfg(2, 4, budget, 0, 0, abi.Tune(8192, 4, 4096, 0))
-            else:
-                cfg = abi.ServeCfg(1, 1, budget, 1, 0, abi.Tune())
-            check(dev.lib.mbx_serve_replay(
-                dev.ctx, qs, nreq, arrivals.ctypes.data_as(C.POINTER(C.c_double)),
-                C.byref(cfg), res.ctypes.data,
-                lat.ctypes.data_as(C.POINTER(C.c_double)), C.byref(st)))
-            res_by_mode[mode] = res
-            gbs = est / st.wall_s / 1e9
-            block[mode] = {
-                "completed": st.completed, "rejected": st.rejected,
-                "wall_s": round(st.wall_s, 4),
-                "G_rows_per_s": round(n * st.completed / st.wall_s / 1e9, 1),
-                "p50_ms": round(st.p50_s * 1e3, 3),
-                "p99_ms": round(st.p99_s * 1e3, 3),
-                "mean_ms": round(st.mean_s * 1e3, 3),
-                "mean_batch": round(st.mean_batch, 2),
-                "peak_mem_MB": round(st.peak_mem / 1e6, 2),
-                "alg_GB_per_s": round(gbs, 1),
-                "roofline_frac": round(gbs / HBM_PEAK_GBS, 4),
-            }
-        same &= bool(np.array_equal(res_by_mode["batched"],
-                                    res_by_mode["one_at_a_time"]))
+        block = {}
+        for mode, cfg in (("batched", batched_cfg),
+                          ("one_at_a_time", plain_cfg)):
+            res, block[mode] = replay(arrivals, cfg)
+            ref = res if ref is None else ref
+            same &= bool(np.array_equal(res, ref))
         block["batched_vs_one_at_a_time"] = round(
             block["one_at_a_time"]["wall_s"] / block["batched"]["wall_s"], 3)
+        block["p99_batched_vs_one_at_a_time"] = round(
+            block["batched"]["p99_ms"] / block["one_at_a_time"]["p99_ms"], 3)
         out[f"rate_{int(rate)}_per_s"] = block
+    # admission binding: the fixed memory plus a few outputs
+    fixed = C.c_uint64()
+    check(dev.lib.mbx_serve_fixed_mem(C.byref(batched_cfg), C.byref(fixed)))
+    tight_cfg = abi.ServeCfg(2, 4, fixed.value + tight, 0, 0,
+                             abi.Tune(8192, 4, 4096, 0))
+    rng = np.random.default_rng(5)
+    arrivals = np.cumsum(rng.exponential(1.0 / max(rates), nreq))
+    arrivals -= arrivals[0]
+    res, tb = replay(arrivals, tight_cfg)
+    same &= bool(np.array_equal(res, ref))
+    tb["budget_MB"] = round((fixed.value + tight) / 1e6, 3)
+    out[f"rate_{int(max(rates))}_per_s_tight_budget"] = tb
     for a in slabs + [c.dev for c in sch.cols.values()]:
         a.free()
     return {
         "workload": f"{n:,} rows x 10 Zipf(1.2) columns (1,000..100,000 values), "
                     f"64 bucket bitmaps per column resident; {nreq} requests per "
-                    f"replay, Poisson arrivals, memory budget {budget >> 20} MiB",
+                    f"replay, Poisson arrivals, memory budget {budget >> 20} MiB "
+                    f"(tight: fixed + {tight >> 10} KiB)",
         **out,
         "identical_results": same,
         "alg_bytes_note": f"estimated from the first {ns} requests",
@@ -491,7 +425,9 @@ def main():
     sch = configs.config2_device(dev, n)
     specs = configs.config2_queries()
     log("building bitmap index (one bitmap per attribute value)")
-    idx, cms, cbytes = build_index(dev, sch, list(configs.C2_ATTRS))
+    t = Timer(dev)
+    idx, slabs, cms, cbytes = configs.config2_index(dev, sch, timer=t)
+    t.close()
     nbm = sum(len(v) for v in idx.values())
     log(f"index: {nbm} bitmaps in {cms:.1f} ms ({cbytes / cms / 1e6:.0f} GB/s)")
 
@@ -501,17 +437,10 @@ def main():
     # static plan of each query (outside the timed region): column densities
     # -> sparsest column first, eager loads unless the early-out pays
     qs, plans = [], []
-    for i, s in enumerate(specs):
-        prg = S.to_vprg(query_cols(s, idx), n, s.instrs)
-        dens = (C.c_double * 4)()
-        check(dev.lib.mbx_col_density(dev.ctx, C.byref(prg), dens))
-        plan = abi.Plan()
-        check(dev.lib.mbx_plan_heuristic(C.byref(prg), dens, C.byref(plan)))
-        planned = abi.Vprg()
-        check(dev.lib.mbx_plan_apply(C.byref(prg), C.byref(plan), C.byref(planned)))
-        op = S.to_rowop(s, sch)
+    for i, (s, (planned, op, flags)) in enumerate(
+            zip(specs, configs.config2_plans(dev, sch, idx, specs))):
         qs.append((s, planned, op, outbuf.ptr + 8 * int(offs[i])))
-        plans.append(int(plan.flags))
+        plans.append(flags)
 
     # algorithmic bytes of each query (device accounting kernel)
     balg, nsel = [], []
@@ -599,12 +528,13 @@ def main():
         log("CPU baseline (oracle) on a bounded sample")
         cb, R, cres = cpu_baseline(dev, sch, specs)
         result["cpu_baseline"] = cb
-        # parity on the sample: the same queries restricted to R rows
+        # parity on the sample: the same queries, under the plans the timed
+        # step runs, restricted to R rows
         ok = True
-        for (s, prg, op, out), want, o in zip(qs, cres, offs):
+        for (s, prg, op, out), want, o, fl in zip(qs, cres, offs, plans):
             prg.factsz = R
             check(dev.lib.mbx_prog_grpby(dev.ctx, C.byref(prg), C.byref(op),
-                                         s.nr_grp, out, abi.PLAN_FUSED))
+                                         s.nr_grp, out, fl & ~abi.ACCUMULATE))
             dev.sync()
             got = outbuf.host()[int(o):int(o) + s.nr_grp]
             ok &= bool(np.array_equal(got, want))
@@ -613,6 +543,8 @@ def main():
         log(f"cpu {cb['value']} G rows/s on {cb['cores']} threads; parity {ok}")
     if not args.no_extras:
         outbuf.free()
+        for a in slabs + [c.dev for c in sch.cols.values()]:
+            a.free()
         log("config 3: batched vs plain")
         result["batched"] = run_batched(dev)
         log(f"  {result['batched']}")

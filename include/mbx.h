@@ -257,21 +257,40 @@ int mbx_batch_destroy(mbx_ctx *ctx, mbx_batch *b);
 /* Times candidate launch shapes on the batch and returns the fastest. */
 int mbx_autotune(mbx_ctx *ctx, const mbx_query *qs, uint32_t nq,
                  mbx_tune *best, float *best_ms);
+/* Same, also reporting every shape timed and its median time (the first
+ * `max` of them; *count = how many were timed). */
+int mbx_autotune_report(mbx_ctx *ctx, const mbx_query *qs, uint32_t nq,
+                        mbx_tune *best, float *best_ms, mbx_tune *cands,
+                        float *cand_ms, int max, int *count);
 int mbx_default_tune(mbx_tune *t);
 
 
 /* ------------------------------------------------------------- serving */
-/* No reference counterpart.  Replays a request trace (arrival times in seconds, sorted) against
- * resident data: memory-based admission (each request's int64 output is
- * allocated from the pool when it fits mem_budget), continuous batching of
- * admitted requests onto nstreams slots, up to max_batch requests per
- * persistent batched launch; plain = 1 runs the one-at-a-time baseline (one
- * fused launch per request, one stream).  reqs[i].d_out is ignored: results
- * land in h_results (requests' outputs concatenated in request order);
- * latency_s[i] = result on the host - arrival, or -1 if refused. */
+/* No reference counterpart (the reference runs one query per launch from a
+ * single host thread).  A server keeps the caller's columns and bitmaps
+ * where they are and runs submitted queries on nstreams slots (a HIP stream
+ * each): memory-based admission, then continuous batching -- whenever a slot
+ * is free, up to max_batch admitted requests go out on it as one persistent
+ * batched launch (requests of >= fused_rows fact rows as fused launches of
+ * their own), while the other slots are still running.  plain = 1 is the
+ * one-at-a-time baseline (one slot, one fused launch per request).
+ *
+ * Memory: the server's fixed working memory (per slot: descriptors, work
+ * counters, flush replicas) is taken from the pool at open and charged
+ * against mem_budget first; each request then reserves its int64 output
+ * (nr_grp * 8 bytes, rounded to 256) at admission.  Admission is FIFO: a
+ * request that can never fit is rejected; one that does not fit now waits
+ * until running requests release memory; if the pool itself refuses it
+ * while nothing of the server's is running or admitted, it is rejected.
+ * Serving runs the fused and batched plans only (no materialise scratch).
+ *
+ * Ordering: every request is ordered after the work enqueued on the
+ * context's stream before its submit call (the slot waits on an event
+ * recorded there).  Results are copied to the host by per-slot completion
+ * threads, never by the dispatching thread. */
 typedef struct mbx_serve_cfg {
-  uint32_t nstreams;
-  uint32_t max_batch;
+  uint32_t nstreams;        /* 1..16 slots */
+  uint32_t max_batch;       /* requests per dispatch, >= 1 */
   uint64_t mem_budget;      /* bytes; 0 = unbounded */
   uint32_t plain;           /* 1: one request at a time, one stream */
   uint32_t fused_rows;      /* requests with >= this many fact rows run as
@@ -281,10 +300,52 @@ typedef struct mbx_serve_cfg {
 typedef struct mbx_serve_stats {
   uint32_t completed, rejected;
   double wall_s, p50_s, p99_s, mean_s, max_s;
-  uint32_t batches, admission_waits;
+  uint32_t batches;         /* dispatches (launch groups) */
+  uint32_t admission_waits; /* requests delayed at least once by admission */
   double mean_batch;
-  uint64_t peak_mem;
+  uint64_t peak_mem;        /* bytes held at the peak, fixed part included */
+  uint32_t cancelled, failed;
+  uint64_t fixed_mem;       /* the server's fixed working memory */
 } mbx_serve_stats;
+
+enum mbx_req_state {
+  MBX_REQ_QUEUED = 0,       /* waiting for admission */
+  MBX_REQ_ADMITTED = 1,     /* output reserved, waiting for a slot */
+  MBX_REQ_RUNNING = 2,      /* on a slot */
+  MBX_REQ_DONE = 3,         /* result in h_result */
+  MBX_REQ_REJECTED = 4,     /* refused by admission */
+  MBX_REQ_CANCELLED = 5,
+  MBX_REQ_FAILED = 6        /* a HIP error; see mbx_serve_close's status */
+};
+
+typedef struct mbx_server mbx_server;
+/* The fixed working memory a server of this configuration takes at open
+ * (so a caller can size mem_budget). */
+int mbx_serve_fixed_mem(const mbx_serve_cfg *cfg, uint64_t *bytes);
+int mbx_serve_open(mbx_ctx *ctx, const mbx_serve_cfg *cfg, mbx_server **out);
+/* q->d_out is ignored; the result (q->nr_grp int64) lands in h_result, host
+ * memory that must stay valid until the request ends.  Returns at once with
+ * a ticket. */
+int mbx_serve_submit(mbx_server *s, const mbx_query *q, int64_t *h_result,
+                     uint64_t *ticket);
+/* State of a request and, once it is done, its latency (result on the host
+ * minus submit time; -1 otherwise). */
+int mbx_serve_poll(mbx_server *s, uint64_t ticket, int *state,
+                   double *latency_s);
+/* Waits until the request ends or timeout_s passes (< 0: no limit). */
+int mbx_serve_wait(mbx_server *s, uint64_t ticket, double timeout_s,
+                   int *state, double *latency_s);
+/* Cancels a queued or admitted request; MBX_ERR_INVALID once it runs. */
+int mbx_serve_cancel(mbx_server *s, uint64_t ticket);
+/* Waits for every submitted request to end, then frees the server.  Returns
+ * the first HIP error any request met, else MBX_OK. */
+int mbx_serve_close(mbx_server *s, mbx_serve_stats *stats);
+
+/* A client of the server: replays a request trace (arrival times in
+ * seconds, sorted), submitting each request at its arrival time.
+ * reqs[i].d_out is ignored: results land in h_results (requests' outputs
+ * concatenated in request order); latency_s[i] = result on the host -
+ * arrival, or -1 if refused. */
 int mbx_serve_replay(mbx_ctx *ctx, const mbx_query *reqs, uint32_t nq,
                      const double *arrival_s, const mbx_serve_cfg *cfg,
                      int64_t *h_results, double *latency_s,

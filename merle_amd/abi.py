@@ -90,7 +90,13 @@ class ServeStats(C.Structure):
                 ("p99_s", C.c_double), ("mean_s", C.c_double),
                 ("max_s", C.c_double), ("batches", C.c_uint32),
                 ("admission_waits", C.c_uint32), ("mean_batch", C.c_double),
-                ("peak_mem", C.c_uint64)]
+                ("peak_mem", C.c_uint64), ("cancelled", C.c_uint32),
+                ("failed", C.c_uint32), ("fixed_mem", C.c_uint64)]
+
+
+# request states of the server (enum mbx_req_state)
+REQ_QUEUED, REQ_ADMITTED, REQ_RUNNING, REQ_DONE, REQ_REJECTED, REQ_CANCELLED, \
+    REQ_FAILED = range(7)
 
 
 NFEAT = 16
@@ -144,10 +150,21 @@ PROTOTYPES = {
     "mbx_batch_run_eager": (_i32, [_vp, _vp]),
     "mbx_batch_destroy": (_i32, [_vp, _vp]),
     "mbx_autotune": (_i32, [_vp, _P(Query), _u32, _P(Tune), _P(C.c_float)]),
+    "mbx_autotune_report": (_i32, [_vp, _P(Query), _u32, _P(Tune),
+                                   _P(C.c_float), _P(Tune), _P(C.c_float),
+                                   _i32, _P(_i32)]),
     "mbx_default_tune": (_i32, [_P(Tune)]),
     "mbx_serve_replay": (_i32, [_vp, _P(Query), _u32, _P(C.c_double),
                                 _P(ServeCfg), _vp, _P(C.c_double),
                                 _P(ServeStats)]),
+    "mbx_serve_fixed_mem": (_i32, [_P(ServeCfg), _P(_u64)]),
+    "mbx_serve_open": (_i32, [_vp, _P(ServeCfg), _P(_vp)]),
+    "mbx_serve_submit": (_i32, [_vp, _P(Query), _vp, _P(_u64)]),
+    "mbx_serve_poll": (_i32, [_vp, _u64, _P(_i32), _P(C.c_double)]),
+    "mbx_serve_wait": (_i32, [_vp, _u64, C.c_double, _P(_i32),
+                              _P(C.c_double)]),
+    "mbx_serve_cancel": (_i32, [_vp, _u64]),
+    "mbx_serve_close": (_i32, [_vp, _P(ServeStats)]),
     "mbx_timer_create": (_i32, [_vp, _P(_vp)]),
     "mbx_timer_start": (_i32, [_vp, _vp]),
     "mbx_timer_stop": (_i32, [_vp, _vp, _P(C.c_float)]),

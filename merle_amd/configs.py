@@ -225,6 +225,80 @@ def config2_device(dev, n=C2_ROWS, seed=SEED):
     return sch
 
 
+def config2_index(dev, sch, attrs=None, timer=None):
+    """One bitmap per attribute value, projected through the fact FK (one
+    create_bin_into pass per attribute, bins [v, v+1)).  Slabs are allocated
+    first, so a timer brackets only the creation kernels (the reference
+    times cudaMalloc too, ssb/bmpidx-create.cu:64-68).
+    -> ({attr: [device ptr per value]}, [slabs], ms, algorithmic bytes)."""
+    from .host import stride_words
+    from .spec import colref
+    idx, slabs, ms, nbytes = {}, [], 0.0, 0
+    sw = stride_words(sch.nrows)
+    for a in (attrs or list(C2_ATTRS)):
+        card = C2_ATTRS[a][2]
+        ref = colref(sch, Src(a, C2_ATTRS[a][1]))
+        slab = dev.alloc(sw * 4 * card)
+        if timer:
+            dev.sync()
+            timer.start()
+        check(dev.lib.mbx_create_bin_into(dev.ctx, C.byref(ref), sch.nrows,
+                                          (C.c_uint64 * card)(*range(card)),
+                                          (C.c_uint64 * card)(*range(1, card + 1)),
+                                          card, slab.ptr, sw))
+        if timer:
+            ms += timer.stop()
+        nbytes += sch.nrows * 4 + card * sch.nrows // 8
+        idx[a] = [slab.ptr + j * sw * 4 for j in range(card)]
+        slabs.append(slab)
+    dev.sync()
+    return idx, slabs, ms, nbytes
+
+
+def config2_cols(spec, idx):
+    """BitmapCols of a config-2 query: every predicate covers whole values,
+    so each column is the OR of the exact per-value bitmaps."""
+    from .spec import BitmapCol
+    cols = []
+    for p in spec.preds:
+        ptrs = idx[p.src.col]
+        assert 0 <= p.lo < p.hi <= len(ptrs) and p.hi - p.lo <= 4
+        cols.append(BitmapCol([ptrs[v] for v in range(p.lo, p.hi)]))
+    return cols
+
+
+def config2_plans(dev, sch, idx, specs):
+    """The static heuristic plan of each query (column densities on the
+    device -> sparsest column first, eager loads unless the ANDM early-out
+    pays).  -> [(planned vprg, row op, plan flags)]."""
+    from .spec import to_rowop, to_vprg
+    out = []
+    for s in specs:
+        prg = to_vprg(config2_cols(s, idx), sch.nrows, s.instrs)
+        dens = (C.c_double * 4)()
+        check(dev.lib.mbx_col_density(dev.ctx, C.byref(prg), dens))
+        plan = abi.Plan()
+        check(dev.lib.mbx_plan_heuristic(C.byref(prg), dens, C.byref(plan)))
+        planned = abi.Vprg()
+        check(dev.lib.mbx_plan_apply(C.byref(prg), C.byref(plan), C.byref(planned)))
+        out.append((planned, to_rowop(s, sch), int(plan.flags)))
+    return out
+
+
+def host_schema(sch, rows=None):
+    """A host copy of a device schema (for the oracle): every column, the
+    fact columns (those with sch.nrows entries) cut to `rows`."""
+    h = Schema(sch.nrows if rows is None else rows)
+    for k, c in sch.cols.items():
+        if c.host is not None:
+            h.add(k, c.host if c.n != sch.nrows or rows is None else c.host[:rows])
+        elif c.n == sch.nrows and rows is not None:
+            h.add(k, c.dev.host(rows))
+        else:
+            h.add(k, c.dev.host())
+    return h
+
+
 # ---------------------------------------------------------------- config 3
 def zipf_pmf(n=C1_DOMAIN, s=1.0):
     w = 1.0 / np.arange(1, n + 1, dtype=np.float64) ** s
@@ -418,3 +492,73 @@ def config5_queries(nq, seed=SEED + 55):
         out.append(pool[int(rng.integers(len(pool)))][1])
     order = rng.permutation(len(out))       # arrival order: mixed selectivities
     return [out[k] for k in order]
+
+
+# ------------------------------------------------------------ shared helpers
+def build_bins(dev, sch, bounds_of):
+    """One create_bin_into slab per column: {col: [bin ptrs]}, [slabs]."""
+    from .host import stride_words
+    sw = stride_words(sch.nrows)
+    out, slabs = {}, []
+    for col, b in bounds_of.items():
+        nb = len(b) - 1
+        slab = dev.alloc(sw * 4 * nb)
+        ref = abi.ColRef(sch[col].dev.ptr, None, 0, 4)
+        check(dev.lib.mbx_create_bin_into(dev.ctx, C.byref(ref), sch.nrows,
+                                          (C.c_uint64 * nb)(*b[:-1]),
+                                          (C.c_uint64 * nb)(*b[1:]), nb,
+                                          slab.ptr, sw))
+        out[col] = [slab.ptr + j * sw * 4 for j in range(nb)]
+        slabs.append(slab)
+    dev.sync()
+    return out, slabs
+
+
+def query_array(sch, specs, bins, bounds):
+    """mbx_query array of specs over bin slabs (programs as written)."""
+    from .spec import to_rowop, to_vprg
+    qs = (abi.Query * len(specs))()
+    for i, s in enumerate(specs):
+        qs[i].prg = to_vprg(bin_cols(s, bins, bounds), sch.nrows, s.instrs)
+        qs[i].op = to_rowop(s, sch)
+        qs[i].nr_grp = s.nr_grp
+    return qs
+
+
+def heuristic_queries(dev, qs):
+    """A copy of a query array with the static heuristic plan applied to
+    every query (sparsest column first; eager loads unless the early-out
+    pays), as the batched and serving paths run them."""
+    out = (abi.Query * len(qs))()
+    C.memmove(out, qs, C.sizeof(out))
+    for i in range(len(qs)):
+        dens = (C.c_double * 4)()
+        check(dev.lib.mbx_col_density(dev.ctx, C.byref(qs[i].prg), dens))
+        plan = abi.Plan()
+        check(dev.lib.mbx_plan_heuristic(C.byref(qs[i].prg), dens, C.byref(plan)))
+        check(dev.lib.mbx_plan_apply(C.byref(qs[i].prg), C.byref(plan),
+                                     C.byref(out[i].prg)))
+        out[i].flags = plan.flags & abi.PLAN_EAGER
+    return out
+
+
+def spec_columns(spec):
+    """Names of every column a query reads (predicates, keys, measures,
+    their foreign keys)."""
+    names = set()
+    for src in [p.src for p in spec.preds] + [k.src for k in spec.keys] + \
+            [spec.ma, spec.mb]:
+        if src is not None:
+            names.add(src.col)
+            if src.fk:
+                names.add(src.fk)
+    return names
+
+
+def host_subset(sch, names):
+    """Host copies of the named columns of a device schema."""
+    h = Schema(sch.nrows)
+    for k in names:
+        c = sch[k]
+        h.add(k, c.host if c.host is not None else c.dev.host())
+    return h

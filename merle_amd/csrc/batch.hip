@@ -284,6 +284,65 @@ static int build_batch(mbx_ctx *ctx, const mbx_query *qs, uint32_t nq,
   return MBX_OK;
 }
 
+// Project-written autotuner: times every candidate launch shape on the
+// batch itself (eager launches, median of 3 after one warm-up) and keeps the
+// fastest.  Results do not depend on the shape: each candidate computes the
+// same sums.  cands/cand_ms (optional) receive every shape timed.
+static int autotune(mbx_ctx *ctx, const mbx_query *qs, uint32_t nq,
+                    mbx_tune *best, float *best_ms, mbx_tune *cands,
+                    float *cand_ms, int max, int *count) {
+  if (!ctx || !qs || !best) return fail(MBX_ERR_INVALID, "autotune: null");
+  static const uint32_t tiles[] = {512, 1024, 2048, 4096, 8192};
+  static const uint32_t wgs[] = {2, 4, 5, 8};
+  static const uint32_t ldsg[] = {1024, 4096};
+  hipEvent_t e0, e1;
+  MBX_HIP(hipEventCreate(&e0));
+  MBX_HIP(hipEventCreate(&e1));
+  float best_t = 1e30f;
+  mbx_tune bt;
+  mbx_default_tune(&bt);
+  int st = MBX_OK, k = 0;
+  for (uint32_t tw : tiles)
+    for (uint32_t wg : wgs)
+      for (uint32_t lg : ldsg) {
+        mbx_tune t{tw, wg, lg, 0};
+        mbx_batch *b = nullptr;
+        st = build_batch(ctx, qs, nq, &t, false, &b);
+        if (st) goto done;
+        float reps[3];
+        for (int r = -1; r < 3 && !st; ++r) {
+          (void)hipEventRecord(e0, ctx->stream);
+          st = enqueue_batch(ctx, b);
+          (void)hipEventRecord(e1, ctx->stream);
+          if (hipEventSynchronize(e1) != hipSuccess)
+            st = fail(MBX_ERR_HIP, "autotune: sync");
+          float ms = 0;
+          (void)hipEventElapsedTime(&ms, e0, e1);
+          if (r >= 0) reps[r] = ms;
+        }
+        destroy_batch(ctx, b);
+        if (st) goto done;
+        std::sort(reps, reps + 3);
+        if (cands && cand_ms && k < max) {
+          cands[k] = t;
+          cand_ms[k] = reps[1];
+        }
+        ++k;
+        if (reps[1] < best_t) {
+          best_t = reps[1];
+          bt = t;
+        }
+      }
+done:
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st) return st;
+  *best = bt;
+  if (best_ms) *best_ms = best_t;
+  if (count) *count = k;
+  return MBX_OK;
+}
+
 }  // namespace mbx
 
 extern "C" {
@@ -320,56 +379,18 @@ int mbx_batch_destroy(mbx_ctx *ctx, mbx_batch *b) {
   return MBX_OK;
 }
 
-// Project-written autotuner: times every candidate launch shape on the
-// batch itself (eager launches, median of 3 after one warm-up) and keeps the
-// fastest.  Results do not depend on the shape: each candidate computes the
-// same sums.
 int mbx_autotune(mbx_ctx *ctx, const mbx_query *qs, uint32_t nq,
                  mbx_tune *best, float *best_ms) {
-  if (!ctx || !qs || !best) return fail(MBX_ERR_INVALID, "autotune: null");
-  static const uint32_t tiles[] = {512, 1024, 2048, 4096, 8192};
-  static const uint32_t wgs[] = {2, 4, 5, 8};
-  static const uint32_t ldsg[] = {1024, 4096};
-  hipEvent_t e0, e1;
-  MBX_HIP(hipEventCreate(&e0));
-  MBX_HIP(hipEventCreate(&e1));
-  float best_t = 1e30f;
-  mbx_tune bt;
-  mbx_default_tune(&bt);
-  int st = MBX_OK;
-  for (uint32_t tw : tiles)
-    for (uint32_t wg : wgs)
-      for (uint32_t lg : ldsg) {
-        mbx_tune t{tw, wg, lg, 0};
-        mbx_batch *b = nullptr;
-        st = build_batch(ctx, qs, nq, &t, false, &b);
-        if (st) goto done;
-        float reps[3];
-        for (int r = -1; r < 3 && !st; ++r) {
-          (void)hipEventRecord(e0, ctx->stream);
-          st = enqueue_batch(ctx, b);
-          (void)hipEventRecord(e1, ctx->stream);
-          if (hipEventSynchronize(e1) != hipSuccess)
-            st = fail(MBX_ERR_HIP, "autotune: sync");
-          float ms = 0;
-          (void)hipEventElapsedTime(&ms, e0, e1);
-          if (r >= 0) reps[r] = ms;
-        }
-        destroy_batch(ctx, b);
-        if (st) goto done;
-        std::sort(reps, reps + 3);
-        if (reps[1] < best_t) {
-          best_t = reps[1];
-          bt = t;
-        }
-      }
-done:
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  if (st) return st;
-  *best = bt;
-  if (best_ms) *best_ms = best_t;
-  return MBX_OK;
+  return mbx::autotune(ctx, qs, nq, best, best_ms, nullptr, nullptr, 0,
+                       nullptr);
+}
+
+int mbx_autotune_report(mbx_ctx *ctx, const mbx_query *qs, uint32_t nq,
+                        mbx_tune *best, float *best_ms, mbx_tune *cands,
+                        float *cand_ms, int max, int *count) {
+  if (!cands || !cand_ms || !count || max < 0)
+    return fail(MBX_ERR_INVALID, "autotune_report: null");
+  return mbx::autotune(ctx, qs, nq, best, best_ms, cands, cand_ms, max, count);
 }
 
 }  // extern "C"

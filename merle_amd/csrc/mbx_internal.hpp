@@ -82,6 +82,10 @@ struct mbx_ctx {
   // launches (the last workgroup of each replica clears what it used)
   unsigned long long *spread = nullptr;
   uint32_t *spread_done = nullptr;
+  // the same for every other stream the context has launched a query on
+  // (mbx_set_stream), so queries on different streams never share replicas
+  std::mutex spread_mu;
+  std::map<hipStream_t, std::pair<unsigned long long *, uint32_t *>> spreads;
 };
 
 struct mbx_timer {

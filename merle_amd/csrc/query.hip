@@ -242,12 +242,42 @@ size_t spread_bytes() {
          (size_t)kSpreadRep * kSpreadDoneStride * 4;
 }
 
-static Spread spread_of(mbx_ctx *ctx, const QueryPlan &q) {
-  Spread sp;
-  sp.rep = q.spread ? q.spread : ctx->spread;
-  sp.done = q.spread ? q.spread_done : ctx->spread_done;
-  sp.gpad = (uint32_t)((q.nr_grp + 15) / 16 * 16);
-  return sp;
+int ctx_spread(mbx_ctx *ctx, hipStream_t s, unsigned long long **rep,
+               uint32_t **done) {
+  if (s == ctx->own_stream) {
+    *rep = ctx->spread;
+    *done = ctx->spread_done;
+    return MBX_OK;
+  }
+  std::lock_guard<std::mutex> g(ctx->spread_mu);
+  auto it = ctx->spreads.find(s);
+  if (it == ctx->spreads.end()) {
+    void *p = nullptr;
+    int st = ctx->pool->alloc(spread_bytes(), &p);
+    if (st) return st;
+    st = launch_fill_u32(s, p, 0, spread_bytes() / 4);
+    if (st) {
+      ctx->pool->free(p);
+      return st;
+    }
+    auto *r = static_cast<unsigned long long *>(p);
+    auto *d = reinterpret_cast<uint32_t *>(static_cast<char *>(p) +
+                                           (size_t)kSpreadRep * kLdsGroups * 8);
+    it = ctx->spreads.emplace(s, std::make_pair(r, d)).first;
+  }
+  *rep = it->second.first;
+  *done = it->second.second;
+  return MBX_OK;
+}
+
+static int spread_of(mbx_ctx *ctx, const QueryPlan &q, Spread *sp) {
+  sp->gpad = (uint32_t)((q.nr_grp + 15) / 16 * 16);
+  if (q.spread) {
+    sp->rep = q.spread;
+    sp->done = q.spread_done;
+    return MBX_OK;
+  }
+  return ctx_spread(ctx, stream_of(ctx, q), &sp->rep, &sp->done);
 }
 
 static size_t grp_lds(const QueryPlan &q) {
@@ -278,7 +308,8 @@ static unsigned occupancy(const void *fn, size_t lds) {
 // registers, more resident waves).  Past the resident limit the waves make
 // several passes.
 template <bool C, bool L, bool A, int E, int W>
-static bool fused_try(mbx_ctx *ctx, const QArgs &a, size_t lds, bool last) {
+static bool fused_try(mbx_ctx *ctx, hipStream_t s, const QArgs &a, size_t lds,
+                      bool last) {
   const void *fn = reinterpret_cast<const void *>(&fused_kernel<C, L, A, W, E>);
   const unsigned per_cu = occupancy(fn, lds);
   const uint64_t chunks = ceil_div(a.prg.nwords, 64);
@@ -287,29 +318,31 @@ static bool fused_try(mbx_ctx *ctx, const QArgs &a, size_t lds, bool last) {
   if (waves > resident && !last) return false;
   const unsigned grid = (unsigned)ceil_div(waves < resident ? waves : resident, 4);
   hipLaunchKernelGGL((fused_kernel<C, L, A, W, E>), dim3(grid), dim3(256),
-                     lds, ctx->stream, a);
+                     lds, s, a);
   return true;
 }
 
 template <bool C, bool L, bool A, int E>
-static void fused_launch(mbx_ctx *ctx, const QArgs &a, size_t lds) {
+static void fused_launch(mbx_ctx *ctx, hipStream_t s, const QArgs &a,
+                         size_t lds) {
   if (A) {
-    (void)fused_try<C, L, A, E, 2>(ctx, a, lds, true);
+    (void)fused_try<C, L, A, E, 2>(ctx, s, a, lds, true);
     return;
   }
-  if (fused_try<C, L, A, E, 2>(ctx, a, lds, false)) return;
+  if (fused_try<C, L, A, E, 2>(ctx, s, a, lds, false)) return;
   if (C) {   // three-valued state doubles the registers: stop at 4 words
-    (void)fused_try<C, L, A, E, 4>(ctx, a, lds, true);
+    (void)fused_try<C, L, A, E, 4>(ctx, s, a, lds, true);
     return;
   }
-  if (fused_try<C, L, A, E, 4>(ctx, a, lds, false)) return;
-  (void)fused_try<C, L, A, E, C ? 4 : 6>(ctx, a, lds, true);
+  if (fused_try<C, L, A, E, 4>(ctx, s, a, lds, false)) return;
+  (void)fused_try<C, L, A, E, C ? 4 : 6>(ctx, s, a, lds, true);
 }
 
 // W words per lane and an even share of passes: the grid is cut so that
 // every wave makes the same number of full passes.
 template <bool C, bool L, bool A, int E, int W>
-static void fused_even(mbx_ctx *ctx, const QArgs &a, size_t lds) {
+static void fused_even(mbx_ctx *ctx, hipStream_t s, const QArgs &a,
+                       size_t lds) {
   const void *fn = reinterpret_cast<const void *>(&fused_kernel<C, L, A, W, E>);
   const unsigned per_cu = occupancy(fn, lds);
   const uint64_t chunks = ceil_div(a.prg.nwords, 64);
@@ -318,7 +351,7 @@ static void fused_even(mbx_ctx *ctx, const QArgs &a, size_t lds) {
   const uint64_t passes = ceil_div(waves, resident);
   const unsigned grid = (unsigned)ceil_div(ceil_div(waves, passes), 4);
   hipLaunchKernelGGL((fused_kernel<C, L, A, W, E>), dim3(grid), dim3(256),
-                     lds, ctx->stream, a);
+                     lds, s, a);
 }
 
 // The conjunction form (EAGER 3).  Its shape comes from a sweep of words per
@@ -333,16 +366,18 @@ static void fused_even(mbx_ctx *ctx, const QArgs &a, size_t lds) {
 // resident grid took 2.75 ms, 2 words in even passes 2.87-2.89, 2 words on
 // the resident grid 3.04, 1 word 3.51.
 template <bool C, bool L>
-static void fused_conj(mbx_ctx *ctx, const QArgs &a, size_t lds) {
-  if (fused_try<C, L, false, 3, 2>(ctx, a, lds, false)) return;
-  if constexpr (L) (void)fused_try<C, L, false, 3, 4>(ctx, a, lds, true);
-  else fused_even<C, L, false, 3, 2>(ctx, a, lds);
+static void fused_conj(mbx_ctx *ctx, hipStream_t s, const QArgs &a,
+                       size_t lds) {
+  if (fused_try<C, L, false, 3, 2>(ctx, s, a, lds, false)) return;
+  if constexpr (L) (void)fused_try<C, L, false, 3, 4>(ctx, s, a, lds, true);
+  else fused_even<C, L, false, 3, 2>(ctx, s, a, lds);
 }
 
 // The no-compaction ablation: early-out evaluation, 2 words per lane, the
 // resident grid.
 template <bool C, bool L>
-static void fused_direct(mbx_ctx *ctx, const QArgs &a, size_t lds) {
+static void fused_direct(mbx_ctx *ctx, hipStream_t s, const QArgs &a,
+                         size_t lds) {
   const void *fn =
       reinterpret_cast<const void *>(&fused_kernel<C, L, false, 2, 0, true>);
   const unsigned per_cu = occupancy(fn, lds);
@@ -350,14 +385,15 @@ static void fused_direct(mbx_ctx *ctx, const QArgs &a, size_t lds) {
   const uint64_t resident = (uint64_t)ctx->num_cus * per_cu * 4;
   const unsigned grid = (unsigned)ceil_div(waves < resident ? waves : resident, 4);
   hipLaunchKernelGGL((fused_kernel<C, L, false, 2, 0, true>), dim3(grid),
-                     dim3(256), lds, ctx->stream, a);
+                     dim3(256), lds, s, a);
 }
 
 template <bool C, bool L>
-static void fused_modes(mbx_ctx *ctx, const QArgs &a, size_t lds, int mode) {
-  if (mode == 3) fused_conj<C, L>(ctx, a, lds);
-  else if (mode != 0) fused_launch<C, L, false, 1>(ctx, a, lds);
-  else fused_launch<C, L, false, 0>(ctx, a, lds);
+static void fused_modes(mbx_ctx *ctx, hipStream_t s, const QArgs &a,
+                        size_t lds, int mode) {
+  if (mode == 3) fused_conj<C, L>(ctx, s, a, lds);
+  else if (mode != 0) fused_launch<C, L, false, 1>(ctx, s, a, lds);
+  else fused_launch<C, L, false, 0>(ctx, s, a, lds);
 }
 
 int launch_fused(mbx_ctx *ctx, const QueryPlan &q, const Acct *acct) {
@@ -368,7 +404,8 @@ int launch_fused(mbx_ctx *ctx, const QueryPlan &q, const Acct *acct) {
   a.out = q.out;
   a.nr_grp = q.nr_grp;
   a.acct = acct;
-  a.sp = spread_of(ctx, q);
+  int st = spread_of(ctx, q, &a.sp);
+  if (st) return st;
   const size_t lds = acct ? 0 : grp_lds(q);
   const bool chk = q.prg.chk != 0;
   // evaluation: 0 early-out; eager: 3 for conjunctions of column literals
@@ -376,20 +413,21 @@ int launch_fused(mbx_ctx *ctx, const QueryPlan &q, const Acct *acct) {
   // accumulators, then the program word by word)
   const int mode = !q.eager ? 0 : (q.prg.conj ? 3 : 1);
   // accounting runs the early-out form: the compulsory-line model
+  const hipStream_t s = stream_of(ctx, q);
   if (acct) {
-    if (chk) fused_launch<true, false, true, 0>(ctx, a, lds);
-    else fused_launch<false, false, true, 0>(ctx, a, lds);
+    if (chk) fused_launch<true, false, true, 0>(ctx, s, a, lds);
+    else fused_launch<false, false, true, 0>(ctx, s, a, lds);
   } else if (q.direct) {
-    if (chk && q.lds) fused_direct<true, true>(ctx, a, lds);
-    else if (chk) fused_direct<true, false>(ctx, a, lds);
-    else if (q.lds) fused_direct<false, true>(ctx, a, lds);
-    else fused_direct<false, false>(ctx, a, lds);
+    if (chk && q.lds) fused_direct<true, true>(ctx, s, a, lds);
+    else if (chk) fused_direct<true, false>(ctx, s, a, lds);
+    else if (q.lds) fused_direct<false, true>(ctx, s, a, lds);
+    else fused_direct<false, false>(ctx, s, a, lds);
   } else if (chk) {
-    if (q.lds) fused_modes<true, true>(ctx, a, lds, mode);
-    else fused_modes<true, false>(ctx, a, lds, mode);
+    if (q.lds) fused_modes<true, true>(ctx, s, a, lds, mode);
+    else fused_modes<true, false>(ctx, s, a, lds, mode);
   } else {
-    if (q.lds) fused_modes<false, true>(ctx, a, lds, mode);
-    else fused_modes<false, false>(ctx, a, lds, mode);
+    if (q.lds) fused_modes<false, true>(ctx, s, a, lds, mode);
+    else fused_modes<false, false>(ctx, s, a, lds, mode);
   }
   MBX_HIP(hipGetLastError());
   return MBX_OK;
@@ -402,15 +440,16 @@ int launch_scan(mbx_ctx *ctx, const QueryPlan &q) {
   a.op = q.op;
   a.out = q.out;
   a.nr_grp = q.nr_grp;
-  a.sp = spread_of(ctx, q);
+  int st = spread_of(ctx, q, &a.sp);
+  if (st) return st;
+  const hipStream_t s = stream_of(ctx, q);
   const unsigned grid =
       persistent_grid(ctx, (uint32_t)ceil_div(q.prg.nrows, 256), 8);
   if (q.lds)
     hipLaunchKernelGGL(scan_kernel<true>, dim3(grid), dim3(256), grp_lds(q),
-                       ctx->stream, a);
+                       s, a);
   else
-    hipLaunchKernelGGL(scan_kernel<false>, dim3(grid), dim3(256), 0,
-                       ctx->stream, a);
+    hipLaunchKernelGGL(scan_kernel<false>, dim3(grid), dim3(256), 0, s, a);
   MBX_HIP(hipGetLastError());
   return MBX_OK;
 }
@@ -456,7 +495,8 @@ int launch_stage3(mbx_ctx *ctx, const QueryPlan &q, const uint32_t *rows,
   a.op = q.op;
   a.out = q.out;
   a.nr_grp = q.nr_grp;
-  a.sp = spread_of(ctx, q);
+  int st = spread_of(ctx, q, &a.sp);
+  if (st) return st;
   const unsigned grid = (unsigned)ctx->num_cus * 4;
   const size_t lds = grp_lds(q);
 #define MBX_S3(C, L)                                                           \

@@ -15,11 +15,23 @@ struct QueryPlan {
   bool eager = false; // all operand loads up front (no ANDM early-out)
   bool direct = false; // no compaction: lanes walk their own set bits
   // replica tables + counters of the LDS-table flush (zero between
-  // launches); null = the context's own.  Launches that may overlap on the
-  // device need their own.
+  // launches); null = the context's set for the launch stream.  Launches
+  // that may overlap on the device need their own.
   unsigned long long *spread = nullptr;
   uint32_t *spread_done = nullptr;
+  // stream of the launch; null = the context's stream
+  hipStream_t stream = nullptr;
 };
+
+inline hipStream_t stream_of(mbx_ctx *ctx, const QueryPlan &q) {
+  return q.stream ? q.stream : ctx->stream;
+}
+
+// The context's flush replicas for stream s: one set per stream, so
+// launches on different streams of one context never share replicas (a
+// set is taken from the pool and zeroed on the stream's first use).
+int ctx_spread(mbx_ctx *ctx, hipStream_t s, unsigned long long **rep,
+               uint32_t **done);
 
 // Bytes of one set of flush replicas (tables, then counters).
 size_t spread_bytes();

@@ -1,5 +1,7 @@
-"""The serving loop: memory-based admission, continuous batching over several
-streams; every result the same alone and in the mix."""
+"""The server: memory-based admission, continuous batching over several
+streams; every result the same alone and in the mix; the live
+submit / poll / wait / cancel API; ordering after the caller's own work on
+the context's stream."""
 import ctypes as C
 
 import numpy as np
@@ -100,15 +102,193 @@ def test_admission_by_memory(dev, tables):
     qs = _queries(sch, bins, specs)
     alone = _alone(dev, qs, specs)
     arrivals = np.zeros(len(specs))           # everything arrives at once
-    # 64 KiB: below the 320,000 B of the 40,000-group request and below the
-    # 86,528 B the other 39 outputs need together, above any one of them
-    budget = 64 * 1024
-    cfg = abi.ServeCfg(2, 4, budget, 0, 0, abi.Tune())
+    # the server's fixed memory plus 64 KiB: below the 320,000 B of the
+    # 40,000-group request and below the 86,528 B the other 39 outputs need
+    # together, above any one of them
+    cfg = abi.ServeCfg(2, 4, 0, 0, 0, abi.Tune())
+    fixed = C.c_uint64()
+    check(dev.lib.mbx_serve_fixed_mem(C.byref(cfg), C.byref(fixed)))
+    budget = fixed.value + 64 * 1024
+    cfg.mem_budget = budget
     mixed, lat, st = _replay(dev, qs, arrivals, cfg, specs)
     assert lat[3] < 0 and st.rejected == 1
     assert st.completed == len(specs) - 1
-    assert st.admission_waits > 0
-    assert st.peak_mem <= budget
+    assert 0 < st.admission_waits < len(specs)   # requests, not passes
+    assert st.fixed_mem == fixed.value
+    assert fixed.value < st.peak_mem <= budget
     for i in range(len(specs)):
         if i != 3:
             np.testing.assert_array_equal(mixed[i], alone[i])
+
+
+def test_budget_below_fixed_memory_refused(dev):
+    cfg = abi.ServeCfg(2, 4, 4096, 0, 0, abi.Tune())
+    srv = C.c_void_p()
+    assert dev.lib.mbx_serve_open(dev.ctx, C.byref(cfg), C.byref(srv)) == \
+        abi.ERR_ADMISSION
+
+
+def _busy_stream(dev, buf, reps):
+    """Work on the context's stream that keeps it busy for a few ms."""
+    for _ in range(reps):
+        check(dev.lib.mbx_fill_u32(dev.ctx, buf.ptr, 7, buf.nbytes // 4))
+
+
+def test_submit_poll_wait_cancel(dev, tables):
+    """The live API.  The context's stream is kept busy, so the first request
+    (whose output fills the budget) cannot finish before the others are
+    submitted: the second is then deterministically queued, the third is
+    cancelled while queued, and both others end with the results they get
+    alone.  The requests also wait for the caller's earlier work."""
+    sch, bins = tables
+    specs = configs.config3_queries(3, seed=41)
+    qs = _queries(sch, bins, specs)
+    alone = _alone(dev, qs, specs)
+    cfg = abi.ServeCfg(2, 4, 0, 0, 0, abi.Tune())
+    fixed = C.c_uint64()
+    check(dev.lib.mbx_serve_fixed_mem(C.byref(cfg), C.byref(fixed)))
+    need = [(8 * s.nr_grp + 255) // 256 * 256 for s in specs]
+    cfg.mem_budget = fixed.value + max(need[:2])   # one request at a time
+    srv = C.c_void_p()
+    check(dev.lib.mbx_serve_open(dev.ctx, C.byref(cfg), C.byref(srv)))
+    buf = dev.alloc(1 << 30)
+    try:
+        _busy_stream(dev, buf, 64)
+        outs = [np.full(s.nr_grp, -7, np.int64) for s in specs]
+        tk = [C.c_uint64() for _ in specs]
+        for i in range(3):
+            check(dev.lib.mbx_serve_submit(srv, C.byref(qs[i]), outs[i].ctypes.data,
+                                           C.byref(tk[i])))
+        state, lat = C.c_int(), C.c_double()
+        check(dev.lib.mbx_serve_poll(srv, tk[1].value, C.byref(state), C.byref(lat)))
+        assert state.value == abi.REQ_QUEUED and lat.value < 0
+        check(dev.lib.mbx_serve_cancel(srv, tk[2].value))
+        check(dev.lib.mbx_serve_poll(srv, tk[2].value, C.byref(state), None))
+        assert state.value == abi.REQ_CANCELLED
+        for i in (0, 1):
+            check(dev.lib.mbx_serve_wait(srv, tk[i].value, -1.0, C.byref(state),
+                                         C.byref(lat)))
+            assert state.value == abi.REQ_DONE and lat.value > 0
+            np.testing.assert_array_equal(outs[i], alone[i])
+        # a finished request cannot be cancelled
+        assert dev.lib.mbx_serve_cancel(srv, tk[0].value) == abi.ERR_INVALID
+        assert np.all(outs[2] == -7)             # never written
+    finally:
+        st = abi.ServeStats()
+        check(dev.lib.mbx_serve_close(srv, C.byref(st)))
+        buf.free()
+    assert st.completed == 2 and st.cancelled == 1 and st.admission_waits >= 1
+
+
+def test_serve_after_unsynced_bitmap_creation(dev):
+    """Bitmaps built on the context's stream without a sync, then served at
+    once: every request must see the finished bitmaps (oracle)."""
+    n = 2_000_003
+    host = configs.config1_host(n, seed=4242)
+    sch = S.Schema(n)
+    for k, v in host.items():
+        sch.add(k, v, dev.upload(v))
+    dev.sync()
+    b = list(configs.C1_BOUNDS)
+    ptrs = {}
+    for c in configs.C1_COLS:      # consecutive bins, no FK: no wait inside
+        ref = S.colref(sch, S.Src(c))
+        outs = (C.c_void_p * 40)()
+        check(dev.lib.mbx_create_bin(dev.ctx, C.byref(ref), n,
+                                     (C.c_uint64 * 40)(*b[:-1]),
+                                     (C.c_uint64 * 40)(*b[1:]), 40, outs, None))
+        ptrs[c] = [outs[j] for j in range(40)]
+    specs = configs.config3_queries(24, seed=43)
+    qs = (abi.Query * len(specs))()
+    for i, s in enumerate(specs):
+        qs[i].prg = S.to_vprg(configs.bin_cols(s, ptrs), n, s.instrs)
+        qs[i].op = S.to_rowop(s, sch)
+        qs[i].nr_grp = s.nr_grp
+    cfg = abi.ServeCfg(2, 4, 0, 0, 0, abi.Tune())
+    mixed, lat, st = _replay(dev, qs, np.zeros(len(specs)), cfg, specs)
+    assert st.completed == len(specs)
+    for i, s in enumerate(specs):
+        np.testing.assert_array_equal(mixed[i], cpuref.query(s, sch)[0],
+                                      err_msg=s.name)
+    for c in configs.C1_COLS:
+        check(dev.lib.mbx_free(dev.ctx, ptrs[c][0]))
+    for c in sch.cols.values():
+        c.dev.free()
+
+
+def test_two_streams_one_context(dev, tables):
+    """Queries of one context on two streams (mbx_set_stream), never
+    synchronised in between, run concurrently; each stream's launches use
+    their own flush replicas, so every sum equals the oracle's."""
+    sch, bins = tables
+    hip = C.CDLL("libamdhip64.so.7")
+    streams = [C.c_void_p(), C.c_void_p()]
+    for s in streams:
+        assert hip.hipStreamCreateWithFlags(C.byref(s), 1) == 0   # non-blocking
+    specs = configs.config3_queries(40, seed=44)
+    qs = _queries(sch, bins, specs)
+    outs = [dev.alloc(8 * s.nr_grp, np.int64, s.nr_grp) for s in specs]
+    try:
+        for rep in range(3):
+            for i, s in enumerate(specs):
+                check(dev.lib.mbx_set_stream(dev.ctx, streams[i % 2]))
+                check(dev.lib.mbx_prog_grpby(dev.ctx, C.byref(qs[i].prg),
+                                             C.byref(qs[i].op), s.nr_grp,
+                                             outs[i].ptr,
+                                             abi.PLAN_FUSED | abi.PLAN_EAGER))
+            for st_ in streams:
+                check(dev.lib.mbx_set_stream(dev.ctx, st_))
+                dev.sync()
+            for i, s in enumerate(specs):
+                np.testing.assert_array_equal(outs[i].host(),
+                                              cpuref.query(s, sch)[0],
+                                              err_msg=f"{s.name} rep {rep}")
+    finally:
+        check(dev.lib.mbx_set_stream(dev.ctx, None))
+        for o in outs:
+            o.free()
+        for s in streams:
+            hip.hipStreamDestroy(s)
+
+
+def test_config5_full(dev):
+    """Config 5 at full size: 600M rows x 10 Zipf(1.2) columns and their 640
+    bucket bitmaps resident; 64 requests from the config-3 program space
+    under the static heuristic plans, Poisson arrivals at 2,000 req/s, two
+    slots of up to four: every result in the mix equals the request run
+    alone, and eight of them equal the oracle over all 600M rows."""
+    n = configs.C5_ROWS
+    print(f"config 5: {n:,} rows", flush=True)
+    sch = configs.config5_device(dev)
+    bins, slabs = configs.build_bins(dev, sch, configs.C5_BOUNDS)
+    specs = configs.config5_queries(64)
+    qs = configs.heuristic_queries(
+        dev, configs.query_array(sch, specs, bins, configs.C5_BOUNDS))
+    try:
+        alone = []
+        for i, s in enumerate(specs):
+            out = dev.alloc(8 * s.nr_grp, np.int64, s.nr_grp)
+            check(dev.lib.mbx_prog_grpby(dev.ctx, C.byref(qs[i].prg),
+                                         C.byref(qs[i].op), s.nr_grp, out.ptr,
+                                         abi.PLAN_FUSED | qs[i].flags))
+            dev.sync()
+            alone.append(out.host())
+            out.free()
+        rng = np.random.default_rng(5)
+        arrivals = np.cumsum(rng.exponential(1 / 2000.0, len(specs)))
+        cfg = abi.ServeCfg(2, 4, 0, 0, 0, abi.Tune(8192, 4, 4096, 0))
+        mixed, lat, st = _replay(dev, qs, arrivals, cfg, specs)
+        assert st.completed == len(specs)
+        for i, s in enumerate(specs):
+            np.testing.assert_array_equal(mixed[i], alone[i], err_msg=s.name)
+        check_idx = list(range(0, len(specs), 8))
+        names = set().union(*(configs.spec_columns(specs[i]) for i in check_idx))
+        host = configs.host_subset(sch, names)
+        for i in check_idx:
+            print(f"  oracle {specs[i].name}", flush=True)
+            np.testing.assert_array_equal(alone[i],
+                                          cpuref.query(specs[i], host)[0],
+                                          err_msg=specs[i].name)
+    finally:
+        for a in slabs + [c.dev for c in sch.cols.values()]:
+            a.free()
