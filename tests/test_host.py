@@ -178,6 +178,49 @@ def test_recipe_plans(lib):
                         assert len(edge) == 1
 
 
+# Known answers of the recipe bin plans, worked out by hand from
+# bmpidx_create.cu:228-316 (step = (max - min) / 3 in integer arithmetic):
+# many_or splits [min, max) into [min, min+s), [min+s, min+2s), [min+2s, max)
+# (the remainder goes to the last bin); candchk widens column 0's first bin
+# to the left by s/2 (down to 0: max(min, s/2) - s/2) as its leftmost edge
+# bin, and column 1's last bin to the right by s/2 as its rightmost edge bin,
+# whose middle bins then come first; a step of 0 gives one bin, widened by 1
+# for candchk.  Slots: 0..3 middle, 4 leftmost, 5 rightmost.
+RECIPE_CASES = [
+    # kind, column, min, max, [(lo, hi, slot)]
+    (0, 0, 3, 9, [(3, 9, 0)]),
+    (0, 1, 0, 1, [(0, 1, 0)]),
+    (1, 0, 10, 40, [(10, 20, 0), (20, 30, 1), (30, 40, 2)]),
+    (1, 3, 10, 41, [(10, 20, 0), (20, 30, 1), (30, 41, 2)]),
+    (1, 0, 10, 42, [(10, 20, 0), (20, 30, 1), (30, 42, 2)]),
+    (1, 2, 5, 7, [(5, 7, 0)]),
+    (1, 0, 0, 21, [(0, 7, 0), (7, 14, 1), (14, 21, 2)]),
+    (2, 0, 10, 40, [(5, 20, 4), (20, 30, 0), (30, 40, 1)]),
+    (2, 0, 10, 41, [(5, 20, 4), (20, 30, 0), (30, 41, 1)]),
+    (2, 0, 2, 32, [(0, 12, 4), (12, 22, 0), (22, 32, 1)]),
+    (2, 0, 0, 21, [(0, 7, 4), (7, 14, 0), (14, 21, 1)]),
+    (2, 0, 100, 121, [(97, 107, 4), (107, 114, 0), (114, 121, 1)]),
+    (2, 0, 4, 6, [(4, 7, 4)]),
+    (2, 1, 10, 40, [(30, 45, 5), (10, 20, 0), (20, 30, 1)]),
+    (2, 1, 10, 41, [(30, 46, 5), (10, 20, 0), (20, 30, 1)]),
+    (2, 1, 0, 21, [(14, 24, 5), (0, 7, 0), (7, 14, 1)]),
+    (2, 1, 4, 6, [(3, 6, 5)]),
+    (2, 1, 0, 2, [(0, 2, 5)]),    # deviation: min - 1 clamped at 0
+    (2, 2, 10, 40, [(10, 20, 0), (20, 30, 1), (30, 40, 2)]),
+    (2, 3, 5, 7, [(5, 7, 0)]),
+]
+
+
+@pytest.mark.parametrize("kind,col,mn,mx,want", RECIPE_CASES)
+def test_recipe_known_answers(lib, kind, col, mn, mx, want):
+    lo_a, hi_a = (C.c_uint32 * 6)(), (C.c_uint32 * 6)()
+    slot, n = (C.c_int * 6)(), C.c_int()
+    assert lib.mbx_recipe_plan(kind, col, mn, mx, lo_a, hi_a, slot,
+                               C.byref(n)) == abi.OK
+    got = [(lo_a[i], hi_a[i], slot[i]) for i in range(n.value)]
+    assert got == want
+
+
 def test_generators(lib):
     """Deterministic, in range; Zipf(1.0) frequencies follow (v+1)^-1/H by a
     chi-square test; uniform is flat."""
