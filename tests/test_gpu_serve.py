@@ -93,7 +93,9 @@ def test_same_alone_and_in_the_mix(dev, tables, mode):
 
 def test_admission_by_memory(dev, tables):
     """A tight budget makes requests wait for memory (outputs are freed as
-    batches finish); a request larger than the whole budget is refused."""
+    batches finish); a request larger than the whole budget is refused.  The
+    context's stream is kept busy first, so no request can finish (and free
+    its output) before all of them have arrived: the budget binds for sure."""
     sch, bins = tables
     specs = configs.config3_queries(40, seed=32)
     huge = specs[3]
@@ -110,7 +112,10 @@ def test_admission_by_memory(dev, tables):
     check(dev.lib.mbx_serve_fixed_mem(C.byref(cfg), C.byref(fixed)))
     budget = fixed.value + 64 * 1024
     cfg.mem_budget = budget
+    buf = dev.alloc(1 << 30)
+    _busy_stream(dev, buf, 64)
     mixed, lat, st = _replay(dev, qs, arrivals, cfg, specs)
+    buf.free()
     assert lat[3] < 0 and st.rejected == 1
     assert st.completed == len(specs) - 1
     assert 0 < st.admission_waits < len(specs)   # requests, not passes
