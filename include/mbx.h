@@ -198,6 +198,19 @@ enum mbx_flags {
 int mbx_prog_grpby(mbx_ctx *ctx, const mbx_vprg *prg, const mbx_rowop *op,
                    uint32_t nr_grp, int64_t *d_out, uint32_t flags);
 
+/* No reference counterpart: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, plans and results as mbx_prog_grpby, but it
+ * calls no hipMalloc, hipFree or pool allocation: its scratch (flush
+ * replicas; the materialise plan's bitmaps and row list) is one 64 MiB
+ * device array per GPU, set up by mbx_init.  The materialise plan takes at
+ * most 15,294,884 rows (MBX_ERR_CAPACITY above).  Every call on a GPU
+ * shares that GPU's array: calls that overlap on one GPU (other streams,
+ * other contexts, other threads) race -- serialise them. */
+int mbx_prog_grpby_noalloc(mbx_ctx *ctx, const mbx_vprg *prg,
+                           const mbx_rowop *op, uint32_t nr_grp, int64_t *d_out,
+                           uint32_t flags);
+
 /* The materialise stages, separately callable (primitive_abla.cuh:262-394).
  * mbx_prog_eval writes ceil(N/64)*2 words each of possible/uncertain
  * (d_unc may be NULL for a program without edge bins); it returns nothing

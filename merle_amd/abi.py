@@ -138,6 +138,8 @@ PROTOTYPES = {
     "mbx_create_join": (_i32, [_vp, _vp, _vp, _u32, _u32, _u32, _u32, _u32,
                                _P(_vp)]),
     "mbx_prog_grpby": (_i32, [_vp, _P(Vprg), _P(RowOp), _u32, _vp, _u32]),
+    "mbx_prog_grpby_noalloc": (_i32, [_vp, _P(Vprg), _P(RowOp), _u32, _vp,
+                                      _u32]),
     "mbx_prog_eval": (_i32, [_vp, _P(Vprg), _vp, _vp]),
     "mbx_compact": (_i32, [_vp, _vp, _vp, _u32, _vp, _vp]),
     "mbx_rows_grpby": (_i32, [_vp, _vp, _vp, _P(Vprg), _P(RowOp), _u32, _vp,

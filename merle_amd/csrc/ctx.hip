@@ -2,6 +2,7 @@
 #include <cstring>
 
 #include "mbx_internal.hpp"
+#include "query_launch.hpp"
 
 namespace mbx {
 
@@ -129,6 +130,9 @@ int mbx_init(int device, size_t pool_bytes, mbx_ctx **out) {
   }
   c->spread = static_cast<unsigned long long *>(p);
   c->spread_done = reinterpret_cast<uint32_t *>(static_cast<char *>(p) + rep_bytes);
+  // the no-allocation entry point's scratch on this device (a failure here
+  // only disables that entry point, which then reports it)
+  (void)noalloc_setup(c);
   *out = c;
   return MBX_OK;
 }

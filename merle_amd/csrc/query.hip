@@ -536,6 +536,78 @@ static int run_materialise(mbx_ctx *ctx, const QueryPlan &q) {
   return st;
 }
 
+// ------------------------------------------------- no-allocation variant
+// Callers that cannot allocate device memory (code under graph capture,
+// tight serving loops, harnesses that forbid it) run mbx_prog_grpby_noalloc:
+// its scratch -- the flush replicas of the LDS group tables and the
+// materialise plan's possible / uncertain bitmaps, row count and row list --
+// lives in one file-scope device array per GPU instead of the pool.
+constexpr size_t kNoAllocScratch = size_t(64) << 20;
+
+// Scratch layout: flush replicas (tables, then counters) first, then the
+// materialise region.
+static size_t noalloc_rep_bytes() { return ceil_div(spread_bytes(), 256) * 256; }
+
+// Bytes of the materialise region for n rows: possible and uncertain
+// bitmaps, the row count (256 B), the row list.
+static size_t noalloc_mat_bytes(size_t n) {
+  return ceil_div(n, 64) * 8 * 2 + 256 + n * 4;
+}
+
+size_t noalloc_max_rows(size_t scratch_bytes) {
+  const size_t room = scratch_bytes - noalloc_rep_bytes();
+  size_t lo = 0, hi = room / 4;   // largest n with noalloc_mat_bytes(n) <= room
+  while (lo < hi) {
+    const size_t mid = (lo + hi + 1) / 2;
+    if (noalloc_mat_bytes(mid) <= room) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+}  // namespace mbx
+
+static __device__ __align__(256) unsigned char scratch[mbx::kNoAllocScratch];
+
+namespace {
+// device address of `scratch` on each GPU (the host-side name is only a
+// stand-in), found once per device by noalloc_setup
+struct NoAllocDev {
+  std::once_flag once;
+  unsigned char *ptr = nullptr;
+  int status = MBX_ERR_HIP;
+};
+constexpr int kMaxDevices = 64;
+NoAllocDev g_noalloc[kMaxDevices];
+}  // namespace
+
+namespace mbx {
+
+// Finds this device's copy of `scratch` and clears its replica tables (they
+// must start at zero; kernels leave them zero).  Called by mbx_init with the
+// context's device current; once per device.
+int noalloc_setup(mbx_ctx *ctx) {
+  if (ctx->device < 0 || ctx->device >= kMaxDevices) return MBX_ERR_INVALID;
+  NoAllocDev &d = g_noalloc[ctx->device];
+  std::call_once(d.once, [&] {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      d.status = MBX_ERR_HIP;
+      return;
+    }
+    int st = launch_fill_u32(ctx->stream, p, 0, noalloc_rep_bytes() / 4);
+    if (st == MBX_OK) {
+      e = hipStreamSynchronize(ctx->stream);
+      if (e != hipSuccess) st = MBX_ERR_HIP;
+    }
+    d.status = st;
+    if (st == MBX_OK) d.ptr = static_cast<unsigned char *>(p);
+  });
+  return d.status;
+}
+
 }  // namespace mbx
 
 using namespace mbx;
@@ -559,6 +631,48 @@ int mbx_prog_grpby(mbx_ctx *ctx, const mbx_vprg *prg, const mbx_rowop *op,
   case MBX_PLAN_SCAN: return launch_scan(ctx, q);
   default: return fail(MBX_ERR_INVALID, "unknown plan");
   }
+}
+
+int mbx_prog_grpby_noalloc(mbx_ctx *ctx, const mbx_vprg *prg,
+                           const mbx_rowop *op, uint32_t nr_grp, int64_t *d_out,
+                           uint32_t flags) {
+  QueryPlan q;
+  int st = prepare(ctx, prg, op, nr_grp, d_out, &q);
+  if (st) return st;
+  const NoAllocDev &d = g_noalloc[ctx->device];
+  if (!d.ptr)
+    return fail(MBX_ERR_HIP, "prog_grpby_noalloc: no scratch on this device");
+  unsigned char *base = d.ptr;   // the device address, never `scratch`
+  q.spread = reinterpret_cast<unsigned long long *>(base);
+  q.spread_done = reinterpret_cast<uint32_t *>(
+      base + (size_t)kSpreadRep * kLdsGroups * 8);
+  q.eager = (flags & MBX_PLAN_EAGER) != 0;
+  q.direct = (flags & MBX_PLAN_DIRECT) != 0;
+  const uint32_t plan = flags & MBX_PLAN_MASK;
+  if (plan == MBX_PLAN_MATERIALISE && q.prg.nrows > noalloc_max_rows(kNoAllocScratch))
+    return fail(MBX_ERR_CAPACITY,
+                "prog_grpby_noalloc: the materialise plan takes at most " +
+                    std::to_string(noalloc_max_rows(kNoAllocScratch)) +
+                    " rows in its " + std::to_string(kNoAllocScratch >> 20) +
+                    " MiB scratch");
+  if (plan > MBX_PLAN_SCAN) return fail(MBX_ERR_INVALID, "unknown plan");
+  if (!(flags & MBX_ACCUMULATE)) {
+    st = launch_fill_u64(ctx->stream, d_out, 0, nr_grp);
+    if (st) return st;
+  }
+  if (plan == MBX_PLAN_FUSED) return launch_fused(ctx, q, nullptr);
+  if (plan == MBX_PLAN_SCAN) return launch_scan(ctx, q);
+  const size_t nw = q.prg.nwords;
+  uint64_t *po = reinterpret_cast<uint64_t *>(base + noalloc_rep_bytes());
+  uint64_t *uo = po + nw;
+  uint32_t *count = reinterpret_cast<uint32_t *>(uo + nw);
+  uint32_t *rows = count + 64;
+  st = launch_stage1(ctx, q.prg, po, q.prg.chk ? uo : nullptr);
+  if (!st)
+    st = launch_stage2(ctx, po, q.prg.chk ? uo : nullptr, (uint32_t)nw, rows,
+                       count);
+  if (!st) st = launch_stage3(ctx, q, rows, count);
+  return st;
 }
 
 int mbx_prog_eval(mbx_ctx *ctx, const mbx_vprg *prg, uint32_t *d_poss,

@@ -36,6 +36,12 @@ int ctx_spread(mbx_ctx *ctx, hipStream_t s, unsigned long long **rep,
 // Bytes of one set of flush replicas (tables, then counters).
 size_t spread_bytes();
 
+// The no-allocation entry point's device scratch: found and cleared once per
+// device (mbx_init calls it with the device current).
+int noalloc_setup(mbx_ctx *ctx);
+// Rows the materialise plan can take in a scratch of that many bytes.
+size_t noalloc_max_rows(size_t scratch_bytes);
+
 // Validates the query and builds its device descriptors.
 int prepare(mbx_ctx *ctx, const mbx_vprg *prg, const mbx_rowop *op,
             uint32_t nr_grp, int64_t *d_out, QueryPlan *q);
