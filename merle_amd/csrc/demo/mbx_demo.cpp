@@ -2,7 +2,7 @@
 // (ssb_demoall, main.c:6-12; ssb/fixedlayout.cu, ssb/lambdas.cuh:154-243)
 // written against the mybmpidx mirror.
 //
-//   mbx_demo <ssb_dir> [reps] [synth_rows]
+//   mbx_demo <ssb_dir> [reps] [synth_rows] [sums_file]
 //
 // <ssb_dir> holds the binary column files of ssb-dbgen-lite -b (4-byte
 // header = element size in bytes, ssb-dbgen-lite.c:192-200).  The driver
@@ -11,17 +11,23 @@
 // part mfgr projected through their foreign keys), then runs the 13 flights
 // (the arguments of ssb_cpujoin, ssb/cpuref.c:244-302) under the Sparse,
 // Medium and Dense programs of ssb/fixedlayout.cu (col::from_bin and
-// col::from_bin2; edge bins -> candidate checks), timed per plan as the mean
-// of `reps` runs.  It prints the reference's five-block report
-// (drawFigs/drawFigs.py:15-46): index creation and the SSB block in full,
-// the method and synthetic blocks in full when synth_rows > 0 (seeded
-// columns of that many rows, see synth_suite), the WAH / RTScan block with
-// headers and n/a values only.  SSB columns: Join = the bitmap-free scan
-// (Sparse rows only, as DOWORK), Ours = fused under the library's static
-// plan, PrgStg1 / Stg2 / Stg3 = the three materialise stages; the plans
-// this library does not build (Basefus, NfuStg1, Dedicat) read 0.  Every
-// plan's int64 sums must equal the scan's, or the driver exits with
-// status 1.
+// col::from_bin2; edge bins -> candidate checks) and then under the
+// controlled layouts of ssb/controlled-demo.cu (MOREWORK: recipe::perfect,
+// many_or and candchk -- exact, three-way-split and misaligned bins built
+// per query with create_join), timed per plan as the mean of `reps` runs.
+// It prints the reference's five-block report (drawFigs/drawFigs.py:15-46):
+// index creation and the SSB block in full (BinType Sparse / Medium / Dense
+// / Perfect / AllAlig / MisAlig), the method and synthetic blocks in full
+// when synth_rows > 0 (seeded columns of that many rows, see synth_suite),
+// the WAH / RTScan block with its header and `nan` values (WAH and RTScan
+// are not built).  SSB columns: Join = the bitmap-free scan (Sparse rows
+// only, as DOWORK), Ours = fused under the library's static plan, PrgStg1 /
+// Stg2 / Stg3 = the three materialise stages; the plans this library does
+// not build (Basefus, NfuStg1, Dedicat) read 0.  Every plan's int64 sums
+// must equal the scan's, or the driver exits with status 1.  With a
+// sums_file the driver also writes every SSB row's sums there, one line per
+// (flight, bin type, plan): name, bin type, plan (join / ours / stg), then
+// the sums, tab-separated.
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -562,6 +568,18 @@ int main(int argc, char **argv) {
   const int reps = argc > 2 ? std::atoi(argv[2]) : 20;
   const uint32_t synth_rows =
       argc > 3 ? (uint32_t)std::strtoul(argv[3], nullptr, 10) : 0;
+  FILE *sums = nullptr;
+  if (argc > 4 && !(sums = std::fopen(argv[4], "w"))) {
+    std::fprintf(stderr, "cannot write %s\n", argv[4]);
+    return 2;
+  }
+  auto write_sums = [&](const char *name, const char *bt, const char *plan,
+                        const int64_t *v, uint32_t g) {
+    if (!sums) return;
+    std::fprintf(sums, "%s\t%s\t%s", name, bt, plan);
+    for (uint32_t i = 0; i < g; ++i) std::fprintf(sums, "\t%lld", (long long)v[i]);
+    std::fprintf(sums, "\n");
+  };
   mbx_ctx *ctx = context();
   if (!ctx) {
     std::fprintf(stderr, "no device: %s\n", mbx_last_error());
@@ -583,7 +601,7 @@ int main(int argc, char **argv) {
   // ---------------------------------------------- block 1: index creation
   std::vector<uint *> bins[2][NSRC];
   std::printf("Case\tWAH\tRTScan\n");
-  for (const Flight &f : kFlights) std::printf("%s\t99999.9\t99999.9\n", f.name);
+  for (const Flight &f : kFlights) std::printf("%s\tnan\tnan\n", f.name);
   std::printf("\nColumn\tMy(ms)\tWAH(ms)\tMy(MB)\tWAH(MB)\n");
   const char *kSet[2] = {"Sparse", "Dense"};
   for (int d = 0; d < 2; ++d) {
@@ -691,6 +709,8 @@ int main(int argc, char **argv) {
       mbx_memcpy_d2h(ctx, h_want.data(), want, 8 * g);
       mbx_memcpy_d2h(ctx, h_got.data(), out, 8 * g);
       bad += std::memcmp(h_want.data(), h_got.data(), 8 * g) != 0;
+      write_sums(f.name, kPlan[level], "join", h_want.data(), g);
+      write_sums(f.name, kPlan[level], "ours", h_got.data(), g);
       const float s1 = timed([&] {
         return mbx_prog_eval(ctx, &ap, poss, p.is_nochk() ? nullptr : unc);
       });
@@ -703,6 +723,7 @@ int main(int argc, char **argv) {
       });
       mbx_memcpy_d2h(ctx, h_got.data(), out, 8 * g);
       bad += std::memcmp(h_want.data(), h_got.data(), 8 * g) != 0;
+      write_sums(f.name, kPlan[level], "stg", h_got.data(), g);
       if (level == 0)
         std::printf("\n%s\t%s\t%.4f", f.name, kPlan[level], join);
       else
@@ -710,6 +731,82 @@ int main(int argc, char **argv) {
       std::printf("\t%.4f\t0\t0\t%.4f\t%.4f\t%.4f\t0", ours, s1, s2, s3);
     }
   }
+  // the controlled layouts (ssb/controlled-demo.cu:7-129): each flight's
+  // predicates get bins made for the query by the recipes -- Perfect (one
+  // exact bitmap per column), AllAlig (many_or: three aligned bins) and
+  // MisAlig (candchk: edge bins widened by step / 2 on columns 0 and 1) --
+  // as one conjunction of all columns; flight 4 drops its date column when
+  // the range spans every date (controlled-demo.cu:104-105)
+  const char *kRecipe[3] = {"Perfect", "AllAlig", "MisAlig"};
+  for (const Flight &f : kFlights) {
+    mbx_rowop op;
+    std::vector<Pred> pr = flight_op(f, t, op);
+    if (f.kind == 1 && pr[2].lo == 0) pr[2].lo = 1;   // quantity >= 1
+    if (f.kind == 4 && f.a[6] <= D920101 && f.a[7] >= D990101) pr.pop_back();
+    recipe r;
+    r.factsz = t.nrows;
+    std::vector<vprg::instr> rins;
+    for (size_t i = 0; i < pr.size(); ++i) {
+      const mbx_colref c = t.ref(pr[i].src);
+      r.nbit[i] = 8 * c.elem_bytes;
+      r.min[i] = (uint)pr[i].lo;
+      r.max[i] = (uint)pr[i].hi;
+      r.dimsz[i] = recipe::nodim;   // keys are in range (ssb-dbgen-lite)
+      r.fk[i] = c.fk;
+      r.attr[i] = c.data;
+      rins.push_back(ins(i == 0 ? vprg::ORM : vprg::ANDM, 0, (uint8_t)i));
+      op.preds[i].src = c;
+      op.preds[i].lo = pr[i].lo;
+      op.preds[i].hi = pr[i].hi;
+    }
+    op.npreds = (uint32_t)pr.size();
+    vprg::instr instrs[MAXNINSTR];
+    for (size_t i = 0; i < rins.size(); ++i) instrs[i] = rins[i];
+    const uint32_t g = f.ngrp;
+    for (int kind = 0; kind < 3; ++kind) {
+      vprg p = kind == 0 ? r.perfect(instrs)
+               : kind == 1 ? r.many_or(instrs) : r.candchk(instrs);
+      for (size_t i = 0; i < pr.size(); ++i)
+        if (p.cols[i].empty()) {
+          std::fprintf(stderr, "%s %s: recipe failed: %s\n", f.name,
+                       kRecipe[kind], mbx_last_error());
+          return 2;
+        }
+      const mbx_vprg ap = p.to_abi();
+      if (mbx_prog_grpby(ctx, &ap, &op, g, want, MBX_PLAN_SCAN) != MBX_OK) {
+        std::fprintf(stderr, "scan: %s\n", mbx_last_error());
+        return 2;
+      }
+      mbx_plan plan;
+      if (!static_plan(ctx, ap, plan)) return 2;
+      const float ours = timed([&] {
+        return mbx_prog_grpby_plan(ctx, &ap, &op, g, out, &plan);
+      });
+      mbx_memcpy_d2h(ctx, h_want.data(), want, 8 * g);
+      mbx_memcpy_d2h(ctx, h_got.data(), out, 8 * g);
+      bad += std::memcmp(h_want.data(), h_got.data(), 8 * g) != 0;
+      write_sums(f.name, kRecipe[kind], "join", h_want.data(), g);
+      write_sums(f.name, kRecipe[kind], "ours", h_got.data(), g);
+      const float s1 = timed([&] {
+        return mbx_prog_eval(ctx, &ap, poss, p.is_nochk() ? nullptr : unc);
+      });
+      const float s2 = timed([&] {
+        return mbx_compact(ctx, poss, p.is_nochk() ? nullptr : unc, t.nrows,
+                           rows, count);
+      });
+      const float s3 = timed([&] {
+        return mbx_rows_grpby(ctx, rows, count, &ap, &op, g, out, 0);
+      });
+      mbx_memcpy_d2h(ctx, h_got.data(), out, 8 * g);
+      bad += std::memcmp(h_want.data(), h_got.data(), 8 * g) != 0;
+      write_sums(f.name, kRecipe[kind], "stg", h_got.data(), g);
+      std::printf("\n%s\t%s\t0\t%.4f\t0\t0\t%.4f\t%.4f\t%.4f\t0", f.name,
+                  kRecipe[kind], ours, s1, s2, s3);
+      mbx_sync(ctx);
+      p.release();
+    }
+  }
+  if (sums) std::fclose(sums);
   // blocks 3 (method 1 / method 2) and 4 (the synthetic suite) when
   // synth_rows > 0, else their headers alone
   std::printf("\n\nSeletiv\tMethod1\tMethod2\n");

@@ -197,19 +197,40 @@ def test_dbgen_flights_on_gpu(dev, ssb_host):
 def test_demo_driver_report(ssb_host, tmp_path_factory):
     """The C++ driver (merle_amd/lib/mbx_demo, the reference's ssb_demoall
     on the mybmpidx mirror) runs every flight's Sparse / Medium / Dense
-    programs, method 1 (no compaction) against method 2 and the synthetic
-    suite (perfect / many-OR / candidate-check recipes on seeded Zipf
-    columns), checks each plan against the scan plan or the other method
-    (exit status 0) and prints the five-block report."""
+    programs and its Perfect / AllAlig / MisAlig controlled layouts
+    (recipes built per query), method 1 (no compaction) against method 2 and
+    the synthetic suite (perfect / many-OR / candidate-check recipes on
+    seeded Zipf columns), checks each plan against the scan plan or the other
+    method (exit status 0) and prints the five-block report.  The sums it
+    writes for every SSB row and plan equal the oracle's straight loops
+    (s1ref..s4ref restated) on the same data."""
     demo = os.path.join(ROOT, "merle_amd", "lib", "mbx_demo")
     if not os.path.exists(demo):
         pytest.skip("mbx_demo not built")
     d = _dbgen_dir(tmp_path_factory)
-    r = subprocess.run([demo, d, "3", "1000000"], capture_output=True, text=True,
-                       timeout=300)
+    sums = str(tmp_path_factory.mktemp("demo") / "sums.tsv")
+    r = subprocess.run([demo, d, "3", "1000000", sums], capture_output=True,
+                       text=True, timeout=300)
     assert r.returncode == 0, r.stderr
     blocks = r.stdout.strip("\n").split("\n\n")
     assert len(blocks) == 5
+    # block 0: WAH and RTScan are not built -- the values are nan
+    wah = blocks[0].splitlines()
+    assert wah[0].split("\t") == ["Case", "WAH", "RTScan"]
+    assert [row.split("\t")[0] for row in wah[1:]] == [q.name for q in ssbq.FLIGHTS]
+    assert all(row.split("\t")[1:] == ["nan", "nan"] for row in wah[1:])
+    # every SSB row's sums (scan, fused, materialise) == the straight loops
+    want = {q.name: _oracle(ssb_host, q) for q in ssbq.FLIGHTS}
+    seen = set()
+    with open(sums) as fh:
+        for line in fh:
+            name, bt, plan, *vals = line.rstrip("\n").split("\t")
+            np.testing.assert_array_equal(np.array(vals, np.int64), want[name],
+                                          err_msg=f"{name} {bt} {plan}")
+            seen.add((name, bt, plan))
+    bts = ("Sparse", "Medium", "Dense", "Perfect", "AllAlig", "MisAlig")
+    assert seen == {(q.name, bt, p) for q in ssbq.FLIGHTS for bt in bts
+                    for p in ("join", "ours", "stg")}
     # method 1 / method 2 (synth/cpuref.c:194-205): 7 selectivities
     meth = blocks[3].splitlines()
     assert meth[0].split("\t") == ["Seletiv", "Method1", "Method2"]
@@ -227,11 +248,16 @@ def test_demo_driver_report(ssb_host, tmp_path_factory):
         assert all(float(x) > 0 for x in f[3:16])
     ssb_rows = blocks[2].splitlines()
     assert ssb_rows[0].split("\t")[:4] == ["Case", "BinType", "Join", "Ours"]
-    assert len(ssb_rows) == 1 + 3 * len(ssbq.FLIGHTS)
+    nf = len(ssbq.FLIGHTS)
+    assert len(ssb_rows) == 1 + 6 * nf
     for i, row in enumerate(ssb_rows[1:]):
         f = row.split("\t")
-        assert f[0] == ssbq.FLIGHTS[i // 3].name
-        assert f[1] == ("Sparse", "Medium", "Dense")[i % 3]
+        fixed = i < 3 * nf
+        assert f[0] == ssbq.FLIGHTS[(i if fixed else i - 3 * nf) // 3].name
+        assert f[1] == (("Sparse", "Medium", "Dense") if fixed else
+                        ("Perfect", "AllAlig", "MisAlig"))[i % 3]
         assert len(f) == 10 and float(f[3]) > 0
-        # DOWORK times the scan on the Sparse row only (ssb/lambdas.cuh:157)
-        assert (float(f[2]) > 0) if i % 3 == 0 else f[2] == "0"
+        assert all(float(x) > 0 for x in f[6:9])   # the materialise stages
+        # DOWORK times the scan on the fixed layouts' Sparse row only
+        # (ssb/lambdas.cuh:157); the controlled rows print 0 there
+        assert (float(f[2]) > 0) if fixed and i % 3 == 0 else f[2] == "0"
