@@ -264,15 +264,48 @@ def run_batched(dev, reps=5, train_programs=1024):
     }
 
 
+def learned_queries(dev, sch, bins, bounds, qs, specs, train_specs):
+    """A copy of a query array with each request's plan picked by a cost
+    model trained on the engine-timed candidates of other requests over the
+    same tables (`train_specs`).  -> (queries, training note, plan kinds)."""
+    from merle_amd import abi, learned
+    from merle_amd.host import check
+    t0 = time.time()
+    X, Y = learned.collect(dev, sch, bins, 0, specs=train_specs, bounds=bounds)
+    model = learned.CostModel(dev)
+    model.train(X, Y)
+    out = (abi.Query * len(qs))()
+    C.memmove(out, qs, C.sizeof(out))
+    kinds = {"fused": 0, "eager": 0, "scan": 0, "materialise": 0}
+    for i, s in enumerate(specs):
+        p = learned.Prepared(dev, sch, bins, s, bounds)
+        plan = model.choose(p)
+        applied = abi.Vprg()
+        check(dev.lib.mbx_plan_apply(C.byref(p.prg), C.byref(plan), C.byref(applied)))
+        out[i].prg = applied
+        out[i].flags = plan.flags & (abi.PLAN_MASK | abi.PLAN_EAGER)
+        k = plan.flags & abi.PLAN_MASK
+        kinds["scan" if k == abi.PLAN_SCAN else "materialise"
+              if k == abi.PLAN_MATERIALISE else "eager"
+              if plan.flags & abi.PLAN_EAGER else "fused"] += 1
+    model.close()
+    note = (f"{len(train_specs)} other requests over the same tables, "
+            f"{X.shape[0]} timed plans, {time.time() - t0:.1f} s")
+    return out, note, kinds
+
+
 def run_serving(dev, nreq=1000, rates=(2000.0, 20000.0), budget=256 << 20,
-                tight=256 << 10):
+                tight=256 << 10, train_requests=600):
     """Config 5: 600M rows x 10 Zipf columns + 640 bucket bitmaps resident;
     Poisson arrivals replayed against the server at a moderate and at a
-    saturating rate, once with continuous batching (2 streams, up to 4
-    requests per dispatch, large requests as their own fused launches) and
-    once one request at a time; then once more at the saturating rate with a
-    budget of the server's fixed memory plus `tight` bytes, so admission
-    binds.  Every request's result must be identical in every replay."""
+    saturating rate, once with continuous batching (4 streams; up to 4 small
+    requests per dispatch, large and scan requests alone as launches of
+    their own) and once one request at a time, under the plans a cost model trained on other
+    requests picks (the scan for requests whose edge bins leave most rows to
+    re-check) and under the static heuristic plans; then once more at the
+    saturating rate with a budget of the server's fixed memory plus `tight`
+    bytes, so admission binds.  Every request's result must be identical in
+    every replay."""
     from merle_amd import abi, configs
     from merle_amd.host import check
     n = configs.C5_ROWS
@@ -282,18 +315,22 @@ def run_serving(dev, nreq=1000, rates=(2000.0, 20000.0), budget=256 << 20,
     log(f"config-5 tables and {sum(len(v) for v in bins.values())} bitmaps "
         f"ready in {time.time() - t0:.1f} s")
     specs = configs.config5_queries(nreq)
-    qs = configs.heuristic_queries(
-        dev, configs.query_array(sch, specs, bins, configs.C5_BOUNDS))
+    written = configs.query_array(sch, specs, bins, configs.C5_BOUNDS)
+    hq = configs.heuristic_queries(dev, written)
+    lq, lnote, lkinds = learned_queries(
+        dev, sch, bins, configs.C5_BOUNDS, written, specs,
+        configs.config5_training_queries(train_requests))
+    log(f"serving plans learned: {lkinds}")
     tot = sum(q.nr_grp for q in specs)
     # algorithmic bytes from a sample of the requests (the accounting kernel
     # is slow), scaled to all of them
     ns = min(nreq, 64)
     nbytes, nsel = C.c_uint64(), C.c_uint64()
-    check(dev.lib.mbx_alg_bytes_batch(dev.ctx, qs, ns, C.byref(nbytes),
+    check(dev.lib.mbx_alg_bytes_batch(dev.ctx, hq, ns, C.byref(nbytes),
                                       C.byref(nsel)))
     est = nbytes.value * nreq / ns
 
-    def replay(arrivals, cfg):
+    def replay(qs, arrivals, cfg):
         res = np.zeros(tot, np.int64)
         lat = np.zeros(nreq)
         st = abi.ServeStats()
@@ -317,7 +354,7 @@ def run_serving(dev, nreq=1000, rates=(2000.0, 20000.0), budget=256 << 20,
             "roofline_frac": round(gbs / HBM_PEAK_GBS, 4),
         }
 
-    batched_cfg = abi.ServeCfg(2, 4, budget, 0, 0, abi.Tune(8192, 4, 4096, 0))
+    batched_cfg = abi.ServeCfg(4, 4, budget, 0, 0, abi.Tune(8192, 4, 4096, 0))
     plain_cfg = abi.ServeCfg(1, 1, budget, 1, 0, abi.Tune())
     out, same, ref = {}, True, None
     for rate in rates:
@@ -325,28 +362,31 @@ def run_serving(dev, nreq=1000, rates=(2000.0, 20000.0), budget=256 << 20,
         arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
         arrivals -= arrivals[0]
         block = {}
-        for mode, cfg in (("batched", batched_cfg),
-                          ("one_at_a_time", plain_cfg)):
-            res, block[mode] = replay(arrivals, cfg)
-            ref = res if ref is None else ref
-            same &= bool(np.array_equal(res, ref))
-        block["batched_vs_one_at_a_time"] = round(
-            block["one_at_a_time"]["wall_s"] / block["batched"]["wall_s"], 3)
-        block["p99_batched_vs_one_at_a_time"] = round(
-            block["batched"]["p99_ms"] / block["one_at_a_time"]["p99_ms"], 3)
+        for plans, qs in (("learned", lq), ("heuristic", hq)):
+            sub = {}
+            for mode, cfg in (("batched", batched_cfg),
+                              ("one_at_a_time", plain_cfg)):
+                res, sub[mode] = replay(qs, arrivals, cfg)
+                ref = res if ref is None else ref
+                same &= bool(np.array_equal(res, ref))
+            sub["batched_vs_one_at_a_time"] = round(
+                sub["one_at_a_time"]["wall_s"] / sub["batched"]["wall_s"], 3)
+            sub["p99_batched_vs_one_at_a_time"] = round(
+                sub["batched"]["p99_ms"] / sub["one_at_a_time"]["p99_ms"], 3)
+            block[f"{plans}_plans"] = sub
         out[f"rate_{int(rate)}_per_s"] = block
     # admission binding: the fixed memory plus a few outputs
     fixed = C.c_uint64()
     check(dev.lib.mbx_serve_fixed_mem(C.byref(batched_cfg), C.byref(fixed)))
-    tight_cfg = abi.ServeCfg(2, 4, fixed.value + tight, 0, 0,
+    tight_cfg = abi.ServeCfg(4, 4, fixed.value + tight, 0, 0,
                              abi.Tune(8192, 4, 4096, 0))
     rng = np.random.default_rng(5)
     arrivals = np.cumsum(rng.exponential(1.0 / max(rates), nreq))
     arrivals -= arrivals[0]
-    res, tb = replay(arrivals, tight_cfg)
+    res, tb = replay(lq, arrivals, tight_cfg)
     same &= bool(np.array_equal(res, ref))
     tb["budget_MB"] = round((fixed.value + tight) / 1e6, 3)
-    out[f"rate_{int(max(rates))}_per_s_tight_budget"] = tb
+    out[f"rate_{int(max(rates))}_per_s_tight_budget_learned_plans"] = tb
     for a in slabs + [c.dev for c in sch.cols.values()]:
         a.free()
     return {
@@ -354,6 +394,7 @@ def run_serving(dev, nreq=1000, rates=(2000.0, 20000.0), budget=256 << 20,
                     f"64 bucket bitmaps per column resident; {nreq} requests per "
                     f"replay, Poisson arrivals, memory budget {budget >> 20} MiB "
                     f"(tight: fixed + {tight >> 10} KiB)",
+        "learned_plans": {"training": lnote, "kinds": lkinds},
         **out,
         "identical_results": same,
         "alg_bytes_note": f"estimated from the first {ns} requests",

@@ -240,7 +240,9 @@ typedef struct mbx_query {
   mbx_vprg prg;
   mbx_rowop op;
   uint32_t nr_grp;
-  uint32_t flags;          /* 0 or MBX_PLAN_EAGER (all operand loads up front) */
+  uint32_t flags;          /* 0 or MBX_PLAN_EAGER (all operand loads up front);
+                              serving also takes MBX_PLAN_SCAN (a scan launch
+                              of its own; other plans run fused) */
   int64_t *d_out;          /* device, nr_grp int64 */
 } mbx_query;
 
@@ -284,8 +286,9 @@ int mbx_default_tune(mbx_tune *t);
  * where they are and runs submitted queries on nstreams slots (a HIP stream
  * each): memory-based admission, then continuous batching -- whenever a slot
  * is free, up to max_batch admitted requests go out on it as one persistent
- * batched launch (requests of >= fused_rows fact rows as fused launches of
- * their own), while the other slots are still running.  plain = 1 is the
+ * batched launch, while the other slots are still running; a request of
+ * >= fused_rows fact rows, or with the scan plan, goes out alone as a launch
+ * of its own.  plain = 1 is the
  * one-at-a-time baseline (one slot, one fused launch per request).
  *
  * Memory: the server's fixed working memory (per slot: descriptors, work
@@ -391,18 +394,33 @@ typedef struct mbx_plan {
   uint8_t norder;
   uint8_t reorderable;      /* program is a pure conjunction */
   uint16_t pad_;
-  uint32_t flags;           /* MBX_PLAN_FUSED or MBX_PLAN_MATERIALISE */
+  uint32_t flags;           /* MBX_PLAN_FUSED (| MBX_PLAN_EAGER),
+                               MBX_PLAN_MATERIALISE or MBX_PLAN_SCAN */
 } mbx_plan;
 typedef struct mbx_mlp mbx_mlp;
 
 /* Fraction of rows in each column's operand (OR of its bins); columns
  * without bitmaps get 0.  Popcounts run on the device. */
 int mbx_col_density(mbx_ctx *ctx, const mbx_vprg *prg, double dens[MBX_MAXCOLS]);
+/* The same, plus edge[c]: the fraction of rows only in column c's edge bins
+ * (rows a candidate check must re-check); edge may be NULL. */
+int mbx_col_stats(mbx_ctx *ctx, const mbx_vprg *prg, double dens[MBX_MAXCOLS],
+                  double edge[MBX_MAXCOLS]);
 int mbx_plan_describe(const mbx_vprg *prg, mbx_plan *out);
 int mbx_plan_apply(const mbx_vprg *in, const mbx_plan *plan, mbx_vprg *out);
 int mbx_plan_features(const mbx_vprg *prg, const double *dens, uint32_t nr_grp,
                       const mbx_plan *plan, float *feat /* MBX_NFEAT */);
+/* With the edge fractions of mbx_col_stats (NULL = none known). */
+int mbx_plan_features_ex(const mbx_vprg *prg, const double *dens,
+                         const double *edge, uint32_t nr_grp,
+                         const mbx_plan *plan, float *feat /* MBX_NFEAT */);
+/* Every plan: all orders x {fused, fused eager, materialise}, and the scan. */
 int mbx_plan_enumerate(const mbx_vprg *prg, mbx_plan *out, int max, int *count);
+/* The plans the learned chooser scores (at most 15): the early-out plan for
+ * every ordered pair of leading columns (the rest sparsest first), and
+ * eager, materialise and scan, sparsest column first. */
+int mbx_plan_candidates(const mbx_vprg *prg, const double *dens, mbx_plan *out,
+                        int max, int *count);
 /* Static heuristic: ascending density, fused. */
 int mbx_plan_heuristic(const mbx_vprg *prg, const double *dens, mbx_plan *out);
 /* Learned chooser (cost-model MLP over every candidate); mlp == NULL falls
@@ -410,6 +428,12 @@ int mbx_plan_heuristic(const mbx_vprg *prg, const double *dens, mbx_plan *out);
 int mbx_plan_choose(mbx_ctx *ctx, mbx_mlp *mlp, const mbx_vprg *prg,
                     const double *dens, uint32_t nr_grp, mbx_plan *best,
                     float *predicted);
+/* With edge fractions, and the heuristic kept unless the model predicts a
+ * plan cheaper by more than `margin` (log runtime; 0 = always the model's). */
+int mbx_plan_choose_ex(mbx_ctx *ctx, mbx_mlp *mlp, const mbx_vprg *prg,
+                       const double *dens, const double *edge,
+                       uint32_t nr_grp, float margin, mbx_plan *best,
+                       float *predicted);
 int mbx_prog_grpby_plan(mbx_ctx *ctx, const mbx_vprg *prg, const mbx_rowop *op,
                         uint32_t nr_grp, int64_t *d_out, const mbx_plan *plan);
 

@@ -19,6 +19,7 @@ END, AND, OR, NOT, ANDM, ORM = range(6)
 KEY_IDENT, KEY_DIV, KEY_BUCKET = range(3)
 MEAS_COUNT, MEAS_COL, MEAS_MUL, MEAS_SUB = range(4)
 PLAN_FUSED, PLAN_MATERIALISE, PLAN_SCAN = 0, 1, 2
+PLAN_MASK = 3
 ACCUMULATE = 16
 PLAN_EAGER = 32
 PLAN_DIRECT = 64
@@ -177,6 +178,14 @@ PROTOTYPES = {
     "mbx_abi_sizes": (_i32, [_P(_sz), _i32]),
     # plans and the cost-model MLP
     "mbx_col_density": (_i32, [_vp, _P(Vprg), _P(C.c_double)]),
+    "mbx_col_stats": (_i32, [_vp, _P(Vprg), _P(C.c_double), _P(C.c_double)]),
+    "mbx_plan_features_ex": (_i32, [_P(Vprg), _P(C.c_double), _P(C.c_double),
+                                    _u32, _P(Plan), _P(C.c_float)]),
+    "mbx_plan_candidates": (_i32, [_P(Vprg), _P(C.c_double), _P(Plan), _i32,
+                                   _P(_i32)]),
+    "mbx_plan_choose_ex": (_i32, [_vp, _vp, _P(Vprg), _P(C.c_double),
+                                  _P(C.c_double), _u32, C.c_float, _P(Plan),
+                                  _P(C.c_float)]),
     "mbx_plan_describe": (_i32, [_P(Vprg), _P(Plan)]),
     "mbx_plan_apply": (_i32, [_P(Vprg), _P(Plan), _P(Vprg)]),
     "mbx_plan_features": (_i32, [_P(Vprg), _P(C.c_double), _u32, _P(Plan),

@@ -494,6 +494,36 @@ def config5_queries(nq, seed=SEED + 55):
     return [out[k] for k in order]
 
 
+def config5_training_queries(nq, seed=SEED + 56, heavy_share=0.2):
+    """Requests for training a plan chooser over the config-5 tables: the
+    config-5 program space, and a share of heavy ones -- a range starting
+    inside the first bucket of a column (its edge bin holds the Zipf head,
+    so most rows need a re-check) AND 1-3 wide ranges -- which the static
+    heuristic runs fused although the scan is cheaper."""
+    rng = np.random.default_rng(seed + 1)
+    nheavy = int(round(nq * heavy_share))
+    out = config5_queries(nq - nheavy, seed)
+    for h in range(nheavy):
+        npred = int(rng.integers(2, 5))
+        cols = rng.choice(len(C5_COLS), size=npred, replace=False)
+        preds = []
+        for i, c in enumerate(cols):
+            b = C5_BOUNDS[C5_COLS[c]]
+            if i == 0:
+                preds.append(Pred(Src(C5_COLS[c]), int(rng.integers(1, b[1])),
+                                  b[int(rng.integers(1, 4))]))
+            else:
+                preds.append(Pred(Src(C5_COLS[c]), 0, b[int(rng.integers(2, 5))]))
+        ins = [(abi.ORM, 0, 0)] + [(abi.ANDM, 0, i) for i in range(1, npred)]
+        c = int(rng.integers(len(C5_COLS)))
+        div = C5_BOUNDS[C5_COLS[c]][1] * 16
+        card = -(-C5_DOMAINS[c] // div)
+        out.append(QuerySpec(preds, ins, [Key(Src(C5_COLS[c]), abi.KEY_DIV, div, 0, 1)],
+                             card, abi.MEAS_COL, Src("z0"), None, f"h{h}"))
+    order = rng.permutation(len(out))
+    return [out[k] for k in order]
+
+
 # ------------------------------------------------------------ shared helpers
 def build_bins(dev, sch, bounds_of):
     """One create_bin_into slab per column: {col: [bin ptrs]}, [slabs]."""

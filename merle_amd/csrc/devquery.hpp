@@ -498,6 +498,36 @@ __device__ __forceinline__ bool row_passes(const DevPrg &P, const DevOp &O,
   return (P.truth >> m) & 1u;
 }
 
+// row_passes for E rows at once: every predicate load of the E rows is in
+// flight together (a row costs two round trips however many there are).
+// live[e] in: the row is to be checked; out: it passes.
+template <bool ACCT, bool CAND, int E>
+__device__ __forceinline__ void rows_pass(const DevPrg &P, const DevOp &O,
+                                          const uint32_t *row, bool *live,
+                                          const Acct *acct) {
+  const uint32_t known = CAND ? P.known : 0u;
+  uint32_t ix[E][MBX_MAXCOLS];
+#pragma unroll
+  for (int e = 0; e < E; ++e)
+#pragma unroll
+    for (int c = 0; c < MBX_MAXCOLS; ++c)
+      if (live[e] && c < (int)O.npreds && !((known >> c) & 1u))
+        ix[e][c] = src_index<ACCT>(O.pred[c], row[e], acct);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    if (!live[e]) continue;
+    uint32_t m = CAND ? P.known_val : 0u;
+#pragma unroll
+    for (int c = 0; c < MBX_MAXCOLS; ++c)
+      if (c < (int)O.npreds && !((known >> c) & 1u)) {
+        bool ok;
+        const uint64_t v = src_value<ACCT>(O.pred[c], ix[e][c], ok, acct);
+        m |= (ok && v >= O.lo[c] && v < O.hi[c]) ? 1u << c : 0u;
+      }
+    live[e] = (P.truth >> m) & 1u;
+  }
+}
+
 // x / kp.div with key_div_magic's multiplier and shifts: exact for every
 // 64-bit x, a few multiplies instead of the 64-bit division sequence.
 __device__ __forceinline__ uint64_t key_div(const DevKey &kp, uint64_t x) {
@@ -671,6 +701,28 @@ __device__ __forceinline__ void flush_groups(const Spread &sp,
                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// One row per lane into the sink, where atomics on one address serialise:
+// the rows of the wave's two leading keys are summed across the wave first
+// (one atomic per key); the remaining rows add directly.
+__device__ __forceinline__ void sink_wave(const GroupSink &sink, uint32_t key,
+                                          int64_t val, bool ok,
+                                          uint32_t nr_grp) {
+  bool pend = ok;
+#pragma unroll
+  for (int round = 0; round < 2; ++round) {
+    const unsigned long long act = __ballot(pend);
+    if (!act) break;
+    const int leader = (int)__builtin_ctzll(act);
+    const uint32_t lk = __shfl(key, leader, 64);
+    const bool mine = pend && key == lk;
+    const unsigned long long v =
+        wave_sum_u64(mine ? (unsigned long long)val : 0ull);
+    if (lane_id() == (uint32_t)leader) sink_add(sink, lk, (int64_t)v, nr_grp);
+    pend = pend && !mine;
+  }
+  if (pend) sink_add(sink, key, val, nr_grp);
+}
+
 // List entries: row id, MSB = uncertain (primitive.cuh:406-407).  E entries
 // per lane share their memory round trips.
 template <bool CHK, bool ACCT, int E>
@@ -704,23 +756,8 @@ __device__ __forceinline__ void process_entries(const DevPrg &P,
       if (ok[e]) sink_add(sink, key[e], val[e], nr_grp);
       continue;
     }
-    // HBM output: atomics on one address serialise, so the rows of the
-    // wave's two leading keys are summed across the wave first (one atomic
-    // per key); the remaining rows add directly.
-    bool pend = ok[e];
-#pragma unroll
-    for (int round = 0; round < 2; ++round) {
-      const unsigned long long act = __ballot(pend);
-      if (!act) break;
-      const int leader = (int)__builtin_ctzll(act);
-      const uint32_t lk = __shfl(key[e], leader, 64);
-      const bool mine = pend && key[e] == lk;
-      const unsigned long long v =
-          wave_sum_u64(mine ? (unsigned long long)val[e] : 0ull);
-      if (lane_id() == (uint32_t)leader) sink_add(sink, lk, (int64_t)v, nr_grp);
-      pend = pend && !mine;
-    }
-    if (pend) sink_add(sink, key[e], val[e], nr_grp);
+    // HBM output: atomics on one address serialise
+    sink_wave(sink, key[e], val[e], ok[e], nr_grp);
   }
 }
 
@@ -757,6 +794,62 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t x,
   return before + incl - x;
 }
 
+// ------------------------------------------------------------- scan words
+// The scan plan (grpby, primitive.cuh:16-48) as 64-row words: every row's
+// exact predicates, no bitmaps.  For each word of its range the wave loads
+// the 64 rows' predicate values lane by lane (one coalesced 256-byte access
+// per column) and a ballot of the rows' results is the word; lane j keeps
+// word j of each 64-word chunk, the layout eval_words produces, so the
+// compaction and row op that follow are the fused plan's.  kScanU words
+// have their loads in flight together, and no load sits behind a
+// lane-divergent branch (rows past the end read row 0 and are dropped).
+constexpr int kScanU = 4;
+
+template <int WPL>
+__device__ __forceinline__ void scan_words(const DevPrg &P, const DevOp &O,
+                                           uint32_t w0, uint32_t ws,
+                                           uint32_t wend, uint64_t (&po)[WPL]) {
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (int k = 0; k < WPL; ++k) {
+    uint64_t mine = 0;
+    const uint32_t wk = w0 + ws * k;   // the chunk's first word
+    for (uint32_t j0 = 0; j0 < 64u && wk + j0 < wend; j0 += kScanU) {
+      uint32_t row[kScanU];
+      bool in[kScanU];
+      uint32_t ix[kScanU][MBX_MAXCOLS];
+#pragma unroll
+      for (int u = 0; u < kScanU; ++u) {
+        const uint32_t w = wk + j0 + (uint32_t)u;
+        row[u] = w * 64u + lane;
+        in[u] = w < wend && row[u] < P.nrows;
+        const uint32_t r = in[u] ? row[u] : 0u;
+#pragma unroll
+        for (int c = 0; c < MBX_MAXCOLS; ++c)
+          if (c < (int)O.npreds)
+            ix[u][c] = O.pred[c].fk ? O.pred[c].fk[r] : r;
+      }
+#pragma unroll
+      for (int u = 0; u < kScanU; ++u) {
+        uint32_t m = 0;
+#pragma unroll
+        for (int c = 0; c < MBX_MAXCOLS; ++c) {
+          if (c >= (int)O.npreds) continue;
+          const DevSrc &sc = O.pred[c];
+          if (sc.fk && sc.dimsz == 0) continue;   // no dimension row at all
+          const bool ok = sc.fk == nullptr || ix[u][c] < sc.dimsz;
+          const uint64_t v = ld_elem(sc.data, sc.eb, ok ? ix[u][c] : 0u, false);
+          m |= (ok && v >= O.lo[c] && v < O.hi[c]) ? 1u << c : 0u;
+        }
+        const bool pass = in[u] && ((P.truth >> m) & 1u);
+        const uint64_t b = __ballot(pass);
+        if (lane == j0 + (uint32_t)u) mine = b;
+      }
+    }
+    po[k] = mine;
+  }
+}
+
 // ------------------------------------------------------------ tile pipeline
 
 // One wave's share: lane l holds the words w0 + l + k * ws (k < WPL) below
@@ -776,7 +869,10 @@ constexpr uint32_t kWaveList = kListCap / 4;
 // more than the row op itself.
 constexpr uint32_t kDenseRowsPerWord = 16;
 
-template <bool CHK, bool ACCT, int WPL, int EAGER, int E = 2, bool DIRECT = false>
+// SCAN: the words come from the rows' exact predicates (scan_words), not
+// from bitmaps.
+template <bool CHK, bool ACCT, int WPL, int EAGER, int E = 2, bool DIRECT = false,
+          bool SCAN = false>
 __device__ __forceinline__ void run_wave_range(const DevPrg &P, const DevOp &O,
                                                uint32_t nr_grp, uint32_t w0,
                                                uint32_t ws, uint32_t wend,
@@ -785,7 +881,14 @@ __device__ __forceinline__ void run_wave_range(const DevPrg &P, const DevOp &O,
                                                const Acct *acct) {
   const uint32_t lane = lane_id();
   uint64_t p[WPL], u[WPL];
-  eval_words<CHK, ACCT, EAGER, WPL>(P, w0 + lane, ws, wend, p, u, acct);
+  if constexpr (SCAN) {
+    static_assert(!CHK && !ACCT && !DIRECT, "the scan words are exact");
+    scan_words<WPL>(P, O, w0, ws, wend, p);
+#pragma unroll
+    for (int k = 0; k < WPL; ++k) u[k] = 0;
+  } else {
+    eval_words<CHK, ACCT, EAGER, WPL>(P, w0 + lane, ws, wend, p, u, acct);
+  }
   if (DIRECT) {
     // The ablation without compaction (abla::method1, primitive_abla.cuh:
     // 85-98): each lane walks the set bits of its own words, re-checks an
@@ -881,7 +984,10 @@ __device__ __forceinline__ void run_wave_range(const DevPrg &P, const DevOp &O,
     }
     wave_lds_sync();
     const uint32_t nl = min(kWaveList, total - base);
-    for (uint32_t j = lane; j < nl; j += 64 * E) {
+    // wave-uniform trip count: every lane takes part in the sink's wave
+    // reductions, also in a partly filled round
+    for (uint32_t j0 = 0; j0 < nl; j0 += 64 * E) {
+      const uint32_t j = j0 + lane;
       uint32_t ent[E];
       bool valid[E];
 #pragma unroll

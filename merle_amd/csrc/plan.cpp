@@ -4,7 +4,9 @@
 // (ORM r0 <- c_a ; ANDM r0 <- c_b ; ...): AND is commutative, so every order
 // selects the same rows and gives the same int64 sums, but the order decides
 // how many bitmap lines the ANDM early-out (primitive.cuh:215) skips, and the
-// plan picks the fused kernel or the three materialise stages.  The static
+// plan picks the fused kernel (early-out or eager loads), the three
+// materialise stages, or the bitmap-free scan (grpby, primitive.cuh:16-48),
+// which wins when edge bins leave most rows to re-check.  The static
 // heuristic puts the sparsest column first and always fuses; the learned
 // chooser scores every candidate with the cost-model MLP and takes the
 // cheapest, falling back to the heuristic when no model is given.
@@ -73,10 +75,23 @@ int mbx_plan_apply(const mbx_vprg *in, const mbx_plan *plan, mbx_vprg *out) {
   return MBX_OK;
 }
 
-// The 16 cost-model features of (program, densities, plan).  dens[c] is the
-// fraction of rows in column c's operand (OR of its bins).
-int mbx_plan_features(const mbx_vprg *p, const double *dens, uint32_t nr_grp,
-                      const mbx_plan *plan, float *f) {
+// The 16 cost-model features of (program, column statistics, plan).
+// dens[c] is the fraction of rows in column c's operand (OR of its bins),
+// edge[c] (NULL = none known) the fraction only in its edge bins.
+//   0-3   densities in plan order        4-7  running products (the share
+//                                              of rows still live)
+//   8     bins read / 24 + 0.5 if the program has edge bins
+//   9     log2(groups + 1) / 20          10   log2(rows + 1) / 32
+//   11    plan is materialise            12   log10(selectivity) / 9 + 1
+//   13    log10(uncertain share) / 9 + 1: rows an edge bin leaves to
+//         re-check, sum over columns of edge[c] x selectivity / dens[c]
+//   14    plan loads eagerly             15   plan is the scan
+// The selectivity is the program run over the densities as independent
+// probabilities (AND: product, OR: a + b - ab, NOT: 1 - a), so OR and
+// ANDNOT programs are priced by what they select.
+int mbx_plan_features_ex(const mbx_vprg *p, const double *dens,
+                         const double *edge, uint32_t nr_grp,
+                         const mbx_plan *plan, float *f) {
   if (!p || !dens || !plan || !f) return fail(MBX_ERR_INVALID, "features: null");
   std::memset(f, 0, sizeof(float) * MBX_NFEAT);
   uint8_t order[4];
@@ -89,35 +104,60 @@ int mbx_plan_features(const mbx_vprg *p, const double *dens, uint32_t nr_grp,
       if (p->cols[c].middle[0] || p->cols[c].leftmost || p->cols[c].rightmost)
         order[n++] = (uint8_t)c;
   }
-  double run = 1.0, lines = 0.0;
-  int nbins = 0, chk = 0, nins = 0;
+  auto clamp01 = [](double x) { return std::min(1.0, std::max(0.0, x)); };
+  double run = 1.0, unc = 0.0;   // run: product in plan order
+  int nbins = 0, chk = 0;
   for (int k = 0; k < n && k < 4; ++k) {
-    const double d = std::min(1.0, std::max(0.0, dens[order[k]]));
-    // share of 1024-row bitmap lines still live when this operand is read
-    lines += k == 0 ? 1.0 : 1.0 - std::pow(1.0 - run, 1024.0);
+    const double d = clamp01(dens[order[k]]);
     run *= d;
     f[k] = (float)d;
     f[4 + k] = (float)run;
   }
+  // the program over the densities: registers hold probabilities
+  double r[3] = {0, 0, 0};
+  for (int i = 0; i < MBX_MAXNINSTR && p->instrs[i].opcode != MBX_END; ++i) {
+    const mbx_instr &in = p->instrs[i];
+    const int d = in.dstreg % 3, sr = in.src1 % 3;
+    const double m = in.src1 < MBX_MAXCOLS ? clamp01(dens[in.src1]) : 0.0;
+    switch (in.opcode) {
+    case MBX_AND: r[d] *= r[sr]; break;
+    case MBX_OR: r[d] = r[d] + r[sr] - r[d] * r[sr]; break;
+    case MBX_NOT: r[d] = 1.0 - r[sr]; break;
+    case MBX_ANDM: r[d] *= m; break;
+    case MBX_ORM: r[d] = r[d] + m - r[d] * m; break;
+    default: break;
+    }
+  }
+  const double sel = clamp01(r[0]);
+  if (edge)
+    for (int c = 0; c < MBX_MAXCOLS; ++c) {
+      const double e = clamp01(edge[c]), d = clamp01(dens[c]);
+      if (e > 0 && d > 0) unc += e * std::min(1.0, sel / d);
+    }
   for (int c = 0; c < MBX_MAXCOLS; ++c) {
     for (int k = 0; k < MBX_MAXBIN_PERCOL; ++k) nbins += p->cols[c].middle[k] != 0;
     nbins += (p->cols[c].leftmost != 0) + (p->cols[c].rightmost != 0);
     chk |= (p->cols[c].leftmost || p->cols[c].rightmost);
   }
-  while (nins < MBX_MAXNINSTR && p->instrs[nins].opcode != MBX_END) ++nins;
-  f[8] = nins / 6.f + 0.5f * chk;
+  const uint32_t kind = plan->flags & MBX_PLAN_MASK;
+  f[8] = nbins / 24.f + 0.5f * chk;
   f[9] = (float)(std::log2((double)nr_grp + 1.0) / 20.0);
   f[10] = (float)(std::log2((double)p->factsz + 1.0) / 32.0);
-  f[11] = (plan->flags & MBX_PLAN_MASK) == MBX_PLAN_MATERIALISE ? 1.f : 0.f;
-  f[12] = (float)(std::log10(std::max(run, 1e-9)) / 9.0 + 1.0);
-  f[13] = nbins / 24.f;
+  f[11] = kind == MBX_PLAN_MATERIALISE ? 1.f : 0.f;
+  f[12] = (float)(std::log10(std::max(sel, 1e-9)) / 9.0 + 1.0);
+  f[13] = (float)(std::log10(std::max(std::min(unc, 1.0), 1e-9)) / 9.0 + 1.0);
   f[14] = (plan->flags & MBX_PLAN_EAGER) ? 1.f : 0.f;
-  f[15] = n ? (float)(lines / n) : 0.f;
+  f[15] = kind == MBX_PLAN_SCAN ? 1.f : 0.f;
   return MBX_OK;
 }
 
-// Every order of a conjunction x {fused, fused eager, materialise}; other
-// programs get their own order only.
+int mbx_plan_features(const mbx_vprg *p, const double *dens, uint32_t nr_grp,
+                      const mbx_plan *plan, float *f) {
+  return mbx_plan_features_ex(p, dens, nullptr, nr_grp, plan, f);
+}
+
+// Every order of a conjunction x {fused, fused eager, materialise}, plus
+// the scan (order-free); other programs get their own order only.
 int mbx_plan_enumerate(const mbx_vprg *p, mbx_plan *out, int max, int *count) {
   if (!p || !out || !count) return fail(MBX_ERR_INVALID, "enumerate: null");
   mbx_plan base;
@@ -136,8 +176,54 @@ int mbx_plan_enumerate(const mbx_vprg *p, mbx_plan *out, int max, int *count) {
       all.push_back(q);
     }
   } while (base.reorderable && std::next_permutation(perm.begin(), perm.end()));
+  mbx_plan sc = base;
+  sc.flags = MBX_PLAN_SCAN;
+  all.push_back(sc);
   *count = (int)all.size();
   if ((int)all.size() > max) return fail(MBX_ERR_CAPACITY, "enumerate: max too small");
+  std::copy(all.begin(), all.end(), out);
+  return MBX_OK;
+}
+
+// The candidates the chooser scores (and the engine times to train it).
+// Only the ANDM early-out depends on the order, and mostly through its
+// first two columns, so: the early-out plan for every ordered pair of
+// leading columns (the rest by ascending density), and one plan each --
+// sparsest column first -- for eager loads (the conjunction form does not
+// depend on the order), materialise (its first stage loads eagerly) and
+// the scan.  At most 4 x 3 + 3 = 15 plans.
+int mbx_plan_candidates(const mbx_vprg *p, const double *dens, mbx_plan *out,
+                        int max, int *count) {
+  if (!p || !dens || !out || !count) return fail(MBX_ERR_INVALID, "candidates: null");
+  mbx_plan base;
+  int st = mbx_plan_heuristic(p, dens, &base);
+  if (st) return st;
+  base.flags = MBX_PLAN_FUSED;
+  std::vector<mbx_plan> all;
+  const int n = base.reorderable ? base.norder : 0;
+  if (n >= 2) {
+    for (int a = 0; a < n; ++a)
+      for (int b = 0; b < n; ++b) {
+        if (a == b) continue;
+        mbx_plan q = base;
+        int k = 0;
+        q.order[k++] = base.order[a];
+        q.order[k++] = base.order[b];
+        for (int j = 0; j < n; ++j)
+          if (j != a && j != b) q.order[k++] = base.order[j];
+        all.push_back(q);
+      }
+  } else {
+    all.push_back(base);
+  }
+  for (uint32_t fl : {(uint32_t)(MBX_PLAN_FUSED | MBX_PLAN_EAGER),
+                      (uint32_t)MBX_PLAN_MATERIALISE, (uint32_t)MBX_PLAN_SCAN}) {
+    mbx_plan q = base;
+    q.flags = fl;
+    all.push_back(q);
+  }
+  *count = (int)all.size();
+  if ((int)all.size() > max) return fail(MBX_ERR_CAPACITY, "candidates: max too small");
   std::copy(all.begin(), all.end(), out);
   return MBX_OK;
 }
@@ -162,21 +248,33 @@ int mbx_plan_heuristic(const mbx_vprg *p, const double *dens, mbx_plan *out) {
   return MBX_OK;
 }
 
-// Learned chooser: scores all candidates with the MLP (output 0 = predicted
-// log runtime) on the device and returns the cheapest.  mlp == NULL falls
-// back to the static heuristic.
-int mbx_plan_choose(mbx_ctx *ctx, mbx_mlp *mlp, const mbx_vprg *p,
-                    const double *dens, uint32_t nr_grp, mbx_plan *best,
-                    float *pred) {
+// Learned chooser: scores the candidates (mbx_plan_candidates) with the MLP
+// (output 0 = predicted log runtime) on the device and returns the
+// cheapest -- unless the static heuristic's plan is predicted within
+// `margin` of it (log units: 0.1 = about 10 %), in which case the heuristic
+// plan stays: the heuristic is the fallback and the yardstick, and a
+// prediction that barely beats it is not worth a different plan.  mlp ==
+// NULL returns the heuristic plan.
+int mbx_plan_choose_ex(mbx_ctx *ctx, mbx_mlp *mlp, const mbx_vprg *p,
+                       const double *dens, const double *edge, uint32_t nr_grp,
+                       float margin, mbx_plan *best, float *pred) {
   if (!ctx || !p || !dens || !best) return fail(MBX_ERR_INVALID, "choose: null");
-  if (!mlp) return mbx_plan_heuristic(p, dens, best);
-  mbx_plan cand[80];
-  int n = 0;
-  int st = mbx_plan_enumerate(p, cand, 80, &n);
+  mbx_plan heur;
+  int st = mbx_plan_heuristic(p, dens, &heur);
   if (st) return st;
+  if (!mlp) {
+    *best = heur;
+    return MBX_OK;
+  }
+  mbx_plan cand[33];
+  int n = 0;
+  st = mbx_plan_candidates(p, dens, cand, 32, &n);
+  if (st) return st;
+  cand[n++] = heur;   // scored too, as the yardstick
   std::vector<float> feat((size_t)n * MBX_NFEAT);
   for (int i = 0; i < n; ++i) {
-    st = mbx_plan_features(p, dens, nr_grp, &cand[i], &feat[(size_t)i * MBX_NFEAT]);
+    st = mbx_plan_features_ex(p, dens, edge, nr_grp, &cand[i],
+                              &feat[(size_t)i * MBX_NFEAT]);
     if (st) return st;
   }
   void *buf = nullptr;
@@ -200,9 +298,17 @@ int mbx_plan_choose(mbx_ctx *ctx, mbx_mlp *mlp, const mbx_vprg *p,
   int bi = 0;
   for (int i = 1; i < n; ++i)
     if (y[i] < y[bi]) bi = i;
+  if (y[bi] > y[n - 1] - margin) bi = n - 1;   // not clearly better
   *best = cand[bi];
   if (pred) *pred = y[bi];
   return MBX_OK;
+}
+
+int mbx_plan_choose(mbx_ctx *ctx, mbx_mlp *mlp, const mbx_vprg *p,
+                    const double *dens, uint32_t nr_grp, mbx_plan *best,
+                    float *pred) {
+  return mbx_plan_choose_ex(ctx, mlp, p, dens, nullptr, nr_grp, 0.f, best,
+                            pred);
 }
 
 // Runs a query under a plan (same results for every plan).

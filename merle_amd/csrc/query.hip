@@ -51,24 +51,25 @@ __global__ __launch_bounds__(256) void fused_kernel(QArgs a) {
 }
 
 // ------------------------------------------------------------- scan plan
-// Every row, all predicates checked (op(row, chk=true)), no bitmaps.
-template <bool LDSG>
+// Every row, all predicates checked (op(row, chk=true)), no bitmaps.  The
+// rows are evaluated 64 at a time into words (scan_words: coalesced column
+// loads, a ballot per word), then compacted and run through the row op like
+// the fused plan's -- wave r takes the 64-word chunks r, r + R, ... as there.
+template <bool LDSG, int WPL>
 __global__ __launch_bounds__(256) void scan_kernel(QArgs a) {
+  __shared__ __align__(16) uint32_t s_list[4][kWaveList];
   GroupSink sink;
   sink.lds = LDSG;
   sink.out = a.out;
   if (LDSG)
     for (uint32_t i = threadIdx.x; i < a.nr_grp; i += 256) s_grp_dyn[i] = 0;
   __syncthreads();
-  const uint32_t n = a.prg.nrows;
-  for (uint32_t row = blockIdx.x * 256 + threadIdx.x; row < n;
-       row += gridDim.x * 256) {
-    if (!row_passes<false, false>(a.prg, a.op, row, nullptr)) continue;
-    uint32_t key;
-    int64_t val;
-    if (row_keyval<false>(a.op, row, a.nr_grp, key, val, nullptr))
-      sink_add(sink, key, val, a.nr_grp);
-  }
+  const uint32_t wv = threadIdx.x >> 6;
+  const uint32_t ws = gridDim.x * 4 * 64;   // words between a wave's chunks
+  for (uint32_t w0 = (blockIdx.x * 4 + wv) * 64; w0 < a.prg.nwords;
+       w0 += ws * WPL)
+    run_wave_range<false, false, WPL, 0, 2, false, true>(
+        a.prg, a.op, a.nr_grp, w0, ws, a.prg.nwords, sink, s_list[wv], nullptr);
   if (LDSG) flush_groups(a.sp, a.out, a.nr_grp, blockIdx.x, gridDim.x);
 }
 
@@ -158,9 +159,15 @@ __global__ __launch_bounds__(256) void stage3_kernel(
     for (uint32_t i = threadIdx.x; i < a.nr_grp; i += 256) s_grp_dyn[i] = 0;
   __syncthreads();
   const uint32_t n = *count;
-  for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j < n;
-       j += gridDim.x * 256)
-    process_entry<CHK, false>(a.prg, a.op, a.nr_grp, rows[j], sink, nullptr);
+  // workgroup-uniform trip count: every lane takes part in the sink's wave
+  // reductions
+  for (uint32_t j0 = blockIdx.x * 256; j0 < n; j0 += gridDim.x * 256) {
+    const uint32_t j = j0 + threadIdx.x;
+    const bool valid = j < n;
+    const uint32_t e = valid ? rows[j] : 0u;
+    process_entries<CHK, false, 1>(a.prg, a.op, a.nr_grp, &e, &valid, sink,
+                                   nullptr);
+  }
   if (LDSG) flush_groups(a.sp, a.out, a.nr_grp, blockIdx.x, gridDim.x);
 }
 
@@ -179,26 +186,35 @@ __global__ void acct_count_kernel(const uint32_t *__restrict__ bits,
 
 
 // Rows in the OR of a column's bins (plan densities).
+// out[0] += rows in the column's operand (OR of its bins), out[1] += rows
+// only in its edge bins (the rows a candidate check must re-check).
 __global__ void density_kernel(DevCol c, uint32_t nwords, uint32_t nrows,
                                unsigned long long *out) {
-  __shared__ unsigned long long s[256];
-  unsigned long long cnt = 0;
+  __shared__ unsigned long long s[2][256];
+  unsigned long long cnt = 0, cnte = 0;
   for (uint32_t w = blockIdx.x * 256 + threadIdx.x; w < nwords;
        w += gridDim.x * 256) {
-    uint64_t x = 0;
+    uint64_t x = 0, e = 0;
     for (uint32_t k = 0; k < c.nm; ++k) x |= c.m[k][w];
-    if (c.l) x |= c.l[w];
-    if (c.r) x |= c.r[w];
-    if (w == nwords - 1 && (nrows & 63u)) x &= (1ull << (nrows & 63u)) - 1ull;
-    cnt += __popcll(x);
+    if (c.l) e |= c.l[w];
+    if (c.r) e |= c.r[w];
+    uint64_t keep = ~0ull;
+    if (w == nwords - 1 && (nrows & 63u)) keep = (1ull << (nrows & 63u)) - 1ull;
+    cnt += __popcll((x | e) & keep);
+    cnte += __popcll(e & ~x & keep);
   }
-  s[threadIdx.x] = cnt;
+  s[0][threadIdx.x] = cnt;
+  s[1][threadIdx.x] = cnte;
   __syncthreads();
   for (int k = 128; k > 0; k >>= 1) {
-    if ((int)threadIdx.x < k) s[threadIdx.x] += s[threadIdx.x + k];
+    if ((int)threadIdx.x < k) {
+      s[0][threadIdx.x] += s[0][threadIdx.x + k];
+      s[1][threadIdx.x] += s[1][threadIdx.x + k];
+    }
     __syncthreads();
   }
-  if (threadIdx.x == 0 && s[0]) atomicAdd(out, s[0]);
+  if (threadIdx.x == 0 && s[0][0]) atomicAdd(out, s[0][0]);
+  if (threadIdx.x == 0 && s[1][0]) atomicAdd(out + 1, s[1][0]);
 }
 
 // ---------------------------------------------------------------- launches
@@ -433,6 +449,8 @@ int launch_fused(mbx_ctx *ctx, const QueryPlan &q, const Acct *acct) {
   return MBX_OK;
 }
 
+// Scan launch: 2 words per lane on the resident grid (several passes over
+// a large table).
 int launch_scan(mbx_ctx *ctx, const QueryPlan &q) {
   QArgs a;
   std::memset(&a, 0, sizeof(a));
@@ -443,13 +461,19 @@ int launch_scan(mbx_ctx *ctx, const QueryPlan &q) {
   int st = spread_of(ctx, q, &a.sp);
   if (st) return st;
   const hipStream_t s = stream_of(ctx, q);
+  constexpr int W = 2;
+  const size_t lds = grp_lds(q);
+  const void *fn = q.lds ? reinterpret_cast<const void *>(&scan_kernel<true, W>)
+                         : reinterpret_cast<const void *>(&scan_kernel<false, W>);
+  const unsigned per_cu = occupancy(fn, lds);
+  const uint64_t waves = ceil_div(ceil_div(q.prg.nwords, 64), W);
+  const uint64_t resident = (uint64_t)ctx->num_cus * per_cu * 4;
   const unsigned grid =
-      persistent_grid(ctx, (uint32_t)ceil_div(q.prg.nrows, 256), 8);
+      (unsigned)std::max<uint64_t>(1, ceil_div(std::min(waves, resident), 4));
   if (q.lds)
-    hipLaunchKernelGGL(scan_kernel<true>, dim3(grid), dim3(256), grp_lds(q),
-                       s, a);
+    hipLaunchKernelGGL((scan_kernel<true, W>), dim3(grid), dim3(256), lds, s, a);
   else
-    hipLaunchKernelGGL(scan_kernel<false>, dim3(grid), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((scan_kernel<false, W>), dim3(grid), dim3(256), 0, s, a);
   MBX_HIP(hipGetLastError());
   return MBX_OK;
 }
@@ -884,32 +908,40 @@ int mbx_alg_bytes_batch(mbx_ctx *ctx, const mbx_query *qs, uint32_t nq,
   return alg_bytes_multi(ctx, qs, nq, bytes, selected);
 }
 
-int mbx_col_density(mbx_ctx *ctx, const mbx_vprg *prg, double dens[MBX_MAXCOLS]) {
-  if (!ctx || !dens) return fail(MBX_ERR_INVALID, "col_density: null");
+int mbx_col_stats(mbx_ctx *ctx, const mbx_vprg *prg, double dens[MBX_MAXCOLS],
+                  double edge[MBX_MAXCOLS]) {
+  if (!ctx || !dens) return fail(MBX_ERR_INVALID, "col_stats: null");
   DevPrg P;
   int st = make_dev_prg(prg, &P);
   if (st) return st;
   void *buf = nullptr;
-  st = ctx->pool->alloc(8 * MBX_MAXCOLS, &buf);
+  st = ctx->pool->alloc(16 * MBX_MAXCOLS, &buf);
   if (st) return st;
   unsigned long long *d = static_cast<unsigned long long *>(buf);
-  st = launch_fill_u64(ctx->stream, d, 0, MBX_MAXCOLS);
+  st = launch_fill_u64(ctx->stream, d, 0, 2 * MBX_MAXCOLS);
   const unsigned grid = persistent_grid(ctx, (uint32_t)ceil_div(P.nwords, 256), 4);
   for (int c = 0; c < MBX_MAXCOLS && !st; ++c) {
     if (!P.cols[c].nm && !P.cols[c].has_edge) continue;
     hipLaunchKernelGGL(density_kernel, dim3(grid), dim3(256), 0, ctx->stream,
-                       P.cols[c], P.nwords, P.nrows, d + c);
+                       P.cols[c], P.nwords, P.nrows, d + 2 * c);
   }
-  unsigned long long h[MBX_MAXCOLS] = {0, 0, 0, 0};
+  unsigned long long h[2 * MBX_MAXCOLS] = {};
   hipError_t e = hipGetLastError();
   if (!st && e == hipSuccess)
     e = hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   ctx->pool->free(buf);
   if (st) return st;
-  if (e != hipSuccess) return hip_fail(e, "col_density");
-  for (int c = 0; c < MBX_MAXCOLS; ++c) dens[c] = (double)h[c] / P.nrows;
+  if (e != hipSuccess) return hip_fail(e, "col_stats");
+  for (int c = 0; c < MBX_MAXCOLS; ++c) {
+    dens[c] = (double)h[2 * c] / P.nrows;
+    if (edge) edge[c] = (double)h[2 * c + 1] / P.nrows;
+  }
   return MBX_OK;
+}
+
+int mbx_col_density(mbx_ctx *ctx, const mbx_vprg *prg, double dens[MBX_MAXCOLS]) {
+  return mbx_col_stats(ctx, prg, dens, nullptr);
 }
 
 }  // extern "C"

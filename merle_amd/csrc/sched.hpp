@@ -8,7 +8,9 @@
 // memory.  When the allocator itself refuses a reservation while nothing is
 // running or admitted -- so nothing the scheduler owns will ever be released
 // -- the request is rejected instead of waiting forever.  A free slot takes
-// up to max_batch admitted requests, in admission order.
+// up to max_batch admitted requests, in admission order; a request marked
+// solo (one that runs as a launch of its own: a large or a scan request)
+// goes out alone, so the requests behind it do not wait for it on its slot.
 //
 // The device side is behind SchedDevice, so the same code runs the GPU
 // server (serve.cpp) and a fake device in CPU tests.  The class is not
@@ -48,9 +50,10 @@ class Scheduler {
         slots_(nslots ? nslots : 1) {}
 
   // A new request needing `need` bytes; returns its id (0, 1, 2, ...).
-  uint32_t add(size_t need) {
+  uint32_t add(size_t need, bool solo = false) {
     Req r;
     r.need = need;
+    r.solo = solo;
     reqs_.push_back(r);
     const uint32_t id = (uint32_t)(reqs_.size() - 1);
     pending_.push_back(id);
@@ -146,6 +149,7 @@ class Scheduler {
     int state = MBX_REQ_QUEUED;
     uint32_t slot = 0;
     bool delayed = false;
+    bool solo = false;
   };
   struct Slot {
     bool busy = false;
@@ -198,9 +202,12 @@ class Scheduler {
     for (uint32_t s = 0; s < slots_.size() && !admitted_.empty(); ++s) {
       Slot &sl = slots_[s];
       if (sl.busy) continue;
-      const uint32_t k = admitted_.size() < max_batch_
-                             ? (uint32_t)admitted_.size()
-                             : max_batch_;
+      // a solo head goes alone; otherwise the leading non-solo requests
+      uint32_t k = 1;
+      if (!reqs_[admitted_.front()].solo)
+        while (k < max_batch_ && k < admitted_.size() &&
+               !reqs_[admitted_[k]].solo)
+          ++k;
       sl.reqs.assign(admitted_.begin(), admitted_.begin() + k);
       admitted_.erase(admitted_.begin(), admitted_.begin() + k);
       for (uint32_t q : sl.reqs) {

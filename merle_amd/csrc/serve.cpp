@@ -4,11 +4,12 @@
 // Threads of a server:
 //  - callers submit, poll, wait and cancel from any thread;
 //  - one dispatcher runs the scheduler (sched.hpp): admission by memory,
-//    then, whenever a slot is free, up to max_batch admitted requests go out
-//    on it -- large ones as fused launches of their own, the rest as one
-//    persistent batched launch (the kernel of mbx_batch_*) -- while the other
-//    slots are still running, so work of requests that arrived later
-//    overlaps on the device with work already in flight.  Enqueueing is all
+//    then, whenever a slot is free, admitted requests go out on it -- a
+//    large or scan request alone, as a launch of its own; else up to
+//    max_batch small ones as one persistent batched launch (the kernel of
+//    mbx_batch_*) -- while the other slots are still running, so work of
+//    requests that arrived later overlaps on the device with work already in
+//    flight.  Enqueueing is all
 //    it does with the GPU: it never waits for a kernel or a copy;
 //  - one completion thread per slot waits for the slot's event, copies the
 //    results (already moved by the slot's stream into pinned staging) into
@@ -160,8 +161,10 @@ struct mbx_server : SchedDevice {
     int st = MBX_OK;
     uint32_t nsmall = 0;
     for (uint32_t i = 0; i < k && !st; ++i) {
-      // plain: every request as one fused launch; else large ones only
-      if (!plain && tmp[i].prg.factsz < fused_rows) {
+      // a scan request as a scan launch of its own; plain: every other
+      // request as one fused launch; else large ones only
+      const bool scan = (tmp[i].flags & MBX_PLAN_MASK) == MBX_PLAN_SCAN;
+      if (!scan && !plain && tmp[i].prg.factsz < fused_rows) {
         tmp[nsmall++] = tmp[i];
         continue;
       }
@@ -174,7 +177,7 @@ struct mbx_server : SchedDevice {
       qp.spread_done = sl.spread_done;
       qp.stream = sl.s;
       st = launch_fill_u64(sl.s, tmp[i].d_out, 0, tmp[i].nr_grp);
-      if (!st) st = launch_fused(ctx, qp, nullptr);
+      if (!st) st = scan ? launch_scan(ctx, qp) : launch_fused(ctx, qp, nullptr);
     }
     if (!st && nsmall) {
       BatchShape shape;
@@ -277,7 +280,11 @@ struct mbx_server : SchedDevice {
     r.arrival = at;
     r.ready = ev;
     reqs.push_back(r);
-    const uint32_t id = sched->add(round256((size_t)q->nr_grp * 8));
+    // large and scan requests run as launches of their own: dispatched
+    // alone, they hold up no batch mates on their slot
+    const bool solo = plain || (q->flags & MBX_PLAN_MASK) == MBX_PLAN_SCAN ||
+                      q->prg.factsz >= fused_rows;
+    const uint32_t id = sched->add(round256((size_t)q->nr_grp * 8), solo);
     *ticket = id;
     ++gen;
     cv_disp.notify_one();

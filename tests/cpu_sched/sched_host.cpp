@@ -154,6 +154,35 @@ static void slots_and_batches() {
   std::printf("slots_and_batches: %u launches\n", s.launches());
 }
 
+// Solo requests (launches of their own) go out alone, in FIFO order, and
+// batches stop in front of them.
+static void solo_requests() {
+  FakeDevice d(1 << 20, 2);
+  Scheduler s(2, 4, 0, 0);
+  s.add(256);
+  s.add(256);
+  s.add(256, true);
+  s.add(256);
+  s.add(256, true);
+  s.add(256, true);
+  s.step(d);
+  CHECK(d.launches_on[0] == std::vector<uint32_t>({0, 1}));
+  CHECK(d.launches_on[1] == std::vector<uint32_t>({2}));
+  s.finish(1, d, nullptr);
+  s.step(d);
+  CHECK(d.launches_on[1] == std::vector<uint32_t>({2, 3}));
+  s.finish(0, d, nullptr);
+  s.step(d);
+  CHECK(d.launches_on[0] == std::vector<uint32_t>({0, 1, 4}));
+  s.finish(1, d, nullptr);
+  s.step(d);
+  CHECK(d.launches_on[1] == std::vector<uint32_t>({2, 3, 5}));
+  s.finish(0, d, nullptr);
+  s.finish(1, d, nullptr);
+  CHECK(s.idle() && s.launches() == 5);
+  std::printf("solo_requests: %u launches\n", s.launches());
+}
+
 // Cancel: a queued or admitted request ends cancelled (an admitted one
 // gives its memory back); a running one is left alone.
 static void cancel_states() {
@@ -213,7 +242,7 @@ static void random_workload(uint64_t seed) {
     if (need.size() < nreq && rng() % 3) {
       const size_t n = 256 * (1 + rng() % 96);
       need.push_back(n);
-      s.add(n);
+      s.add(n, rng() % 5 == 0);
     }
     if (rng() % 11 == 0 && !need.empty()) {
       const size_t back = std::min<size_t>(need.size(), 20);
@@ -264,6 +293,7 @@ int main(int argc, char **argv) {
   pool_full_waits_for_running();
   budget_fifo();
   slots_and_batches();
+  solo_requests();
   cancel_states();
   launch_failure();
   for (int i = 0; i < seeds && !g_fail; ++i) random_workload(1000 + i);

@@ -123,26 +123,29 @@ def test_every_plan_same_result(dev):
 
 def test_learned_chooser_gain(dev):
     """The cost model trained on runtimes the engine measures (config-4 form
-    at 4M rows, 300 training programs): on 60 held-out programs its choices
-    give the same sums as the heuristic and as-written plans, and the gain it
-    measures over the heuristic is printed; it may not be materially slower.
+    at 32M rows, every candidate plan of 300 training programs): on 60
+    held-out programs its choices give the same sums as the heuristic and
+    as-written plans and, in sum, run no slower than the static heuristic.
     The fused-only choice that the batched launch uses returns fused plans."""
     from merle_amd import learned
-    n = 4_000_000
+    n = 32_000_000
     sch = learned.config4_device(dev, n)
-    bins = learned.build_bins(dev, sch, [f"c{c}" for c in range(learned.C4_COLS)])
+    cols = [f"c{c}" for c in range(learned.C4_COLS)]
+    bins = learned.build_bins(dev, sch, cols)
     X, Y = learned.collect(dev, sch, bins, 300)
     model = learned.CostModel(dev)
     model.train(X, Y)
     ev = learned.evaluate(dev, sch, bins, model, 60)
     print("learned chooser:", ev)
     assert ev["identical_results"]
-    assert ev["learned_ms"] <= 1.15 * ev["heuristic_ms"]
+    assert ev["learned_ms"] <= ev["heuristic_ms"]
     rng = np.random.default_rng(5)
     for _ in range(10):
         p = learned.Prepared(dev, sch, bins, learned.random_program(rng))
         assert model.choose_fused(p).flags & 3 == abi.PLAN_FUSED
     model.close()
+    for c in cols:
+        check(dev.lib.mbx_free(dev.ctx, bins[c][0]))
     for c in sch.cols.values():
         c.dev.free()
 

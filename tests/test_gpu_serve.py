@@ -85,8 +85,11 @@ def test_same_alone_and_in_the_mix(dev, tables, mode):
     mixed, lat, st = _replay(dev, qs, arrivals, cfg, specs)
     assert st.completed == len(specs) and st.rejected == 0
     assert np.all(lat > 0) and st.p99_s >= st.p50_s > 0
-    if mode != "plain":
+    if mode == "batched":
         assert st.mean_batch > 1.0          # requests actually shared launches
+    else:
+        # plain, and large requests (fused launches of their own) go alone
+        assert st.mean_batch == 1.0
     for i in range(len(specs)):
         np.testing.assert_array_equal(mixed[i], alone[i], err_msg=specs[i].name)
 

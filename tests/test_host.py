@@ -279,3 +279,70 @@ def test_library_missing_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(merle_amd, "_lib", None)
     with pytest.raises(merle_amd.LibraryMissing):
         merle_amd.load()
+
+
+def _prg(ncols, instrs):
+    """A vprg whose columns 0..ncols-1 each have one (fake, never read)
+    middle bin, with the given instructions."""
+    p = abi.Vprg()
+    for c in range(ncols):
+        p.cols[c].middle[0] = 0x1000 * (c + 1)
+    p.factsz = 1 << 20
+    for i, (op, d, s1) in enumerate(instrs):
+        p.instrs[i].opcode, p.instrs[i].dstreg, p.instrs[i].src1 = op, d, s1
+    return p
+
+
+def test_plan_features_and_candidates(lib):
+    """The cost-model features price a program by what it selects: the
+    program run over the column densities as independent probabilities
+    (AND: product, OR: a + b - ab, NOT: 1 - a), and the re-check share as
+    edge[c] x selectivity / dens[c]; plan-kind flags are one-hot.  The
+    candidates of a 3-column conjunction are the early-out plan for each of
+    the 6 ordered pairs of leading columns (the rest sparsest first), then
+    eager, materialise and scan in sparsest-first order."""
+    def feats(prg, dens, edge, plan):
+        f = (C.c_float * abi.NFEAT)()
+        assert lib.mbx_plan_features_ex(C.byref(prg), (C.c_double * 4)(*dens),
+                                        (C.c_double * 4)(*edge), 10,
+                                        C.byref(plan), f) == abi.OK
+        return np.array(f[:])
+
+    lg = lambda x: np.log10(x) / 9 + 1
+    conj = _prg(3, [(abi.ORM, 0, 0), (abi.ANDM, 0, 1), (abi.ANDM, 0, 2)])
+    dens, edge = [0.5, 0.2, 0.4, 0], [0.1, 0, 0, 0]
+    plan = abi.Plan()
+    assert lib.mbx_plan_heuristic(C.byref(conj), (C.c_double * 4)(*dens),
+                                  C.byref(plan)) == abi.OK
+    assert list(plan.order[:plan.norder]) == [1, 2, 0]
+    f = feats(conj, dens, edge, plan)
+    np.testing.assert_allclose(f[:4], [0.2, 0.4, 0.5, 0], rtol=1e-6)
+    np.testing.assert_allclose(f[4:8], [0.2, 0.08, 0.04, 0], rtol=1e-6)
+    np.testing.assert_allclose(f[12], lg(0.04), rtol=1e-5)
+    np.testing.assert_allclose(f[13], lg(0.1 * 0.04 / 0.5), rtol=1e-5)
+    # ANDNOT: c0 AND NOT c1 with an empty c1 selects all of c0
+    andnot = _prg(2, [(abi.ORM, 0, 0), (abi.ORM, 1, 1), (abi.NOT, 1, 1),
+                      (abi.AND, 0, 1)])
+    d2 = [0.9, 0.0, 0, 0]
+    desc = abi.Plan()
+    assert lib.mbx_plan_describe(C.byref(andnot), C.byref(desc)) == abi.OK
+    f = feats(andnot, d2, [0.8, 0, 0, 0], desc)
+    np.testing.assert_allclose(f[12], lg(0.9), rtol=1e-5)
+    np.testing.assert_allclose(f[13], lg(0.8), rtol=1e-5)
+    # OR: a + b - ab
+    orp = _prg(2, [(abi.ORM, 0, 0), (abi.ORM, 0, 1)])
+    f = feats(orp, [0.3, 0.5, 0, 0], [0, 0, 0, 0], desc)
+    np.testing.assert_allclose(f[12], lg(0.3 + 0.5 - 0.15), rtol=1e-5)
+    # candidates and the one-hot plan kinds
+    cand, cnt = (abi.Plan * 32)(), C.c_int()
+    assert lib.mbx_plan_candidates(C.byref(conj), (C.c_double * 4)(*dens), cand,
+                                   32, C.byref(cnt)) == abi.OK
+    got = [(list(c.order[:c.norder]), c.flags) for c in cand[:cnt.value]]
+    assert got == [([1, 2, 0], 0), ([1, 0, 2], 0), ([2, 1, 0], 0),
+                   ([2, 0, 1], 0), ([0, 1, 2], 0), ([0, 2, 1], 0),
+                   ([1, 2, 0], abi.PLAN_EAGER), ([1, 2, 0], abi.PLAN_MATERIALISE),
+                   ([1, 2, 0], abi.PLAN_SCAN)]
+    kinds = [feats(conj, dens, edge, c)[[11, 14, 15]] for c in cand[6:9]]
+    np.testing.assert_array_equal(kinds, [[0, 1, 0], [1, 0, 0], [0, 0, 1]])
+    assert lib.mbx_plan_candidates(C.byref(conj), (C.c_double * 4)(*dens), cand,
+                                   8, C.byref(cnt)) == abi.ERR_CAPACITY
