@@ -435,6 +435,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--streams", type=int, default=4,
+                    help="concurrent streams the 12 queries are issued on")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--rows", type=int, default=None,
                     help="fact rows (default: config 2's 120M)")
@@ -493,41 +495,77 @@ def main():
         nsel.append(k.value)
     log("selectivity:", ", ".join(f"{s.name}={k / n:.1e}" for s, k in zip(specs, nsel)))
 
-    def step(timers=None):
+    # The step: the 12 queries, one fused launch each, issued round robin on
+    # `streams` concurrent streams of the context (queries are independent;
+    # each zeroes its own output on its stream before its launch), then the
+    # context's stream joins the others.
+    nstreams = max(1, args.streams)
+    own = C.c_void_p(dev.lib.mbx_stream(dev.ctx))
+    extra = [C.c_void_p() for _ in range(nstreams - 1)]
+    for x in extra:
+        check(dev.lib.mbx_stream_create(C.byref(x)))
+    streams = [own] + extra
+    joinv = (C.c_void_p * max(1, len(extra)))(*[x.value for x in extra])
+
+    def step(ns):
+        for i, (s, prg, op, out) in enumerate(qs):
+            check(dev.lib.mbx_set_stream(dev.ctx, streams[i % ns]))
+            check(dev.lib.mbx_prog_grpby(dev.ctx, C.byref(prg), C.byref(op),
+                                         s.nr_grp, out, plans[i]))
+        check(dev.lib.mbx_set_stream(dev.ctx, None))
+        if ns > 1:
+            check(dev.lib.mbx_stream_join(dev.ctx, joinv, ns - 1))
+
+    def step_isolated(timers):
+        """One stream; one fill zeroes all outputs, then each launch adds
+        into its output between its own event pair."""
         check(dev.lib.mbx_fill_u32(dev.ctx, outbuf.ptr, 0, 2 * int(offs[-1])))
         for i, (s, prg, op, out) in enumerate(qs):
-            if timers:
-                timers[i].start()
+            timers[i].start()
             check(dev.lib.mbx_prog_grpby(dev.ctx, C.byref(prg), C.byref(op),
                                          s.nr_grp, out, plans[i] | abi.ACCUMULATE))
-            if timers:
-                timers[i].mark_stop()
+            timers[i].mark_stop()
+
+    def timed_steps(ns, k):
+        whole = Timer(dev)
+        dev.sync()
+        whole.start()
+        for _ in range(k):
+            step(ns)
+        ms = whole.stop()
+        whole.close()
+        return ms / k
 
     for _ in range(args.warmup):
-        step()
+        step(nstreams)
     dev.sync()
-    log(f"warmup done; timing {args.steps} steps")
-    whole = Timer(dev)
-    dev.sync()
-    whole.start()
-    for _ in range(args.steps):
-        step()
-    total_ms = whole.stop()
-    # per-launch durations of the fused kernel: event pairs around each
-    # launch on the kernels' stream, enqueued back to back (no host sync
-    # inside a step), read after the step
+    log(f"warmup done; timing {args.steps} steps on {nstreams} streams")
+    ms_per_step = timed_steps(nstreams, args.steps)
+    # the same step with every launch on one stream, for comparison
+    for _ in range(args.warmup):
+        step(1)
+    ms_one_stream = timed_steps(1, args.steps)
+    # per-launch durations of the fused kernel in isolation: event pairs
+    # around each launch, all on one stream, enqueued back to back (no host
+    # sync inside a step), read after the step
     timers = [Timer(dev) for _ in qs]
     per_q = [[] for _ in qs]
     for _ in range(max(3, args.steps)):
-        step(timers)
+        step_isolated(timers)
         for i, t in enumerate(timers):
             per_q[i].append(t.elapsed())
+    for t in timers:
+        t.close()
+    dev.sync()
+    for x in extra:
+        check(dev.lib.mbx_stream_destroy(x))
     qms = [float(np.median(v)) for v in per_q]
-    ms_per_step = total_ms / args.steps
     rows_per_s = n * len(qs) / (ms_per_step / 1e3)
     achieved = sum(balg) / (sum(qms) / 1e3) / 1e9
-    log(f"{ms_per_step:.3f} ms/step, {rows_per_s / 1e9:.1f} G rows/s, "
-        f"fused kernel {achieved:.0f} GB/s algorithmic")
+    step_gbs = sum(balg) / (ms_per_step / 1e3) / 1e9
+    log(f"{ms_per_step:.3f} ms/step on {nstreams} streams ({ms_one_stream:.3f} on "
+        f"one), {rows_per_s / 1e9:.1f} G rows/s; fused kernel {achieved:.0f} GB/s "
+        f"algorithmic in isolation, {step_gbs:.0f} GB/s over the step")
     for (s, *_), b, q in zip(qs, balg, qms):
         log(f"  {s.name}: {q * 1e3:.1f} us, {b / 1e6:.1f} MB alg, "
             f"{b / q / 1e6:.0f} GB/s")
@@ -547,9 +585,11 @@ def main():
         "data": "synthetic (seeded star schema, uniform keys)",
         "config": {
             "workload": f"star schema, {n:,} fact rows, 12 queries of 2-4 "
-                        "predicates, one fused launch per query",
+                        "predicates, one fused launch per query, the "
+                        f"launches issued round robin on {nstreams} "
+                        "concurrent streams",
             "fact_rows": n, "queries": len(qs),
-            "bitmaps": nbm,
+            "bitmaps": nbm, "streams": nstreams,
             "plan": "fused, static heuristic (sparsest first; eager loads "
                     f"for {sum(1 for f in plans if f & abi.PLAN_EAGER)} of 12)",
         },
@@ -559,7 +599,17 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": profiled_traffic(),
             "alg_bytes_per_launch": int(sum(balg) / len(balg)),
+            "note": "fused kernel launches timed in isolation (one stream, "
+                    "event pairs around each launch)",
         },
+        "roofline_step": {
+            "achieved": round(step_gbs, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(step_gbs / HBM_PEAK_GBS, 4),
+            "note": "algorithmic bytes of the 12 launches / ms_per_step "
+                    "(launches overlapping on the streams)",
+        },
+        "one_stream": {"ms_per_step": round(ms_one_stream, 4),
+                       "value": round(n * len(qs) / ms_one_stream / 1e6, 3)},
         "index_build": {"ms": round(cms, 2),
                         "GB_per_s": round(cbytes / cms / 1e6, 1)},
         "per_query_us": {s.name: round(q * 1e3, 2) for (s, *_), q in zip(qs, qms)},

@@ -60,6 +60,13 @@ int mbx_sync(mbx_ctx *ctx);
 /* The HIP stream all launches of this context go to (hipStream_t). */
 void *mbx_stream(mbx_ctx *ctx);
 int mbx_set_stream(mbx_ctx *ctx, void *hip_stream);  /* NULL = own stream */
+/* No reference counterpart: streams for running independent queries of one
+ * context concurrently (mbx_set_stream before each call).  Non-blocking,
+ * default priority.  mbx_stream_join makes the context's current stream
+ * wait on the device for the work enqueued so far on the given streams. */
+int mbx_stream_create(void **hip_stream);
+int mbx_stream_destroy(void *hip_stream);
+int mbx_stream_join(mbx_ctx *ctx, void *const *hip_streams, int n);
 
 /* ----------------------------------------------------------------- memory */
 /* Pool-backed, 256-byte aligned.  Replaces __cum (bmpidx_create.cu:4-9) and

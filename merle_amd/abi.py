@@ -117,6 +117,9 @@ PROTOTYPES = {
     "mbx_sync": (_i32, [_vp]),
     "mbx_stream": (_vp, [_vp]),
     "mbx_set_stream": (_i32, [_vp, _vp]),
+    "mbx_stream_create": (_i32, [_P(_vp)]),
+    "mbx_stream_destroy": (_i32, [_vp]),
+    "mbx_stream_join": (_i32, [_vp, _P(_vp), _i32]),
     "mbx_alloc": (_i32, [_vp, _sz, _P(_vp)]),
     "mbx_free": (_i32, [_vp, _vp]),
     "mbx_pool_stats": (_i32, [_vp, _P(_sz), _P(_sz), _P(_sz), _P(_sz)]),

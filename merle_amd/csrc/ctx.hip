@@ -141,6 +141,8 @@ int mbx_destroy(mbx_ctx *ctx) {
   if (!ctx) return MBX_OK;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
+  for (auto &kv : ctx->join_events)
+    if (kv.second) (void)hipEventDestroy(kv.second);
   delete ctx->pool;
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
   delete ctx;
@@ -158,6 +160,38 @@ void *mbx_stream(mbx_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
 int mbx_set_stream(mbx_ctx *ctx, void *s) {
   if (!ctx) return fail(MBX_ERR_INVALID, "null ctx");
   ctx->stream = s ? static_cast<hipStream_t>(s) : ctx->own_stream;
+  return MBX_OK;
+}
+
+int mbx_stream_create(void **stream) {
+  if (!stream) return fail(MBX_ERR_INVALID, "stream_create: null");
+  hipStream_t s = nullptr;
+  MBX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = s;
+  return MBX_OK;
+}
+
+int mbx_stream_destroy(void *stream) {
+  if (!stream) return MBX_OK;
+  MBX_HIP(hipStreamDestroy(static_cast<hipStream_t>(stream)));
+  return MBX_OK;
+}
+
+// The context's current stream waits (on the device) for the work enqueued
+// so far on each of the given streams.
+int mbx_stream_join(mbx_ctx *ctx, void *const *streams, int n) {
+  if (!ctx || (n > 0 && !streams)) return fail(MBX_ERR_INVALID, "stream_join: null");
+  std::lock_guard<std::mutex> g(ctx->spread_mu);
+  for (int i = 0; i < n; ++i) {
+    hipStream_t s = static_cast<hipStream_t>(streams[i]);
+    if (!s || s == ctx->stream) continue;
+    // one event per stream, re-recorded: a wait takes the record made
+    // before it
+    hipEvent_t &e = ctx->join_events[s];
+    if (!e) MBX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    MBX_HIP(hipEventRecord(e, s));
+    MBX_HIP(hipStreamWaitEvent(ctx->stream, e, 0));
+  }
   return MBX_OK;
 }
 

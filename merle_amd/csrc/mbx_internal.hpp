@@ -86,6 +86,8 @@ struct mbx_ctx {
   // (mbx_set_stream), so queries on different streams never share replicas
   std::mutex spread_mu;
   std::map<hipStream_t, std::pair<unsigned long long *, uint32_t *>> spreads;
+  // mbx_stream_join's event per joined stream (guarded by spread_mu)
+  std::map<hipStream_t, hipEvent_t> join_events;
 };
 
 struct mbx_timer {
