@@ -124,6 +124,26 @@ int mbx_create_bin(mbx_ctx *ctx, const mbx_colref *src, uint32_t nrows,
 int mbx_create_bin_into(mbx_ctx *ctx, const mbx_colref *src, uint32_t nrows,
                         const uint64_t *mins, const uint64_t *maxs,
                         size_t nbins, uint32_t *slab, size_t stride_words);
+/* Several attributes of one dimension read through one foreign-key column
+ * (c_region, c_nation and c_city through lo_custkey): the bitmaps of every
+ * set as mbx_create_bin_into(ctx, {set.data, fk, dimsz, set.elem_bytes},
+ * nrows, set.mins, set.maxs, set.nbins, set.slab, stride_words) would write
+ * them, in one pass over the keys (one key load and one lookup of a table of
+ * packed per-row bin indexes per fact row).  Sets of consecutive bins, at
+ * most 4 sets whose indexes fit 32 bits together (8 bits below 255 bins,
+ * else 16); other cases take one pass per set.  No reference counterpart
+ * beyond create_bin itself (bmpidx_create.cu:14-56).  Waits for its
+ * kernels (the table is temporary pool memory). */
+typedef struct mbx_binset {
+  const void *data;          /* device: dimension attribute */
+  uint32_t elem_bytes;       /* 1, 2, 4 or 8 */
+  uint32_t nbins;
+  const uint64_t *mins, *maxs;   /* host, nbins each */
+  uint32_t *slab;            /* device: nbins bitmaps, stride_words apart */
+} mbx_binset;
+int mbx_create_bin_multi(mbx_ctx *ctx, const uint32_t *fk, uint32_t dimsz,
+                         uint32_t nrows, const mbx_binset *sets, size_t nsets,
+                         size_t stride_words);
 /* create_select / create_join (bmpidx_create.cu:58-136): one bitmap;
  * *out = NULL and MBX_ERR_OOM on allocation failure. */
 int mbx_create_select(mbx_ctx *ctx, const void *src, uint32_t elem_bytes,

@@ -63,6 +63,12 @@ class RowOp(C.Structure):
                 ("ma", ColRef), ("mb", ColRef)]
 
 
+class BinSet(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("elem_bytes", C.c_uint32),
+                ("nbins", C.c_uint32), ("mins", C.POINTER(C.c_uint64)),
+                ("maxs", C.POINTER(C.c_uint64)), ("slab", C.c_void_p)]
+
+
 class Query(C.Structure):
     _fields_ = [("prg", Vprg), ("op", RowOp), ("nr_grp", C.c_uint32),
                 ("flags", C.c_uint32), ("d_out", C.c_void_p)]
@@ -138,6 +144,8 @@ PROTOTYPES = {
                               _P(_vp), _P(_sz)]),
     "mbx_create_bin_into": (_i32, [_vp, _P(ColRef), _u32, _P(_u64), _P(_u64),
                                    _sz, _vp, _sz]),
+    "mbx_create_bin_multi": (_i32, [_vp, _vp, _u32, _u32, _P(BinSet), _sz,
+                                    _sz]),
     "mbx_create_select": (_i32, [_vp, _vp, _u32, _u32, _u32, _u32, _P(_vp)]),
     "mbx_create_join": (_i32, [_vp, _vp, _vp, _u32, _u32, _u32, _u32, _u32,
                                _P(_vp)]),

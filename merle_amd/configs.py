@@ -226,31 +226,57 @@ def config2_device(dev, n=C2_ROWS, seed=SEED):
 
 
 def config2_index(dev, sch, attrs=None, timer=None):
-    """One bitmap per attribute value, projected through the fact FK (one
-    create_bin_into pass per attribute, bins [v, v+1)).  Slabs are allocated
-    first, so a timer brackets only the creation kernels (the reference
-    times cudaMalloc too, ssb/bmpidx-create.cu:64-68).
-    -> ({attr: [device ptr per value]}, [slabs], ms, algorithmic bytes)."""
+    """One bitmap per attribute value, projected through the fact FK (bins
+    [v, v+1)).  The attributes of one dimension are built in one pass over
+    its foreign keys (mbx_create_bin_multi: c_region, c_nation and c_city
+    from one read of lo_cust); a dimension with one attribute takes
+    create_bin_into.  Slabs are allocated first, so a timer brackets only
+    the creation (the reference times cudaMalloc too,
+    ssb/bmpidx-create.cu:64-68).
+    -> ({attr: [device ptr per value]}, [slabs], ms, algorithmic bytes:
+    each key column once, plus every bitmap)."""
     from .host import stride_words
     from .spec import colref
+    attrs = list(attrs or C2_ATTRS)
     idx, slabs, ms, nbytes = {}, [], 0.0, 0
     sw = stride_words(sch.nrows)
-    for a in (attrs or list(C2_ATTRS)):
-        card = C2_ATTRS[a][2]
-        ref = colref(sch, Src(a, C2_ATTRS[a][1]))
-        slab = dev.alloc(sw * 4 * card)
+    by_fk = {}
+    for a in attrs:
+        by_fk.setdefault(C2_ATTRS[a][1], []).append(a)
+    for fk, group in by_fk.items():
+        keep = []
+        for a in group:
+            card = C2_ATTRS[a][2]
+            slab = dev.alloc(sw * 4 * card)
+            slabs.append(slab)
+            idx[a] = [slab.ptr + j * sw * 4 for j in range(card)]
+            lo = (C.c_uint64 * card)(*range(card))
+            hi = (C.c_uint64 * card)(*range(1, card + 1))
+            keep.append((a, card, slab, lo, hi))
         if timer:
             dev.sync()
             timer.start()
-        check(dev.lib.mbx_create_bin_into(dev.ctx, C.byref(ref), sch.nrows,
-                                          (C.c_uint64 * card)(*range(card)),
-                                          (C.c_uint64 * card)(*range(1, card + 1)),
-                                          card, slab.ptr, sw))
+        if len(group) == 1:
+            a, card, slab, lo, hi = keep[0]
+            ref = colref(sch, Src(a, fk))
+            check(dev.lib.mbx_create_bin_into(dev.ctx, C.byref(ref), sch.nrows,
+                                              lo, hi, card, slab.ptr, sw))
+        else:
+            sets = (abi.BinSet * len(group))()
+            for i, (a, card, slab, lo, hi) in enumerate(keep):
+                col = sch[a]
+                sets[i].data = col.dev.ptr
+                sets[i].elem_bytes = col.elem_bytes
+                sets[i].nbins = card
+                sets[i].mins = C.cast(lo, C.POINTER(C.c_uint64))
+                sets[i].maxs = C.cast(hi, C.POINTER(C.c_uint64))
+                sets[i].slab = slab.ptr
+            check(dev.lib.mbx_create_bin_multi(dev.ctx, sch[fk].dev.ptr,
+                                               sch[group[0]].n, sch.nrows, sets,
+                                               len(group), sw))
         if timer:
             ms += timer.stop()
-        nbytes += sch.nrows * 4 + card * sch.nrows // 8
-        idx[a] = [slab.ptr + j * sw * 4 for j in range(card)]
-        slabs.append(slab)
+        nbytes += sch.nrows * 4 + sum(card for _, card, *_ in keep) * sch.nrows // 8
     dev.sync()
     return idx, slabs, ms, nbytes
 

@@ -273,6 +273,134 @@ __global__ __launch_bounds__(256) void bins_kernel(BinArgs a) {
   }
 }
 
+// ---------------------------------------------- several attributes, one FK
+// The attributes of one dimension read through one foreign-key column
+// (c_region, c_nation and c_city through lo_custkey): one pass over the keys
+// for all of them.  A table holds, per dimension row, the bin index of every
+// attribute packed in one 32-bit entry (8 bits for fewer than 255 bins, else
+// 16), so each fact row costs one key load and one table lookup however
+// many attributes it feeds -- where separate passes read the keys and look
+// the table up once per attribute.
+constexpr int kMaxSets = 4;
+struct MultiSet {
+  uint32_t nb, shift, bits;   // bins; field of the entry
+  uint32_t *out;              // nb bitmaps, stride_words apart
+};
+struct MultiArgs {
+  const uint32_t *fk;
+  uint32_t dimsz, nrows, nblocks, nsets;
+  size_t stride_words;
+  const uint32_t *tab;        // dimsz + 1 packed entries
+  MultiSet set[kMaxSets];
+};
+
+template <bool SORTED>
+__global__ __launch_bounds__(256) void dim_pack_kernel(BinArgs a, uint32_t shift,
+                                                       uint32_t bits,
+                                                       uint32_t *tab) {
+  __shared__ uint64_t s_bounds[SORTED ? kMaxSortedBounds : 1];
+  if (SORTED) {
+    for (uint32_t i = threadIdx.x; i <= a.nb; i += blockDim.x)
+      s_bounds[i] = a.mins[i];
+    __syncthreads();
+  }
+  const uint32_t none = (1u << bits) - 1u;
+  const size_t n = (size_t)a.src.dimsz + 1;
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (size_t)gridDim.x * blockDim.x) {
+    const uint64_t v = k < a.src.dimsz ? ld_elem(a.src.data, a.src.eb, k, true)
+                                       : (uint64_t)INT64_MAX;
+    const uint32_t i = bin_index(a, v, s_bounds);
+    const uint32_t f = i == kNone ? none : i;
+    // each attribute owns its bits: the entries start all zero and the
+    // attributes' kernels run one after the other on the stream
+    tab[k] |= f << shift;
+  }
+}
+
+__global__ __launch_bounds__(256) void bins_multi_kernel(MultiArgs a) {
+  const uint32_t lane = lane_id();
+  const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const uint32_t nwaves = gridDim.x * (blockDim.x >> 6);
+  u64x2 *tr = s_tr_dyn + (threadIdx.x >> 6) * 64 * 8;
+  for (uint32_t blk = wave; blk < a.nblocks; blk += nwaves) {
+    const size_t row0 = (size_t)blk * 1024;
+    uint32_t ent[16];
+    bool live[16];
+#pragma unroll
+    for (int g = 0; g < 16; ++g) {
+      const size_t row = row0 + 64 * g + lane;
+      live[g] = row < a.nrows;
+      // a key past the dimension reads entry dimsz (the bins of INT64_MAX)
+      const uint32_t k = live[g] ? min(ld_stream(a.fk + row), a.dimsz) : a.dimsz;
+      ent[g] = a.tab[k];
+    }
+    for (uint32_t si = 0; si < a.nsets; ++si) {
+      const MultiSet &ms = a.set[si];
+      const uint32_t none = (1u << ms.bits) - 1u;
+      BinArgs out;
+      out.out = ms.out;
+      out.stride_words = a.stride_words;
+      for (uint32_t cbase = 0; cbase < ms.nb; cbase += 64) {
+        const uint32_t inchunk = min(64u, ms.nb - cbase);
+        const uint32_t nbits = inchunk <= 1 ? 0 : 32 - __clz(inchunk - 1);
+        uint64_t w[16];
+#pragma unroll
+        for (int g = 0; g < 16; ++g) {
+          const uint32_t f = (ent[g] >> ms.shift) & none;
+          const uint32_t rel = f - cbase;   // none and other chunks: out of range
+          const bool inb = live[g] && f != none && rel < inchunk;
+          uint64_t acc = __ballot(inb);
+          for (uint32_t k = 0; k < nbits; ++k) {
+            const uint64_t bk = __ballot(inb && ((rel >> k) & 1u));
+            const uint64_t flip = ((lane >> k) & 1u) ? 0ull : ~0ull;
+            acc &= bk ^ flip;
+          }
+          w[g] = acc;
+        }
+        // a few bins: each lane its own line (no transpose round)
+        if (inchunk <= 8) {
+          if (lane < inchunk)
+            store_line(out.out + (size_t)(cbase + lane) * out.stride_words +
+                           (size_t)blk * 32,
+                       w);
+        } else {
+          store_lines(out, tr, w, cbase, inchunk, blk);
+        }
+      }
+    }
+  }
+}
+
+static int bin_mode(const uint64_t *mins, const uint64_t *maxs, size_t nb,
+                    BinArgs *a) {
+  bool consecutive = true, equal = true;
+  for (size_t j = 0; j < nb; ++j) {
+    if (!(mins[j] < maxs[j])) consecutive = false;
+    if (j + 1 < nb && maxs[j] != mins[j + 1]) consecutive = false;
+    if (maxs[j] - mins[j] != maxs[0] - mins[0]) equal = false;
+  }
+  const uint64_t w0 = maxs[0] - mins[0];
+  if (consecutive && equal && w0 == 1) {
+    a->mode = kUnit;
+    a->b0 = mins[0];
+  } else if (consecutive && equal && w0 != 0 && w0 < (1ull << 40) &&
+             (double)nb * (double)w0 < 4.0e15) {
+    a->mode = kEqual;
+    a->b0 = mins[0];
+    a->width = w0;
+    a->inv_width = 1.0 / (double)w0;
+  } else if (consecutive && nb + 1 <= kMaxSortedBounds) {
+    a->mode = kSorted;
+    uint32_t top = 1;
+    while (top * 2 <= nb + 1) top *= 2;
+    a->search_top = top;
+  } else {
+    a->mode = kGeneric;
+  }
+  return a->mode;
+}
+
 // Classifies the bin set and launches the kernel.  d_tmp (nb+1 or 2nb
 // uint64, device) is used for bounds the kernel must read from memory.
 static int launch_bins(mbx_ctx *ctx, const mbx_colref *src, uint32_t nrows,
@@ -480,6 +608,126 @@ int mbx_create_bin(mbx_ctx *ctx, const mbx_colref *src, uint32_t nrows,
     out[j] = static_cast<uint32_t *>(slab) + j * stride;
   if (single_bytes) *single_bytes = mbx_bitmap_words(nrows) * 4;
   return MBX_OK;
+}
+
+int mbx_create_bin_multi(mbx_ctx *ctx, const uint32_t *fk, uint32_t dimsz,
+                         uint32_t nrows, const mbx_binset *sets, size_t nsets,
+                         size_t stride_words) {
+  if (!ctx || !fk || !sets) return fail(MBX_ERR_INVALID, "create_bin_multi: null");
+  if (nsets == 0 || nrows == 0)
+    return fail(MBX_ERR_INVALID, "create_bin_multi: no sets or rows");
+  if (reinterpret_cast<uintptr_t>(fk) % 4)
+    return fail(MBX_ERR_ALIGN, "create_bin_multi: misaligned fk");
+  if (stride_words % 64 != 0 || stride_words < mbx_bitmap_words(nrows))
+    return fail(MBX_ERR_INVALID,
+                "create_bin_multi: stride_words must be a multiple of 64 and "
+                ">= ceil(N/32)");
+  for (size_t i = 0; i < nsets; ++i) {
+    const mbx_binset &b = sets[i];
+    if (!b.data || !b.mins || !b.maxs || !b.slab || b.nbins == 0)
+      return fail(MBX_ERR_INVALID, "create_bin_multi: bad set");
+    if (b.elem_bytes != 1 && b.elem_bytes != 2 && b.elem_bytes != 4 &&
+        b.elem_bytes != 8)
+      return fail(MBX_ERR_INVALID, "create_bin_multi: elem_bytes");
+    if (reinterpret_cast<uintptr_t>(b.slab) % 16)
+      return fail(MBX_ERR_ALIGN, "create_bin_multi: output not 16-byte aligned");
+  }
+  // the packed path: radix-able bin sets whose indexes fit 32 bits together
+  std::vector<BinArgs> args(nsets);
+  std::vector<uint32_t> bits(nsets);
+  uint32_t total = 0;
+  bool packed = nsets <= (size_t)kMaxSets && dimsz < 0x80000000u;
+  for (size_t i = 0; i < nsets && packed; ++i) {
+    BinArgs &a = args[i];
+    std::memset(&a, 0, sizeof(a));
+    a.src.data = sets[i].data;
+    a.src.fk = fk;
+    a.src.dimsz = dimsz;
+    a.src.eb = sets[i].elem_bytes;
+    a.nb = sets[i].nbins;
+    if (sets[i].nbins >= 0xffff ||
+        bin_mode(sets[i].mins, sets[i].maxs, sets[i].nbins, &a) == kGeneric)
+      packed = false;
+    bits[i] = sets[i].nbins < 0xff ? 8 : 16;
+    total += bits[i];
+  }
+  if (total > 32) packed = false;
+  void *d_tab = nullptr;
+  if (packed &&
+      ctx->pool->alloc((((size_t)dimsz + 1) * 4 + 255) & ~(size_t)255, &d_tab) != MBX_OK)
+    packed = false;
+  if (!packed) {   // one pass per attribute, as create_bin_into
+    for (size_t i = 0; i < nsets; ++i) {
+      mbx_colref r;
+      r.data = sets[i].data;
+      r.fk = fk;
+      r.dimsz = dimsz;
+      r.elem_bytes = sets[i].elem_bytes;
+      int st = launch_bins(ctx, &r, nrows, sets[i].mins, sets[i].maxs,
+                           sets[i].nbins, sets[i].slab, stride_words);
+      if (st) return st;
+    }
+    return MBX_OK;
+  }
+  uint32_t *tab = static_cast<uint32_t *>(d_tab);
+  int st = launch_fill_u32(ctx->stream, tab, 0, (size_t)dimsz + 1);
+  std::vector<void *> tmps;
+  MultiArgs m;
+  std::memset(&m, 0, sizeof(m));
+  m.fk = fk;
+  m.dimsz = dimsz;
+  m.nrows = nrows;
+  m.nblocks = (uint32_t)(stride_words / 32);
+  m.nsets = (uint32_t)nsets;
+  m.stride_words = stride_words;
+  m.tab = tab;
+  uint32_t shift = 0;
+  size_t tg = ceil_div((size_t)dimsz + 1, 256);
+  if (tg > (size_t)ctx->num_cus * 8) tg = (size_t)ctx->num_cus * 8;
+  for (size_t i = 0; i < nsets && !st; ++i) {
+    BinArgs &a = args[i];
+    if (a.mode == kSorted) {   // bounds on the device for the search
+      void *tmp = nullptr;
+      st = ctx->pool->alloc((a.nb + 1) * 8, &tmp);
+      if (st) break;
+      tmps.push_back(tmp);
+      std::vector<uint64_t> h(a.nb + 1);
+      for (uint32_t j = 0; j < a.nb; ++j) h[j] = sets[i].mins[j];
+      h[a.nb] = sets[i].maxs[a.nb - 1];
+      hipError_t e = hipMemcpyAsync(tmp, h.data(), h.size() * 8,
+                                    hipMemcpyHostToDevice, ctx->stream);
+      if (e != hipSuccess) {
+        st = hip_fail(e, "create_bin_multi: bounds upload");
+        break;
+      }
+      a.mins = static_cast<uint64_t *>(tmp);
+      hipLaunchKernelGGL(dim_pack_kernel<true>, dim3((unsigned)tg), dim3(256), 0,
+                         ctx->stream, a, shift, bits[i], tab);
+    } else {
+      hipLaunchKernelGGL(dim_pack_kernel<false>, dim3((unsigned)tg), dim3(256), 0,
+                         ctx->stream, a, shift, bits[i], tab);
+    }
+    m.set[i].nb = a.nb;
+    m.set[i].shift = shift;
+    m.set[i].bits = bits[i];
+    m.set[i].out = sets[i].slab;
+    shift += bits[i];
+  }
+  if (!st) {
+    size_t wgs = ceil_div(m.nblocks, 4);
+    if (wgs > (size_t)ctx->num_cus * 16) wgs = (size_t)ctx->num_cus * 16;
+    if (wgs == 0) wgs = 1;
+    hipLaunchKernelGGL(bins_multi_kernel, dim3((unsigned)wgs), dim3(256),
+                       kTransposeLds, ctx->stream, m);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) st = hip_fail(e, "create_bin_multi: launch");
+  }
+  // the table and bounds must outlive the kernels: wait first
+  hipError_t e2 = hipStreamSynchronize(ctx->stream);
+  for (void *t : tmps) ctx->pool->free(t);
+  ctx->pool->free(d_tab);
+  if (!st && e2 != hipSuccess) st = hip_fail(e2, "create_bin_multi");
+  return st;
 }
 
 int mbx_create_join(mbx_ctx *ctx, const uint32_t *fk, const void *dimattr,

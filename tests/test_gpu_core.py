@@ -409,3 +409,64 @@ def test_accumulate_and_group_count(dev):
         gpuq.free_all(dev, allocs)
     for c in schema.cols.values():
         c.dev.free()
+
+
+@pytest.mark.parametrize("n,dimsz", [(1_000_003, 5_000), (333, 77), (2_097_152, 300_000)])
+def test_create_bin_multi_equals_single(dev, n, dimsz):
+    """Several attributes through one foreign-key column in one pass
+    (mbx_create_bin_multi) write the same bytes as create_bin_into per
+    attribute (whose bytes the oracle pins): unit, equal-width and sorted
+    bins, a 300-bin set (16-bit index), keys past the dimension; and the
+    cases the packed path does not take (bins that are not consecutive,
+    indexes wider than 32 bits together) fall back to one pass per set."""
+    rng = np.random.default_rng(n)
+    fk = rng.integers(0, dimsz + dimsz // 10 + 1, n).astype(np.uint32)
+    a8 = rng.integers(0, 250, dimsz).astype(np.uint8)
+    a16 = rng.integers(0, 300, dimsz).astype(np.uint16)
+    a32 = rng.integers(0, 1 << 20, dimsz).astype(np.uint32)
+    d_fk, d8, d16, d32 = (dev.upload(x) for x in (fk, a8, a16, a32))
+    sw = (((n + 31) // 32 + 63) // 64) * 64
+    sorted_b = sorted(set(int(x) for x in rng.integers(0, 1 << 20, 40))) + [1 << 20]
+    cases = {
+        "packed": [(d8, 1, list(range(250)), list(range(1, 251))),
+                   (d16, 2, list(range(300)), list(range(1, 301))),
+                   (d32, 4, sorted_b[:-1], sorted_b[1:])],
+        "equal": [(d8, 1, list(range(0, 250, 10)), list(range(10, 260, 10))),
+                  (d32, 4, list(range(0, 1 << 20, 1 << 16)),
+                   list(range(1 << 16, (1 << 20) + 1, 1 << 16)))],
+        "generic": [(d8, 1, [0, 5, 100], [3, 50, 250]),
+                    (d16, 2, list(range(300)), list(range(1, 301)))],
+        "wide": [(d16, 2, list(range(300)), list(range(1, 301))),
+                 (d16, 2, list(range(0, 300, 1)), list(range(1, 301))),
+                 (d8, 1, list(range(250)), list(range(1, 251)))],
+    }
+    try:
+        for name, sets in cases.items():
+            outs = [dev.alloc(sw * 4 * len(lo)) for _, _, lo, _ in sets]
+            arr = (abi.BinSet * len(sets))()
+            keep = []
+            for i, ((col, eb, lo, hi), o) in enumerate(zip(sets, outs)):
+                L, H = (C.c_uint64 * len(lo))(*lo), (C.c_uint64 * len(hi))(*hi)
+                keep += [L, H]
+                arr[i].data, arr[i].elem_bytes, arr[i].nbins = col.ptr, eb, len(lo)
+                arr[i].mins = C.cast(L, C.POINTER(C.c_uint64))
+                arr[i].maxs = C.cast(H, C.POINTER(C.c_uint64))
+                arr[i].slab = o.ptr
+            check(dev.lib.mbx_create_bin_multi(dev.ctx, d_fk.ptr, dimsz, n, arr,
+                                               len(sets), sw))
+            for (col, eb, lo, hi), o in zip(sets, outs):
+                ref = abi.ColRef(col.ptr, d_fk.ptr, dimsz, eb)
+                single = dev.alloc(sw * 4 * len(lo))
+                check(dev.lib.mbx_create_bin_into(
+                    dev.ctx, C.byref(ref), n, (C.c_uint64 * len(lo))(*lo),
+                    (C.c_uint64 * len(hi))(*hi), len(lo), single.ptr, sw))
+                dev.sync()
+                np.testing.assert_array_equal(o.host(dtype=np.uint32),
+                                              single.host(dtype=np.uint32),
+                                              err_msg=f"{name} n={n}")
+                single.free()
+            for o in outs:
+                o.free()
+    finally:
+        for x in (d_fk, d8, d16, d32):
+            x.free()
