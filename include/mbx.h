@@ -214,10 +214,14 @@ enum mbx_flags {
   MBX_PLAN_EAGER = 32,       /* fused/materialise: load every bitmap operand
                                 up front instead of skipping ANDM loads for
                                 empty words (same bits, fewer round trips) */
-  MBX_PLAN_DIRECT = 64       /* fused, ablation: no compaction -- each lane
+  MBX_PLAN_DIRECT = 64,      /* fused, ablation: no compaction -- each lane
                                 walks the set bits of its own words and runs
                                 the row op on them (abla::method1,
                                 primitive_abla.cuh:62-108) */
+  MBX_PLAN_BASEFUSE = 128    /* fused, ablation: method 1 with the program's
+                                virtual registers in device memory
+                                (abla::base_fuse, primitive_abla.cuh:15-59);
+                                waits for its kernel (pool scratch) */
 };
 
 /* Program + group-by in one call.  d_out[nr_grp] int64 on the device.
@@ -246,6 +250,11 @@ int mbx_prog_grpby_noalloc(mbx_ctx *ctx, const mbx_vprg *prg,
  * mbx_rows_grpby reads the count on the device (no host round trip). */
 int mbx_prog_eval(mbx_ctx *ctx, const mbx_vprg *prg, uint32_t *d_poss,
                   uint32_t *d_unc);
+/* flags & MBX_PLAN_BASEFUSE: the ablation with the registers in device
+ * memory (the reference report's NfuStg1: _stg1 with idxbuf,
+ * primitive_abla.cuh:262-288); waits for its kernel. */
+int mbx_prog_eval_ex(mbx_ctx *ctx, const mbx_vprg *prg, uint32_t *d_poss,
+                     uint32_t *d_unc, uint32_t flags);
 int mbx_compact(mbx_ctx *ctx, const uint32_t *d_poss, const uint32_t *d_unc,
                 uint32_t nrows, uint32_t *d_rows, uint32_t *d_count);
 int mbx_rows_grpby(mbx_ctx *ctx, const uint32_t *d_rows,

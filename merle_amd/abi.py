@@ -23,6 +23,7 @@ PLAN_MASK = 3
 ACCUMULATE = 16
 PLAN_EAGER = 32
 PLAN_DIRECT = 64
+PLAN_BASEFUSE = 128
 
 
 class ColRef(C.Structure):
@@ -153,6 +154,7 @@ PROTOTYPES = {
     "mbx_prog_grpby_noalloc": (_i32, [_vp, _P(Vprg), _P(RowOp), _u32, _vp,
                                       _u32]),
     "mbx_prog_eval": (_i32, [_vp, _P(Vprg), _vp, _vp]),
+    "mbx_prog_eval_ex": (_i32, [_vp, _P(Vprg), _vp, _vp, _u32]),
     "mbx_compact": (_i32, [_vp, _vp, _vp, _u32, _vp, _vp]),
     "mbx_rows_grpby": (_i32, [_vp, _vp, _vp, _P(Vprg), _P(RowOp), _u32, _vp,
                               _u32]),

@@ -22,9 +22,12 @@
 // the WAH / RTScan block with its header and `nan` values (WAH and RTScan
 // are not built).  SSB columns: Join = the bitmap-free scan (Sparse rows
 // only, as DOWORK), Ours = fused under the library's static plan, PrgStg1 /
-// Stg2 / Stg3 = the three materialise stages; the plans this library does
-// not build (Basefus, NfuStg1, Dedicat) read 0.  Every plan's int64 sums
-// must equal the scan's, or the driver exits with status 1.  With a
+// Stg2 / Stg3 = the three materialise stages, Basefus = the base_fuse
+// ablation (method 1 with the program's registers in device memory),
+// NfuStg1 = the first stage with its registers in device memory; the
+// dedicated per-query kernels (Dedicat) are not built and read 0.  Every
+// plan's int64 sums must equal the scan's, or the driver exits with status
+// 1.  With a
 // sums_file the driver also writes every SSB row's sums there, one line per
 // (flight, bin type, plan): name, bin type, plan (join / ours / stg), then
 // the sums, tab-separated.
@@ -658,6 +661,32 @@ int main(int argc, char **argv) {
     }
     return sum / reps;
   };
+  // Basefus: the base_fuse ablation (method 1, the program's registers in
+  // device memory); NfuStg1: the first materialise stage with its registers
+  // in device memory (the reference's _stg1 with idxbuf).  Both under the
+  // static plan's order; Basefus's sums must equal the scan's too.
+  auto ablations = [&](const mbx_vprg &ap0, const mbx_rowop &op0, uint32_t g,
+                       const mbx_plan &plan, float &basefus, float &nfu1) {
+    mbx_vprg ap;
+    if (mbx_plan_apply(&ap0, &plan, &ap) != MBX_OK) {
+      std::fprintf(stderr, "plan: %s\n", mbx_last_error());
+      return false;
+    }
+    basefus = timed([&] {
+      return mbx_prog_grpby(ctx, &ap, &op0, g, out,
+                            MBX_PLAN_FUSED | MBX_PLAN_BASEFUSE);
+    });
+    mbx_memcpy_d2h(ctx, h_got.data(), out, 8 * g);
+    bad += std::memcmp(h_want.data(), h_got.data(), 8 * g) != 0;
+    bool chk = false;
+    for (int c = 0; c < MBX_MAXCOLS; ++c)
+      chk |= ap.cols[c].leftmost || ap.cols[c].rightmost;
+    nfu1 = timed([&] {
+      return mbx_prog_eval_ex(ctx, &ap, poss, chk ? unc : nullptr,
+                              MBX_PLAN_BASEFUSE);
+    });
+    return true;
+  };
   const char *kPlan[3] = {"Sparse", "Medium", "Dense"};
   for (const Flight &f : kFlights) {
     mbx_rowop op;
@@ -711,6 +740,8 @@ int main(int argc, char **argv) {
       bad += std::memcmp(h_want.data(), h_got.data(), 8 * g) != 0;
       write_sums(f.name, kPlan[level], "join", h_want.data(), g);
       write_sums(f.name, kPlan[level], "ours", h_got.data(), g);
+      float basefus = 0, nfu1 = 0;
+      if (!ablations(ap, op, g, plan, basefus, nfu1)) return 2;
       const float s1 = timed([&] {
         return mbx_prog_eval(ctx, &ap, poss, p.is_nochk() ? nullptr : unc);
       });
@@ -728,7 +759,8 @@ int main(int argc, char **argv) {
         std::printf("\n%s\t%s\t%.4f", f.name, kPlan[level], join);
       else
         std::printf("\n%s\t%s\t0", f.name, kPlan[level]);
-      std::printf("\t%.4f\t0\t0\t%.4f\t%.4f\t%.4f\t0", ours, s1, s2, s3);
+      std::printf("\t%.4f\t%.4f\t%.4f\t%.4f\t%.4f\t%.4f\t0", ours, basefus,
+                  nfu1, s1, s2, s3);
     }
   }
   // the controlled layouts (ssb/controlled-demo.cu:7-129): each flight's
@@ -787,6 +819,8 @@ int main(int argc, char **argv) {
       bad += std::memcmp(h_want.data(), h_got.data(), 8 * g) != 0;
       write_sums(f.name, kRecipe[kind], "join", h_want.data(), g);
       write_sums(f.name, kRecipe[kind], "ours", h_got.data(), g);
+      float basefus = 0, nfu1 = 0;
+      if (!ablations(ap, op, g, plan, basefus, nfu1)) return 2;
       const float s1 = timed([&] {
         return mbx_prog_eval(ctx, &ap, poss, p.is_nochk() ? nullptr : unc);
       });
@@ -800,8 +834,8 @@ int main(int argc, char **argv) {
       mbx_memcpy_d2h(ctx, h_got.data(), out, 8 * g);
       bad += std::memcmp(h_want.data(), h_got.data(), 8 * g) != 0;
       write_sums(f.name, kRecipe[kind], "stg", h_got.data(), g);
-      std::printf("\n%s\t%s\t0\t%.4f\t0\t0\t%.4f\t%.4f\t%.4f\t0", f.name,
-                  kRecipe[kind], ours, s1, s2, s3);
+      std::printf("\n%s\t%s\t0\t%.4f\t%.4f\t%.4f\t%.4f\t%.4f\t%.4f\t0",
+                  f.name, kRecipe[kind], ours, basefus, nfu1, s1, s2, s3);
       mbx_sync(ctx);
       p.release();
     }

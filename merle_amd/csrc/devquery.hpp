@@ -331,6 +331,51 @@ __device__ __forceinline__ void eval_words(const DevPrg &P, uint32_t w0,
     }
     return;
   }
+  if (EAGER == 4) {
+    // EAGER 4, the ablation of where the registers live (abla::base_fuse,
+    // primitive_abla.cuh:15-59): the early-out interpreter of EAGER 0, with
+    // the virtual registers of each word in device memory (P.greg) -- every
+    // instruction reads its registers from there and writes them back.
+    uint64_t *gr[WPL];
+#pragma unroll
+    for (int k = 0; k < WPL; ++k) {
+      gr[k] = P.greg + (size_t)(in[k] ? w0 + ws * k : 0) * 6;
+      if (in[k])
+#pragma unroll
+        for (int r = 0; r < 6; ++r) gr[k][r] = 0;
+    }
+#pragma unroll
+    for (uint32_t i = 0; i < MBX_MAXNINSTR; ++i) {
+      if (i >= P.nins) break;
+      const uint32_t op = P.ins[i][0], d = P.ins[i][1], s = P.ins[i][2];
+      uint64_t cs[1][WPL], us[1][WPL];
+#pragma unroll
+      for (int k = 0; k < WPL; ++k) cs[0][k] = us[0][k] = 0;
+      if (op == MBX_ANDM || op == MBX_ORM) {
+        bool ld[WPL];
+#pragma unroll
+        for (int k = 0; k < WPL; ++k)
+          ld[k] = in[k] && (op == MBX_ORM || gr[k][d % 3] != 0);
+        load_ops<ACCT, CHK, WPL, 1>(P, P.beg[s], P.end[s], w0, ws, ld, cs, us,
+                                    acct);
+      }
+#pragma unroll
+      for (int k = 0; k < WPL; ++k) {
+        if (!in[k]) continue;
+        uint64_t p0 = gr[k][0], p1 = gr[k][1], p2 = gr[k][2];
+        uint64_t u0 = gr[k][3], u1 = gr[k][4], u2 = gr[k][5];
+        apply_ins<CHK>(op, d, s, cs[0][k], us[0][k], p0, p1, p2, u0, u1, u2);
+        gr[k][0] = p0, gr[k][1] = p1, gr[k][2] = p2;
+        gr[k][3] = u0, gr[k][4] = u1, gr[k][5] = u2;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < WPL; ++k) {
+      po[k] = in[k] ? gr[k][0] & keep[k] : 0;
+      uo[k] = (CHK && in[k]) ? gr[k][3] & keep[k] : 0;
+    }
+    return;
+  }
   if (EAGER == 2) {
     // EAGER 2: the flat list holds <= kDirectOps bitmaps; v[e][k] is entry e
     // at word k; a column's operand is the OR of its entries [beg, end)

@@ -136,6 +136,10 @@ struct DevPrg {
   // columns whose predicate the re-check of a candidate row need not load
   // (bit c), and the value it takes for them (known_val bit c)
   uint32_t known, known_val;
+  // ablation only (abla::base_fuse / _stg1 with idxbuf): the program's
+  // registers in device memory, 6 uint64 per 64-row word (P r0..r2, then
+  // U r0..r2); null in every product plan
+  uint64_t *greg;
 };
 struct DevSrc {
   const void *data;

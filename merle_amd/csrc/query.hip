@@ -75,14 +75,16 @@ __global__ __launch_bounds__(256) void scan_kernel(QArgs a) {
 
 // ------------------------------------------------------- materialise plan
 // Stage 1: program -> possible/uncertain bitmaps in HBM.
-template <bool CHK>
+// GREG: the ablation with the registers in device memory (NfuStg1 of the
+// reference's report: _stg1 with idxbuf, primitive_abla.cuh:262-288).
+template <bool CHK, bool GREG = false>
 __global__ __launch_bounds__(256) void stage1_kernel(DevPrg P,
                                                      uint64_t *__restrict__ po,
                                                      uint64_t *__restrict__ uo) {
   for (uint32_t w = blockIdx.x * 256 + threadIdx.x; w < P.nwords;
        w += gridDim.x * 256) {
     uint64_t p, u;
-    eval_word<CHK, false, 1>(P, w, p, u, nullptr);
+    eval_word<CHK, false, GREG ? 4 : 1>(P, w, p, u, nullptr);
     po[w] = p;
     if (CHK) uo[w] = u;
   }
@@ -391,16 +393,17 @@ static void fused_conj(mbx_ctx *ctx, hipStream_t s, const QArgs &a,
 
 // The no-compaction ablation: early-out evaluation, 2 words per lane, the
 // resident grid.
-template <bool C, bool L>
+// E = 4: the base_fuse ablation (registers in device memory, P.greg).
+template <bool C, bool L, int E = 0>
 static void fused_direct(mbx_ctx *ctx, hipStream_t s, const QArgs &a,
                          size_t lds) {
   const void *fn =
-      reinterpret_cast<const void *>(&fused_kernel<C, L, false, 2, 0, true>);
+      reinterpret_cast<const void *>(&fused_kernel<C, L, false, 2, E, true>);
   const unsigned per_cu = occupancy(fn, lds);
   const uint64_t waves = ceil_div(ceil_div(a.prg.nwords, 64), 2);
   const uint64_t resident = (uint64_t)ctx->num_cus * per_cu * 4;
   const unsigned grid = (unsigned)ceil_div(waves < resident ? waves : resident, 4);
-  hipLaunchKernelGGL((fused_kernel<C, L, false, 2, 0, true>), dim3(grid),
+  hipLaunchKernelGGL((fused_kernel<C, L, false, 2, E, true>), dim3(grid),
                      dim3(256), lds, s, a);
 }
 
@@ -433,6 +436,11 @@ int launch_fused(mbx_ctx *ctx, const QueryPlan &q, const Acct *acct) {
   if (acct) {
     if (chk) fused_launch<true, false, true, 0>(ctx, s, a, lds);
     else fused_launch<false, false, true, 0>(ctx, s, a, lds);
+  } else if (q.direct && q.prg.greg) {
+    if (chk && q.lds) fused_direct<true, true, 4>(ctx, s, a, lds);
+    else if (chk) fused_direct<true, false, 4>(ctx, s, a, lds);
+    else if (q.lds) fused_direct<false, true, 4>(ctx, s, a, lds);
+    else fused_direct<false, false, 4>(ctx, s, a, lds);
   } else if (q.direct) {
     if (chk && q.lds) fused_direct<true, true>(ctx, s, a, lds);
     else if (chk) fused_direct<true, false>(ctx, s, a, lds);
@@ -481,7 +489,13 @@ int launch_scan(mbx_ctx *ctx, const QueryPlan &q) {
 int launch_stage1(mbx_ctx *ctx, const DevPrg &P, uint64_t *po, uint64_t *uo) {
   const unsigned grid =
       persistent_grid(ctx, (uint32_t)ceil_div(P.nwords, 256), 16);
-  if (P.chk)
+  if (P.greg && P.chk)
+    hipLaunchKernelGGL((stage1_kernel<true, true>), dim3(grid), dim3(256), 0,
+                       ctx->stream, P, po, uo);
+  else if (P.greg)
+    hipLaunchKernelGGL((stage1_kernel<false, true>), dim3(grid), dim3(256), 0,
+                       ctx->stream, P, po, uo);
+  else if (P.chk)
     hipLaunchKernelGGL(stage1_kernel<true>, dim3(grid), dim3(256), 0,
                        ctx->stream, P, po, uo);
   else
@@ -649,6 +663,20 @@ int mbx_prog_grpby(mbx_ctx *ctx, const mbx_vprg *prg, const mbx_rowop *op,
   }
   q.eager = (flags & MBX_PLAN_EAGER) != 0;
   q.direct = (flags & MBX_PLAN_DIRECT) != 0;
+  if ((flags & MBX_PLAN_BASEFUSE) && (flags & MBX_PLAN_MASK) == MBX_PLAN_FUSED) {
+    // the ablation: method 1 with the registers in device memory, which
+    // are pool scratch freed after a wait
+    void *greg = nullptr;
+    st = ctx->pool->alloc((size_t)q.prg.nwords * 48, &greg);
+    if (st) return st;
+    q.prg.greg = static_cast<uint64_t *>(greg);
+    q.direct = true;
+    st = launch_fused(ctx, q, nullptr);
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    ctx->pool->free(greg);
+    if (!st && e != hipSuccess) st = hip_fail(e, "base_fuse ablation");
+    return st;
+  }
   switch (flags & MBX_PLAN_MASK) {
   case MBX_PLAN_FUSED: return launch_fused(ctx, q, nullptr);
   case MBX_PLAN_MATERIALISE: return run_materialise(ctx, q);
@@ -680,6 +708,9 @@ int mbx_prog_grpby_noalloc(mbx_ctx *ctx, const mbx_vprg *prg,
                     " rows in its " + std::to_string(kNoAllocScratch >> 20) +
                     " MiB scratch");
   if (plan > MBX_PLAN_SCAN) return fail(MBX_ERR_INVALID, "unknown plan");
+  if (flags & MBX_PLAN_BASEFUSE)
+    return fail(MBX_ERR_INVALID,
+                "prog_grpby_noalloc: the base_fuse ablation needs scratch");
   if (!(flags & MBX_ACCUMULATE)) {
     st = launch_fill_u64(ctx->stream, d_out, 0, nr_grp);
     if (st) return st;
@@ -699,8 +730,8 @@ int mbx_prog_grpby_noalloc(mbx_ctx *ctx, const mbx_vprg *prg,
   return st;
 }
 
-int mbx_prog_eval(mbx_ctx *ctx, const mbx_vprg *prg, uint32_t *d_poss,
-                  uint32_t *d_unc) {
+int mbx_prog_eval_ex(mbx_ctx *ctx, const mbx_vprg *prg, uint32_t *d_poss,
+                     uint32_t *d_unc, uint32_t flags) {
   if (!ctx || !d_poss) return fail(MBX_ERR_INVALID, "prog_eval: null");
   DevPrg P;
   int st = make_dev_prg(prg, &P);
@@ -710,8 +741,24 @@ int mbx_prog_eval(mbx_ctx *ctx, const mbx_vprg *prg, uint32_t *d_poss,
     return fail(MBX_ERR_ALIGN, "prog_eval: outputs not 8-byte aligned");
   if (P.chk && !d_unc)
     return fail(MBX_ERR_INVALID, "prog_eval: program needs d_unc");
-  return launch_stage1(ctx, P, reinterpret_cast<uint64_t *>(d_poss),
-                       reinterpret_cast<uint64_t *>(d_unc));
+  if (!(flags & MBX_PLAN_BASEFUSE))
+    return launch_stage1(ctx, P, reinterpret_cast<uint64_t *>(d_poss),
+                         reinterpret_cast<uint64_t *>(d_unc));
+  void *greg = nullptr;   // the ablation's registers, freed after a wait
+  st = ctx->pool->alloc((size_t)P.nwords * 48, &greg);
+  if (st) return st;
+  P.greg = static_cast<uint64_t *>(greg);
+  st = launch_stage1(ctx, P, reinterpret_cast<uint64_t *>(d_poss),
+                     reinterpret_cast<uint64_t *>(d_unc));
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  ctx->pool->free(greg);
+  if (!st && e != hipSuccess) st = hip_fail(e, "prog_eval ablation");
+  return st;
+}
+
+int mbx_prog_eval(mbx_ctx *ctx, const mbx_vprg *prg, uint32_t *d_poss,
+                  uint32_t *d_unc) {
+  return mbx_prog_eval_ex(ctx, prg, d_poss, d_unc, 0);
 }
 
 int mbx_compact(mbx_ctx *ctx, const uint32_t *d_poss, const uint32_t *d_unc,

@@ -213,21 +213,35 @@ def test_random_queries_all_plans(dev, seed):
     """Random programs (AND/OR/NOT/ANDM/ORM, exact and edge bins), group
     counts from 1 to >4096 (LDS table and HBM atomics): fused (early-out,
     eager -- the conjunction form for two- and three-valued conjunctions --
-    and the no-compaction ablation) == materialise == scan == oracle,
-    bit-exact int64."""
+    and the no-compaction and registers-in-memory ablations) == materialise
+    == scan == oracle, bit-exact int64; the first stage with its registers
+    in device memory writes the same bitmaps as the product one."""
     rng = np.random.default_rng(1000 + seed)
     n = int(rng.choice([1, 63, 64, 65, 5000, 70001, 300007]))
     schema = gpuq.upload_schema(dev, datagen.small_schema(n, seed))
+    nw = ((n + 63) // 64) * 2
+    poss = [dev.alloc(8 * nw, np.uint32, 2 * nw) for _ in range(4)]
     for _ in range(6):
         spec = datagen.random_spec(rng, schema, extra_groups=int(rng.integers(0, 3)))
         want, _ = cpuref.query(spec, schema)
         cols, allocs = gpuq.plan_cols(dev, schema, spec, rng)
         for plan in (abi.PLAN_FUSED, abi.PLAN_FUSED | abi.PLAN_EAGER,
                      abi.PLAN_FUSED | abi.PLAN_DIRECT,
+                     abi.PLAN_FUSED | abi.PLAN_BASEFUSE,
                      abi.PLAN_MATERIALISE, abi.PLAN_SCAN):
             got = gpuq.run_query(dev, schema, spec, cols, plan)
             np.testing.assert_array_equal(got, want, err_msg=f"plan {plan} {spec}")
+        prg = S.to_vprg(cols, n, spec.instrs)
+        for i, fl in ((0, 0), (2, abi.PLAN_BASEFUSE)):
+            check(dev.lib.mbx_prog_eval_ex(dev.ctx, C.byref(prg), poss[i].ptr,
+                                           poss[i + 1].ptr, fl))
+        dev.sync()
+        np.testing.assert_array_equal(poss[2].host()[:nw], poss[0].host()[:nw])
+        if any(c.leftmost or c.rightmost for c in cols):
+            np.testing.assert_array_equal(poss[3].host()[:nw], poss[1].host()[:nw])
         gpuq.free_all(dev, allocs)
+    for p in poss:
+        p.free()
     for c in schema.cols.values():
         c.dev.free()
 

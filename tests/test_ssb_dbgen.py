@@ -257,7 +257,8 @@ def test_demo_driver_report(ssb_host, tmp_path_factory):
         assert f[1] == (("Sparse", "Medium", "Dense") if fixed else
                         ("Perfect", "AllAlig", "MisAlig"))[i % 3]
         assert len(f) == 10 and float(f[3]) > 0
-        assert all(float(x) > 0 for x in f[6:9])   # the materialise stages
+        assert all(float(x) > 0 for x in f[4:9])   # ablations and stages
+        assert f[9] == "0"                         # no dedicated kernels
         # DOWORK times the scan on the fixed layouts' Sparse row only
         # (ssb/lambdas.cuh:157); the controlled rows print 0 there
         assert (float(f[2]) > 0) if fixed and i % 3 == 0 else f[2] == "0"
