@@ -201,30 +201,50 @@ __global__ __launch_bounds__(256) void bins_kernel(BinArgs a) {
     uint64_t v[16];
     uint32_t ok = 0;
     uint32_t idx[16];
+    // Every load is issued unconditionally (rows past N read row N-1) and
+    // the tail is masked afterwards: a load under `if (row < N)` becomes a
+    // branch whose join waits for it, which serialised the 16 key loads.
+    const size_t last = (size_t)a.nrows - 1;
     if constexpr (TAB != 0) {
       using T = typename std::conditional<TAB == 1, uint8_t, uint16_t>::type;
       const T *tab = LT ? reinterpret_cast<const T *>(s_tab)
                         : static_cast<const T *>(a.tab);
+      uint32_t key[16];
+#pragma unroll
+      for (int g = 0; g < 16; ++g)
+        key[g] = ld_stream(a.src.fk + min(row0 + 64 * g + lane, last));
 #pragma unroll
       for (int g = 0; g < 16; ++g) {
-        const size_t row = row0 + 64 * g + lane;
-        idx[g] = kNone;
-        if (row < a.nrows) {
-          const uint32_t k = min(ld_stream(a.src.fk + row), a.src.dimsz);
-          const uint32_t t = tab[k];
-          if (t != (uint32_t)(T)~(T)0) idx[g] = t;
-        }
+        const uint32_t t = tab[min(key[g], a.src.dimsz)];
+        const bool live = row0 + 64 * g + lane <= last;
+        idx[g] = (live && t != (uint32_t)(T)~(T)0) ? t : kNone;
       }
     } else {
+      if (a.src.fk == nullptr) {
 #pragma unroll
-      for (int g = 0; g < 16; ++g) {
-        const size_t row = row0 + 64 * g + lane;
-        v[g] = 0;
-        if (row < a.nrows) {
-          v[g] = bin_value(a.src, row);
-          ok |= 1u << g;
+        for (int g = 0; g < 16; ++g)
+          v[g] = ld_elem(a.src.data, a.src.eb, min(row0 + 64 * g + lane, last),
+                         true);
+      } else {
+        // a key past the dimension reads as INT64_MAX (create_bin's fetch,
+        // bmpidx_create.cu:17-35); its load reads row dimsz - 1
+        uint32_t key[16];
+#pragma unroll
+        for (int g = 0; g < 16; ++g)
+          key[g] = ld_stream(a.src.fk + min(row0 + 64 * g + lane, last));
+        const uint32_t dl = a.src.dimsz ? a.src.dimsz - 1 : 0;
+#pragma unroll
+        for (int g = 0; g < 16; ++g) {
+          const uint64_t x = a.src.dimsz
+                                 ? ld_elem(a.src.data, a.src.eb,
+                                           min(key[g], dl), false)
+                                 : 0;
+          v[g] = key[g] < a.src.dimsz ? x : (uint64_t)INT64_MAX;
         }
       }
+#pragma unroll
+      for (int g = 0; g < 16; ++g)
+        if (row0 + 64 * g + lane <= last) ok |= 1u << g;
       if (RADIX) {
 #pragma unroll
         for (int g = 0; g < 16; ++g)
@@ -318,56 +338,117 @@ __global__ __launch_bounds__(256) void dim_pack_kernel(BinArgs a, uint32_t shift
   }
 }
 
+// Lines of one attribute with more than 8 bins go out through a per-wave
+// LDS image of up to kScatterBins lines (128 bytes = 1024 rows each): the
+// lane of each row ORs the row's bit into its bin's line (ds_or_b32), then
+// the wave streams the lines out, 8 lanes per line.  The cost per 1024 rows
+// is one LDS atomic per row plus a zero and a read of each line, against
+// 1 + log2(64) ballots and four 64-bit ALU operations per ballot for every
+// 64 rows and every 64 bins of the ballot transpose.
+// Word w of line b sits at b*32 + (w ^ (b & 31)): the 32 lanes of one
+// ds_or_b32, which share w, reach 32 banks when their bins differ mod 32.
+constexpr uint32_t kScatterBins = 64;
+constexpr size_t kScatterLds = 4 * kScatterBins * 128;
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void scatter_lines(const MultiSet &ms, uint32_t none,
+                                              const uint32_t *ent,
+                                              const bool *live, uint32_t *img,
+                                              size_t stride_words, uint32_t blk) {
+  const uint32_t lane = lane_id();
+  u32x4 *img4 = reinterpret_cast<u32x4 *>(img);
+  const uint32_t bit = 1u << (lane & 31u);
+  for (uint32_t cbase = 0; cbase < ms.nb; cbase += kScatterBins) {
+    const uint32_t inchunk = min(kScatterBins, ms.nb - cbase);
+    const u32x4 zero = {0u, 0u, 0u, 0u};
+    for (uint32_t s = lane; s < inchunk * 8; s += 64) img4[s] = zero;
+    wave_lds_sync();
+#pragma unroll
+    for (int g = 0; g < 16; ++g) {
+      const uint32_t f = (ent[g] >> ms.shift) & none;
+      const uint32_t rel = f - cbase;   // none and other chunks: out of range
+      if (live[g] && f != none && rel < inchunk) {
+        const uint32_t w = 2u * (uint32_t)g + (lane >> 5);
+        atomicOr(img + rel * 32 + (w ^ (rel & 31u)), bit);
+      }
+    }
+    wave_lds_sync();
+    for (uint32_t s = lane; s < inchunk * 8; s += 64) {
+      const uint32_t line = s >> 3, part = s & 7u, sw = line & 31u;
+      u32x4 v = img4[line * 8 + (part ^ (sw >> 2))];
+      // logical word 4*part + i sits at position i ^ (sw & 3)
+      if (sw & 1u) v = v.yxwz;
+      if (sw & 2u) v = v.zwxy;
+      u32x4 *dst = reinterpret_cast<u32x4 *>(
+          ms.out + (size_t)(cbase + line) * stride_words + (size_t)blk * 32);
+      __builtin_nontemporal_store(v, dst + part);
+    }
+    wave_lds_sync();   // the next chunk zeroes the image
+  }
+}
+
+// SCATTER: some set has more than 8 bins (the launch gives each wave
+// kScatterBins lines of dynamic LDS).
+template <bool SCATTER>
 __global__ __launch_bounds__(256) void bins_multi_kernel(MultiArgs a) {
   const uint32_t lane = lane_id();
   const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const uint32_t nwaves = gridDim.x * (blockDim.x >> 6);
-  u64x2 *tr = s_tr_dyn + (threadIdx.x >> 6) * 64 * 8;
+  uint32_t *img = reinterpret_cast<uint32_t *>(s_tr_dyn) +
+                  (threadIdx.x >> 6) * kScatterBins * 32;
+  // Loads are issued unconditionally, the tail masked after (as
+  // bins_kernel), and the keys of the wave's next block are loaded while
+  // this block's lines go out: a block waits for one round trip (its table
+  // entries), not two.
+  const size_t last = (size_t)a.nrows - 1;
+  uint32_t key[16];
+  if (wave < a.nblocks) {
+#pragma unroll
+    for (int g = 0; g < 16; ++g)
+      key[g] = ld_stream(a.fk + min((size_t)wave * 1024 + 64 * g + lane, last));
+  }
   for (uint32_t blk = wave; blk < a.nblocks; blk += nwaves) {
     const size_t row0 = (size_t)blk * 1024;
     uint32_t ent[16];
     bool live[16];
 #pragma unroll
     for (int g = 0; g < 16; ++g) {
-      const size_t row = row0 + 64 * g + lane;
-      live[g] = row < a.nrows;
+      live[g] = row0 + 64 * g + lane <= last;
       // a key past the dimension reads entry dimsz (the bins of INT64_MAX)
-      const uint32_t k = live[g] ? min(ld_stream(a.fk + row), a.dimsz) : a.dimsz;
-      ent[g] = a.tab[k];
+      ent[g] = a.tab[min(key[g], a.dimsz)];
+    }
+    if (blk + nwaves < a.nblocks) {
+      const size_t next0 = (size_t)(blk + nwaves) * 1024;
+#pragma unroll
+      for (int g = 0; g < 16; ++g)
+        key[g] = ld_stream(a.fk + min(next0 + 64 * g + lane, last));
     }
     for (uint32_t si = 0; si < a.nsets; ++si) {
       const MultiSet &ms = a.set[si];
       const uint32_t none = (1u << ms.bits) - 1u;
-      BinArgs out;
-      out.out = ms.out;
-      out.stride_words = a.stride_words;
-      for (uint32_t cbase = 0; cbase < ms.nb; cbase += 64) {
-        const uint32_t inchunk = min(64u, ms.nb - cbase);
-        const uint32_t nbits = inchunk <= 1 ? 0 : 32 - __clz(inchunk - 1);
-        uint64_t w[16];
-#pragma unroll
-        for (int g = 0; g < 16; ++g) {
-          const uint32_t f = (ent[g] >> ms.shift) & none;
-          const uint32_t rel = f - cbase;   // none and other chunks: out of range
-          const bool inb = live[g] && f != none && rel < inchunk;
-          uint64_t acc = __ballot(inb);
-          for (uint32_t k = 0; k < nbits; ++k) {
-            const uint64_t bk = __ballot(inb && ((rel >> k) & 1u));
-            const uint64_t flip = ((lane >> k) & 1u) ? 0ull : ~0ull;
-            acc &= bk ^ flip;
-          }
-          w[g] = acc;
-        }
-        // a few bins: each lane its own line (no transpose round)
-        if (inchunk <= 8) {
-          if (lane < inchunk)
-            store_line(out.out + (size_t)(cbase + lane) * out.stride_words +
-                           (size_t)blk * 32,
-                       w);
-        } else {
-          store_lines(out, tr, w, cbase, inchunk, blk);
-        }
+      if (SCATTER && ms.nb > 8) {
+        scatter_lines(ms, none, ent, live, img, a.stride_words, blk);
+        continue;
       }
+      // a few bins (every set, without SCATTER): radix ballots, each lane
+      // its own line
+      const uint32_t nbits = ms.nb <= 1 ? 0 : 32 - __clz(ms.nb - 1);
+      uint64_t w[16];
+#pragma unroll
+      for (int g = 0; g < 16; ++g) {
+        const uint32_t f = (ent[g] >> ms.shift) & none;
+        const bool inb = live[g] && f < ms.nb;
+        uint64_t acc = __ballot(inb);
+        for (uint32_t k = 0; k < nbits; ++k) {
+          const uint64_t bk = __ballot(inb && ((f >> k) & 1u));
+          const uint64_t flip = ((lane >> k) & 1u) ? 0ull : ~0ull;
+          acc &= bk ^ flip;
+        }
+        w[g] = acc;
+      }
+      if (lane < ms.nb)
+        store_line(ms.out + (size_t)lane * a.stride_words + (size_t)blk * 32, w);
     }
   }
 }
@@ -408,6 +489,7 @@ static int launch_bins(mbx_ctx *ctx, const mbx_colref *src, uint32_t nrows,
                        uint32_t *slab, size_t stride_words) {
   int st = check_src(*src, "create_bin");
   if (st) return st;
+  if (nrows == 0) return fail(MBX_ERR_INVALID, "create_bin: 0 rows");
   if (src->fk && reinterpret_cast<uintptr_t>(src->fk) % 4)
     return fail(MBX_ERR_ALIGN, "create_bin: misaligned fk");
   if (nb == 0 || nb > 0x7fffffffu)
@@ -714,11 +796,20 @@ int mbx_create_bin_multi(mbx_ctx *ctx, const uint32_t *fk, uint32_t dimsz,
     shift += bits[i];
   }
   if (!st) {
+    bool scatter = false;
+    for (size_t i = 0; i < nsets; ++i) scatter |= sets[i].nbins > 8;
+    // with the LDS images (32 KB per workgroup) and 113 VGPRs four
+    // workgroups fit a CU: all resident, none queued
+    const size_t cap = (size_t)ctx->num_cus * (scatter ? 4 : 16);
     size_t wgs = ceil_div(m.nblocks, 4);
-    if (wgs > (size_t)ctx->num_cus * 16) wgs = (size_t)ctx->num_cus * 16;
+    if (wgs > cap) wgs = cap;
     if (wgs == 0) wgs = 1;
-    hipLaunchKernelGGL(bins_multi_kernel, dim3((unsigned)wgs), dim3(256),
-                       kTransposeLds, ctx->stream, m);
+    if (scatter)
+      hipLaunchKernelGGL(bins_multi_kernel<true>, dim3((unsigned)wgs), dim3(256),
+                         kScatterLds, ctx->stream, m);
+    else
+      hipLaunchKernelGGL(bins_multi_kernel<false>, dim3((unsigned)wgs), dim3(256),
+                         0, ctx->stream, m);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) st = hip_fail(e, "create_bin_multi: launch");
   }

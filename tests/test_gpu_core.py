@@ -430,7 +430,9 @@ def test_create_bin_multi_equals_single(dev, n, dimsz):
     """Several attributes through one foreign-key column in one pass
     (mbx_create_bin_multi) write the same bytes as create_bin_into per
     attribute (whose bytes the oracle pins): unit, equal-width and sorted
-    bins, a 300-bin set (16-bit index), keys past the dimension; and the
+    bins, a 300-bin set (16-bit index), 1,000 bins beside 5 and 25 (a
+    few-bin set through ballots beside several LDS chunks), keys past the
+    dimension; and the
     cases the packed path does not take (bins that are not consecutive,
     indexes wider than 32 bits together) fall back to one pass per set."""
     rng = np.random.default_rng(n)
@@ -450,6 +452,9 @@ def test_create_bin_multi_equals_single(dev, n, dimsz):
                    list(range(1 << 16, (1 << 20) + 1, 1 << 16)))],
         "generic": [(d8, 1, [0, 5, 100], [3, 50, 250]),
                     (d16, 2, list(range(300)), list(range(1, 301)))],
+        "mixed": [(d8, 1, [0, 50, 100, 150, 200], [50, 100, 150, 200, 250]),
+                  (d16, 2, list(range(1000)), list(range(1, 1001))),
+                  (d8, 1, list(range(0, 250, 10)), list(range(10, 260, 10)))],
         "wide": [(d16, 2, list(range(300)), list(range(1, 301))),
                  (d16, 2, list(range(0, 300, 1)), list(range(1, 301))),
                  (d8, 1, list(range(250)), list(range(1, 251)))],
