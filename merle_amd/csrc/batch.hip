@@ -13,8 +13,11 @@
 // flushes the table to the query's int64 output.  Queries come in two
 // classes, each drained by its own persistent kernel: eager conjunctions
 // with a small LDS table (most queries), whose kernel holds only that
-// evaluation form and sink (93 VGPRs: 5 waves per SIMD), and the rest (every
-// form, LDS or HBM sink: 120 VGPRs, 4 waves).  Output zeroing, counter reset
+// evaluation form and sink (held to 80 VGPRs: 6 waves per SIMD), and the
+// rest (every form, LDS or HBM sink: 111 VGPRs, 4 waves).  The waves spend
+// most of their time waiting on loads (L2 hits, dependent rounds), so the
+// two kernels run side by side on two streams, and the conjunction kernel
+// trades words per lane for resident waves.  Output zeroing, counter reset
 // and the two launches are captured once into a hipGraph (fill kernels only:
 // no memset nodes) and replayed per run.
 #include <algorithm>
@@ -39,9 +42,9 @@ template <bool CHK, int EAGER>
 __device__ __forceinline__ void batch_item(const BatchQ &q, uint32_t w_begin,
                                            uint32_t w_end, const GroupSink &sink,
                                            uint32_t *s_list) {
-  // the item's 64-word chunks, interleaved over the workgroup's four waves
-  // (three-valued programs take half as many per pass: twice the registers)
-  constexpr int W = CHK ? kBWPL / 2 : kBWPL;
+  // the item's 64-word chunks, interleaved over the workgroup's four waves,
+  // kBWPL per wave and pass (a 512-word item in one pass)
+  constexpr int W = kBWPL;
   const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   for (uint32_t w0 = w_begin + wv * 64; w0 < w_end; w0 += 4 * 64 * W)
     run_wave_range<CHK, false, W, EAGER>(q.prg, q.op, q.nr_grp, w0, 4 * 64,
@@ -51,7 +54,7 @@ __device__ __forceinline__ void batch_item(const BatchQ &q, uint32_t w_begin,
 // CONJ: the class-0 kernel (eager conjunctions, LDS table); else every form.
 template <bool CONJ>
 __global__ __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(CONJ ? 5 : 1, 8))) void batch_kernel(BatchArgs a) {
+__attribute__((amdgpu_waves_per_eu(CONJ ? 6 : 1, 8))) void batch_kernel(BatchArgs a) {
   __shared__ __align__(16) uint32_t s_list[4][kWaveList];
   __shared__ uint32_t s_item;
   // uniform, so the row-list addresses stay scalar (4 VGPRs fewer)
@@ -130,6 +133,7 @@ struct mbx_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   mbx_tune tune{};
+  mbx::BatchFork fork{};      // side stream of the general kernel
 };
 
 namespace mbx {
@@ -142,15 +146,23 @@ static unsigned batch_grid(mbx_ctx *ctx, const mbx_tune &t,
 }
 
 int batch_launch(mbx_ctx *ctx, hipStream_t s, const BatchQ *d_qs,
-                 const BatchShape &sh, uint32_t *d_counter, const mbx_tune &t) {
+                 const BatchShape &sh, uint32_t *d_counter, const mbx_tune &t,
+                 const BatchFork *fork) {
   const uint32_t nq = sh.nq[0] + sh.nq[1];
   hipLaunchKernelGGL(zero_outputs_kernel, dim3(std::min(nq, 1024u)),
                      dim3(256), 0, s, d_qs, nq);
   MBX_HIP(hipGetLastError());
   int st = launch_fill_u32(s, d_counter, 0, kCounterBytes / 4);
   if (st) return st;
+  // both classes: the general kernel on the side stream, after the zeroing
+  const bool split = fork && sh.nq[0] && sh.nq[1];
+  if (split) {
+    MBX_HIP(hipEventRecord(fork->ev_fork, s));
+    MBX_HIP(hipStreamWaitEvent(fork->side, fork->ev_fork, 0));
+  }
   for (int c = 0; c < 2; ++c) {
     if (sh.nq[c] == 0) continue;
+    const hipStream_t ks = (split && c == 1) ? fork->side : s;
     const unsigned grid = batch_grid(ctx, t, sh.items[c]);
     BatchArgs a;
     a.qs = d_qs + (c ? sh.nq[0] : 0);
@@ -160,18 +172,22 @@ int batch_launch(mbx_ctx *ctx, hipStream_t s, const BatchQ *d_qs,
     a.counter = d_counter + c * kXcdLists * kXcdCounterStride;
     if (c == 0)
       hipLaunchKernelGGL(batch_kernel<true>, dim3(grid), dim3(256),
-                         sh.lds_bytes[0], s, a);
+                         sh.lds_bytes[0], ks, a);
     else
       hipLaunchKernelGGL(batch_kernel<false>, dim3(grid), dim3(256),
-                         sh.lds_bytes[1], s, a);
+                         sh.lds_bytes[1], ks, a);
     MBX_HIP(hipGetLastError());
+  }
+  if (split) {
+    MBX_HIP(hipEventRecord(fork->ev_join, fork->side));
+    MBX_HIP(hipStreamWaitEvent(s, fork->ev_join, 0));
   }
   return MBX_OK;
 }
 
 static int enqueue_batch(mbx_ctx *ctx, mbx_batch *b) {
   return batch_launch(ctx, ctx->stream, b->d_qs, b->shape, b->d_counter,
-                      b->tune);
+                      b->tune, b->fork.side ? &b->fork : nullptr);
 }
 
 int batch_prepare(mbx_ctx *ctx, const mbx_query *qs, uint32_t nq,
@@ -212,6 +228,12 @@ static void destroy_batch(mbx_ctx *ctx, mbx_batch *b) {
   if (!b) return;
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->fork.side) {
+    (void)hipStreamSynchronize(b->fork.side);
+    (void)hipStreamDestroy(b->fork.side);
+  }
+  if (b->fork.ev_fork) (void)hipEventDestroy(b->fork.ev_fork);
+  if (b->fork.ev_join) (void)hipEventDestroy(b->fork.ev_join);
   if (b->d_qs && ctx) ctx->pool->free(b->d_qs);
   delete b;
 }
@@ -253,6 +275,17 @@ static int build_batch(mbx_ctx *ctx, const mbx_query *qs, uint32_t nq,
   }
   b->d_qs = static_cast<BatchQ *>(mem);
   b->d_counter = reinterpret_cast<uint32_t *>(static_cast<char *>(mem) + qbytes);
+  if (shape.nq[0] && shape.nq[1]) {
+    hipError_t fe = hipStreamCreateWithFlags(&b->fork.side, hipStreamNonBlocking);
+    if (fe == hipSuccess)
+      fe = hipEventCreateWithFlags(&b->fork.ev_fork, hipEventDisableTiming);
+    if (fe == hipSuccess)
+      fe = hipEventCreateWithFlags(&b->fork.ev_join, hipEventDisableTiming);
+    if (fe != hipSuccess) {
+      destroy_batch(ctx, b);
+      return hip_fail(fe, "batch: side stream");
+    }
+  }
   hipError_t e = hipMemcpyAsync(b->d_qs, h.data(), sizeof(BatchQ) * nq,
                                 hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);

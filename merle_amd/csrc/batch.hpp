@@ -4,15 +4,16 @@
 
 namespace mbx {
 
-constexpr int kBWPL = 4;          // 64-word chunks per wave per pass
+constexpr int kBWPL = 2;          // 64-word chunks per wave per pass
 constexpr uint32_t kItemQuantum = 512;   // item sizes are multiples of this
 constexpr uint32_t kXcdLists = 8;        // one work list per XCD
 constexpr uint32_t kXcdCounterStride = 32;   // uint32: counters 128 B apart
 // Counters of the two launches of a batch (one set per query class).
 constexpr size_t kCounterBytes = 2 * kXcdLists * kXcdCounterStride * 4;
 // Largest LDS group table of the conjunction kernel: with its 8 KB of row
-// lists, five 4-wave workgroups then fit a CU's 160 KB of LDS, as its 93
-// VGPRs allow (5 waves per SIMD; the general kernel's 120 allow 4).
+// lists, five 4-wave workgroups fit a CU's 160 KB of LDS at the largest
+// table, six at tables up to 2,300 groups, as its 80 VGPRs allow (6 waves
+// per SIMD; the general kernel's 111 allow 4).
 constexpr uint32_t kConjLdsGroups = 3000;
 
 struct BatchQ {
@@ -38,10 +39,17 @@ struct BatchShape {
 // h[0..nq) (class 0 first) and *shape.  Host only, no device calls.
 int batch_prepare(mbx_ctx *ctx, const mbx_query *qs, uint32_t nq,
                   const mbx_tune &t, BatchQ *h, BatchShape *shape);
-// Enqueues output zeroing, counter reset and the persistent kernels on s.
-// d_counter: kCounterBytes of device memory.
+// Side stream and events for running the two classes' kernels side by side.
+struct BatchFork {
+  hipStream_t side;
+  hipEvent_t ev_fork, ev_join;
+};
+
+// Enqueues output zeroing, counter reset and the persistent kernels on s
+// (with fork and both classes present, the general kernel on fork->side,
+// joined back into s).  d_counter: kCounterBytes of device memory.
 int batch_launch(mbx_ctx *ctx, hipStream_t s, const BatchQ *d_qs,
                  const BatchShape &shape, uint32_t *d_counter,
-                 const mbx_tune &t);
+                 const mbx_tune &t, const BatchFork *fork = nullptr);
 
 }  // namespace mbx
