@@ -450,9 +450,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="run only a plain device copy of this many bytes")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="run only the config-3 batched section (profiling)")
     args = ap.parse_args()
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration, max(1, args.steps))
+    if args.batched_only:
+        from merle_amd.host import Device
+        dev = Device(0)
+        print(json.dumps({"batched": run_batched(dev)}), flush=True)
+        dev.close()
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the "
               "path shards by row range with one int64 group reduction at the "
