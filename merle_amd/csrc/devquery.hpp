@@ -709,6 +709,12 @@ struct Spread {
   uint32_t gpad;             // table stride: nr_grp rounded up to 16
 };
 
+// A thread's groups are i = tid + k * blockDim.x; kFlushU of them go out
+// together, every atomic issued before any result is used, so a thread's
+// adds cost one round trip per kFlushU groups (a result consumed inside the
+// lane-divergent `if (v)` would be waited for there, one round trip each).
+constexpr uint32_t kFlushU = 4;
+
 __device__ __forceinline__ void flush_groups(const Spread &sp,
                                              unsigned long long *out,
                                              uint32_t nr_grp, uint32_t member,
@@ -717,13 +723,22 @@ __device__ __forceinline__ void flush_groups(const Spread &sp,
   const uint32_t R = nmembers < kSpreadRep ? nmembers : kSpreadRep;
   const uint32_t r = member % R;
   unsigned long long *rep = sp.rep + (size_t)r * sp.gpad;
+  const uint32_t nt = blockDim.x;
   __syncthreads();
   unsigned long long seen = 0;
-  for (uint32_t i = threadIdx.x; i < nr_grp; i += blockDim.x) {
-    const unsigned long long v = s_grp_dyn[i];
-    if (v)
-      seen |= __hip_atomic_fetch_add(rep + i, v, __ATOMIC_RELAXED,
-                                     __HIP_MEMORY_SCOPE_AGENT);
+  for (uint32_t i0 = threadIdx.x; i0 < nr_grp; i0 += kFlushU * nt) {
+    unsigned long long got[kFlushU];
+#pragma unroll
+    for (uint32_t u = 0; u < kFlushU; ++u) {
+      const uint32_t i = i0 + u * nt;
+      const unsigned long long v = i < nr_grp ? s_grp_dyn[i] : 0ull;
+      got[u] = 0;
+      if (v)
+        got[u] = __hip_atomic_fetch_add(rep + i, v, __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_AGENT);
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kFlushU; ++u) seen |= got[u];
   }
   asm volatile("" ::"v"(seen));   // every add has returned
   __syncthreads();
@@ -736,10 +751,19 @@ __device__ __forceinline__ void flush_groups(const Spread &sp,
   }
   __syncthreads();
   if (!s_last) return;
-  for (uint32_t i = threadIdx.x; i < nr_grp; i += blockDim.x) {
-    const unsigned long long v = __hip_atomic_exchange(
-        rep + i, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (v) atomicAdd(out + i, v);
+  for (uint32_t i0 = threadIdx.x; i0 < nr_grp; i0 += kFlushU * nt) {
+    unsigned long long v[kFlushU];
+#pragma unroll
+    for (uint32_t u = 0; u < kFlushU; ++u) {
+      const uint32_t i = i0 + u * nt;
+      v[u] = 0;
+      if (i < nr_grp)
+        v[u] = __hip_atomic_exchange(rep + i, 0ull, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT);
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kFlushU; ++u)
+      if (v[u]) atomicAdd(out + i0 + u * nt, v[u]);
   }
   if (threadIdx.x == 0)
     (void)__hip_atomic_exchange(sp.done + (size_t)r * kSpreadDoneStride, 0u,
