@@ -41,7 +41,9 @@ enum mbx_status {
   MBX_ERR_ALIGN = 4,       /* a bitmap pointer is not 8-byte aligned */
   MBX_ERR_CAPACITY = 5,    /* a fixed capacity was exceeded */
   MBX_ERR_NODEVICE = 6,    /* no usable GPU */
-  MBX_ERR_ADMISSION = 7    /* serving: request refused by memory admission */
+  MBX_ERR_ADMISSION = 7,   /* serving: request refused by memory admission */
+  MBX_ERR_UNSUPPORTED = 8  /* MBX_PLAN_DEDICATED: no kernel built for the
+                              program's shape */
 };
 
 typedef struct mbx_ctx mbx_ctx;   /* device, stream, memory pool */
@@ -218,10 +220,18 @@ enum mbx_flags {
                                 walks the set bits of its own words and runs
                                 the row op on them (abla::method1,
                                 primitive_abla.cuh:62-108) */
-  MBX_PLAN_BASEFUSE = 128    /* fused, ablation: method 1 with the program's
+  MBX_PLAN_BASEFUSE = 128,   /* fused, ablation: method 1 with the program's
                                 virtual registers in device memory
                                 (abla::base_fuse, primitive_abla.cuh:15-59);
                                 waits for its kernel (pool scratch) */
+  MBX_PLAN_DEDICATED = 256   /* fused, ablation: a kernel built for the
+                                program's shape, the flat operand list fixed
+                                at compile time (abla::hardcoded and the
+                                dedicated c::s** kernels,
+                                primitive_abla.cuh:188-260,
+                                ssb/inst/1c.cu:5-22); MBX_ERR_UNSUPPORTED
+                                when the library holds none for the shape
+                                (see mbx_prog_shape) */
 };
 
 /* Program + group-by in one call.  d_out[nr_grp] int64 on the device.
@@ -241,6 +251,16 @@ int mbx_prog_grpby(mbx_ctx *ctx, const mbx_vprg *prg, const mbx_rowop *op,
 int mbx_prog_grpby_noalloc(mbx_ctx *ctx, const mbx_vprg *prg,
                            const mbx_rowop *op, uint32_t nr_grp, int64_t *d_out,
                            uint32_t flags);
+
+/* Shape of a program for MBX_PLAN_DEDICATED: a conjunction of clauses of
+ * one column each (at most 4), clause k in bits 6k..6k+5 as
+ * nm | left << 3 | right << 4 | negated << 5 (nm = middle bins, left /
+ * right = edge bins present), clauses in the order the fused kernel reads
+ * them; *code = 0 for any other program.  *built = 1 when the library holds
+ * a dedicated kernel for the code.  Host only.  Replaces the choice of a
+ * hand-written c::s** function per query and layout (ssb/fixedlayout.cu:
+ * 15-32, where a missing one reads as 0). */
+int mbx_prog_shape(const mbx_vprg *prg, uint32_t *code, int *built);
 
 /* The materialise stages, separately callable (primitive_abla.cuh:262-394).
  * mbx_prog_eval writes ceil(N/64)*2 words each of possible/uncertain

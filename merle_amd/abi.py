@@ -13,7 +13,7 @@ MAXBIN_PERCOL, MAXCOLS, MAXNINSTR, MAXKEYS, MAXBOUNDS = 4, 4, 6, 3, 8
 
 # status codes
 OK, ERR_INVALID, ERR_OOM, ERR_HIP, ERR_ALIGN, ERR_CAPACITY, ERR_NODEVICE, \
-    ERR_ADMISSION = range(8)
+    ERR_ADMISSION, ERR_UNSUPPORTED = range(9)
 # opcodes (vprg::ops, primitive.cuh:169-173)
 END, AND, OR, NOT, ANDM, ORM = range(6)
 KEY_IDENT, KEY_DIV, KEY_BUCKET = range(3)
@@ -24,6 +24,7 @@ ACCUMULATE = 16
 PLAN_EAGER = 32
 PLAN_DIRECT = 64
 PLAN_BASEFUSE = 128
+PLAN_DEDICATED = 256
 
 
 class ColRef(C.Structure):
@@ -155,6 +156,7 @@ PROTOTYPES = {
                                       _u32]),
     "mbx_prog_eval": (_i32, [_vp, _P(Vprg), _vp, _vp]),
     "mbx_prog_eval_ex": (_i32, [_vp, _P(Vprg), _vp, _vp, _u32]),
+    "mbx_prog_shape": (_i32, [_P(Vprg), _P(_u32), _P(C.c_int)]),
     "mbx_compact": (_i32, [_vp, _vp, _vp, _u32, _vp, _vp]),
     "mbx_rows_grpby": (_i32, [_vp, _vp, _vp, _P(Vprg), _P(RowOp), _u32, _vp,
                               _u32]),

@@ -24,12 +24,14 @@
 // only, as DOWORK), Ours = fused under the library's static plan, PrgStg1 /
 // Stg2 / Stg3 = the three materialise stages, Basefus = the base_fuse
 // ablation (method 1 with the program's registers in device memory),
-// NfuStg1 = the first stage with its registers in device memory; the
-// dedicated per-query kernels (Dedicat) are not built and read 0.  Every
+// NfuStg1 = the first stage with its registers in device memory, Dedicat
+// = the kernel built for the program's shape (MBX_PLAN_DEDICATED; fixed
+// layouts only, as in the reference, and 0 where the library holds no
+// kernel for the shape).  Every
 // plan's int64 sums must equal the scan's, or the driver exits with status
 // 1.  With a
 // sums_file the driver also writes every SSB row's sums there, one line per
-// (flight, bin type, plan): name, bin type, plan (join / ours / stg), then
+// (flight, bin type, plan): name, bin type, plan (join / ours / stg / ded), then
 // the sums, tab-separated.
 #include <chrono>
 #include <cmath>
@@ -742,6 +744,32 @@ int main(int argc, char **argv) {
       write_sums(f.name, kPlan[level], "ours", h_got.data(), g);
       float basefus = 0, nfu1 = 0;
       if (!ablations(ap, op, g, plan, basefus, nfu1)) return 2;
+      // Dedicat: the kernel built for the planned program's shape (the
+      // reference's hand-written c::s** functions, ssb/fixedlayout.cu:
+      // 15-32); 0 where the library holds none, as DEDICATE reads 0 for a
+      // missing function
+      float ded = 0;
+      {
+        mbx_vprg dp;
+        if (mbx_plan_apply(&ap, &plan, &dp) != MBX_OK) {
+          std::fprintf(stderr, "plan: %s\n", mbx_last_error());
+          return 2;
+        }
+        const uint32_t dfl = MBX_PLAN_FUSED | MBX_PLAN_DEDICATED;
+        const int st = mbx_prog_grpby(ctx, &dp, &op, g, out, dfl);
+        if (st == MBX_OK) {
+          ded = timed([&] { return mbx_prog_grpby(ctx, &dp, &op, g, out, dfl); });
+          mbx_memcpy_d2h(ctx, h_got.data(), out, 8 * g);
+          bad += std::memcmp(h_want.data(), h_got.data(), 8 * g) != 0;
+          write_sums(f.name, kPlan[level], "ded", h_got.data(), g);
+        } else if (st == MBX_ERR_UNSUPPORTED) {
+          std::fprintf(stderr, "%s %s: %s\n", f.name, kPlan[level],
+                       mbx_last_error());
+        } else {
+          std::fprintf(stderr, "dedicated: %s\n", mbx_last_error());
+          return 2;
+        }
+      }
       const float s1 = timed([&] {
         return mbx_prog_eval(ctx, &ap, poss, p.is_nochk() ? nullptr : unc);
       });
@@ -759,8 +787,8 @@ int main(int argc, char **argv) {
         std::printf("\n%s\t%s\t%.4f", f.name, kPlan[level], join);
       else
         std::printf("\n%s\t%s\t0", f.name, kPlan[level]);
-      std::printf("\t%.4f\t%.4f\t%.4f\t%.4f\t%.4f\t%.4f\t0", ours, basefus,
-                  nfu1, s1, s2, s3);
+      std::printf("\t%.4f\t%.4f\t%.4f\t%.4f\t%.4f\t%.4f\t%.4f", ours, basefus,
+                  nfu1, s1, s2, s3, ded);
     }
   }
   // the controlled layouts (ssb/controlled-demo.cu:7-129): each flight's

@@ -269,6 +269,29 @@ int make_dev_prg(const mbx_vprg *p, DevPrg *out) {
   return MBX_OK;
 }
 
+// Clause k of a conjunction, in flat-list order, as nm | left << 3 |
+// right << 4 | negated << 5 at bits 6k..6k+5 (the dedicated kernels'
+// template argument); 0 unless every clause holds exactly one column.
+uint32_t conj_shape_code(const DevPrg &P) {
+  if (!P.conj || P.nfl == 0) return 0;
+  uint32_t code = 0, k = 0, e = 0;
+  while (e < P.nfl) {
+    if (k == MBX_MAXCOLS) return 0;
+    const uint32_t c = P.flcol[e];
+    const DevCol &d = P.cols[c];
+    const uint32_t n = d.nm + (d.l ? 1u : 0u) + (d.r ? 1u : 0u);
+    if (n == 0 || e + n > P.nfl) return 0;
+    for (uint32_t j = e; j < e + n; ++j)
+      if (P.flcol[j] != c || (P.flend[j] != 0) != (j == e + n - 1)) return 0;
+    const uint32_t cl = d.nm | (d.l ? 8u : 0u) | (d.r ? 16u : 0u) |
+                        (P.flneg[e + n - 1] ? 32u : 0u);
+    code |= cl << (6 * k);
+    ++k;
+    e += n;
+  }
+  return code;
+}
+
 void key_div_magic(uint32_t d, uint64_t *m, uint8_t *sh1, uint8_t *sh2) {
   uint32_t l = 0;
   while ((1ull << l) < d) ++l;

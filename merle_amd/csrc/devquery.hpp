@@ -204,6 +204,126 @@ __device__ __forceinline__ void apply_ins(uint32_t op, uint32_t d, uint32_t s,
   }
 }
 
+// Where the conjunction form reads the shape of its flat operand list:
+// from the program descriptor (RuntimeShape, every product launch) or from
+// compile-time constants (the dedicated kernels, whose shape is a template
+// argument).
+struct RuntimeShape {
+  static __device__ __forceinline__ uint32_t nfl(const DevPrg &P) { return P.nfl; }
+  static __device__ __forceinline__ bool edge(const DevPrg &P, uint32_t e) {
+    return P.fledge[e] != 0;
+  }
+  static __device__ __forceinline__ bool end(const DevPrg &P, uint32_t e) {
+    return P.flend[e] != 0;
+  }
+  static __device__ __forceinline__ bool neg(const DevPrg &P, uint32_t e) {
+    return P.flneg[e] != 0;
+  }
+};
+
+// EAGER 3: a conjunction of clauses (P.conj).  in[k] / keep[k]: word k is
+// inside the range / its surviving row bits.
+template <bool CHK, bool ACCT, int WPL, class F>
+__device__ __forceinline__ void eval_conj(const DevPrg &P, uint32_t w0,
+                                          uint32_t ws, uint32_t wend,
+                                          const bool (&in)[WPL],
+                                          const uint64_t (&keep)[WPL],
+                                          uint64_t (&po)[WPL],
+                                          uint64_t (&uo)[WPL],
+                                          const Acct *acct) {
+  (void)wend;
+  // EAGER 3: a conjunction of clauses (P.conj): r0 = AND over clauses of
+  // the OR of their bins (a clause is an OR of columns, or NOT of one
+  // column; the flat list holds it contiguously, flend on its last entry),
+  // complemented where flneg.  The flat list is walked in groups of
+  // kConjGroup, every load of a group in flight at once; per word the work
+  // is one OR per bitmap and one AND per clause -- no program
+  // interpretation.  Three-valued (CHK), with c = OR of a clause's middle
+  // bins and u = OR of its edge bins, the instructions reduce to
+  //   p = AND(plain clauses: c | u) & AND(negated clauses: ~c)
+  //   u = OR(clauses: u & ~c) & p
+  // (ORM into an empty register gives p = c | u, u = u; ORM / OR onto a
+  // clause give p |= p_s, u = (u | u_s) & ~(c | c_s) for disjoint bins;
+  // ANDM and AND give p &= p_s, u = (u | u_s) & p; NOT gives
+  // p = ~p_s | u_s = ~c, u = u_s).
+  uint32_t wa[WPL];
+  bool any[WPL];
+  uint64_t acc[WPL], cur[WPL], accu[WPL], curu[WPL];
+#pragma unroll
+  for (int k = 0; k < WPL; ++k) {
+    const uint32_t w = w0 + ws * k;
+    const uint64_t m = __ballot(in[k]);
+    any[k] = m != 0;
+    const uint32_t wf = __shfl(w, any[k] ? (int)__builtin_ctzll(m) : 0, 64);
+    wa[k] = in[k] ? w : wf;
+    acc[k] = keep[k];
+    cur[k] = 0;
+    accu[k] = curu[k] = 0;
+  }
+  for (uint32_t g = 0; g < F::nfl(P); g += kConjGroup) {
+    uint64_t v[kConjGroup][WPL];
+#pragma unroll
+    for (int j = 0; j < kConjGroup; ++j) {
+#pragma unroll
+      for (int k = 0; k < WPL; ++k) v[j][k] = 0;
+      if (g + j < F::nfl(P)) {
+        const uint64_t *b = P.fl[g + j];
+#pragma unroll
+        for (int k = 0; k < WPL; ++k)
+          if (any[k]) {
+            if (ACCT && in[k]) mark<true>(acct, b + wa[k]);
+            v[j][k] = ld_stream(b + wa[k]);
+          }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kConjGroup; ++j) {
+      if (g + j >= F::nfl(P)) break;
+      if (CHK && F::edge(P, g + j)) {
+#pragma unroll
+        for (int k = 0; k < WPL; ++k) curu[k] |= v[j][k];
+      } else {
+#pragma unroll
+        for (int k = 0; k < WPL; ++k) cur[k] |= v[j][k];
+      }
+      if (F::end(P, g + j)) {
+        // plain: c | u; negated: ~c = (c | (u & 0)) ^ ~0
+        const uint64_t flip = F::neg(P, g + j) ? ~0ull : 0ull;
+#pragma unroll
+        for (int k = 0; k < WPL; ++k) {
+          const uint64_t lit = CHK ? (cur[k] | (curu[k] & ~flip)) : cur[k];
+          acc[k] &= lit ^ flip;
+          if (CHK) {
+            accu[k] |= curu[k] & ~cur[k];
+            curu[k] = 0;
+          }
+          cur[k] = 0;
+        }
+      }
+    }
+    if (g + kConjGroup < F::nfl(P)) {
+      // Early-out between groups: a word whose running AND is empty in
+      // every lane skips the later groups' loads (uniform branch), and a
+      // lane whose word is empty loads the first live lane's address, so
+      // a 128-byte line with no live word is not fetched.
+#pragma unroll
+      for (int k = 0; k < WPL; ++k) {
+        const bool live = acc[k] != 0;
+        const uint64_t m = __ballot(live);
+        any[k] = m != 0;
+        const uint32_t w = w0 + ws * k;
+        const uint32_t wf = __shfl(w, any[k] ? (int)__builtin_ctzll(m) : 0, 64);
+        wa[k] = live ? w : wf;
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < WPL; ++k) {
+    po[k] = acc[k];
+    uo[k] = CHK ? (accu[k] & acc[k]) : 0;
+  }
+}
+
 // Evaluates the program on this lane's words w0 + k * ws (k < WPL, word <
 // wend).  EAGER = false keeps the ANDM early-out of primitive.cuh:215/292: a
 // word whose destination is already empty loads nothing, so a 128-byte line
@@ -220,7 +340,7 @@ __device__ __forceinline__ void apply_ins(uint32_t op, uint32_t d, uint32_t s,
 // whose flat operand list has at most kDirectOps bitmaps.
 constexpr uint32_t kDirectOps = 5;
 
-template <bool CHK, bool ACCT, int EAGER, int WPL>
+template <bool CHK, bool ACCT, int EAGER, int WPL, class F = RuntimeShape>
 __device__ __forceinline__ void eval_words(const DevPrg &P, uint32_t w0,
                                            uint32_t ws, uint32_t wend,
                                            uint64_t (&po)[WPL],
@@ -239,96 +359,7 @@ __device__ __forceinline__ void eval_words(const DevPrg &P, uint32_t w0,
       keep[k] = (1ull << (P.nrows & 63u)) - 1ull;
   }
   if (EAGER == 3) {
-    // EAGER 3: a conjunction of clauses (P.conj): r0 = AND over clauses of
-    // the OR of their bins (a clause is an OR of columns, or NOT of one
-    // column; the flat list holds it contiguously, flend on its last entry),
-    // complemented where flneg.  The flat list is walked in groups of
-    // kConjGroup, every load of a group in flight at once; per word the work
-    // is one OR per bitmap and one AND per clause -- no program
-    // interpretation.  Three-valued (CHK), with c = OR of a clause's middle
-    // bins and u = OR of its edge bins, the instructions reduce to
-    //   p = AND(plain clauses: c | u) & AND(negated clauses: ~c)
-    //   u = OR(clauses: u & ~c) & p
-    // (ORM into an empty register gives p = c | u, u = u; ORM / OR onto a
-    // clause give p |= p_s, u = (u | u_s) & ~(c | c_s) for disjoint bins;
-    // ANDM and AND give p &= p_s, u = (u | u_s) & p; NOT gives
-    // p = ~p_s | u_s = ~c, u = u_s).
-    uint32_t wa[WPL];
-    bool any[WPL];
-    uint64_t acc[WPL], cur[WPL], accu[WPL], curu[WPL];
-#pragma unroll
-    for (int k = 0; k < WPL; ++k) {
-      const uint32_t w = w0 + ws * k;
-      const uint64_t m = __ballot(in[k]);
-      any[k] = m != 0;
-      const uint32_t wf = __shfl(w, any[k] ? (int)__builtin_ctzll(m) : 0, 64);
-      wa[k] = in[k] ? w : wf;
-      acc[k] = keep[k];
-      cur[k] = 0;
-      accu[k] = curu[k] = 0;
-    }
-    for (uint32_t g = 0; g < P.nfl; g += kConjGroup) {
-      uint64_t v[kConjGroup][WPL];
-#pragma unroll
-      for (int j = 0; j < kConjGroup; ++j) {
-#pragma unroll
-        for (int k = 0; k < WPL; ++k) v[j][k] = 0;
-        if (g + j < P.nfl) {
-          const uint64_t *b = P.fl[g + j];
-#pragma unroll
-          for (int k = 0; k < WPL; ++k)
-            if (any[k]) {
-              if (ACCT && in[k]) mark<true>(acct, b + wa[k]);
-              v[j][k] = ld_stream(b + wa[k]);
-            }
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < kConjGroup; ++j) {
-        if (g + j >= P.nfl) break;
-        if (CHK && P.fledge[g + j]) {
-#pragma unroll
-          for (int k = 0; k < WPL; ++k) curu[k] |= v[j][k];
-        } else {
-#pragma unroll
-          for (int k = 0; k < WPL; ++k) cur[k] |= v[j][k];
-        }
-        if (P.flend[g + j]) {
-          // plain: c | u; negated: ~c = (c | (u & 0)) ^ ~0
-          const uint64_t flip = P.flneg[g + j] ? ~0ull : 0ull;
-#pragma unroll
-          for (int k = 0; k < WPL; ++k) {
-            const uint64_t lit = CHK ? (cur[k] | (curu[k] & ~flip)) : cur[k];
-            acc[k] &= lit ^ flip;
-            if (CHK) {
-              accu[k] |= curu[k] & ~cur[k];
-              curu[k] = 0;
-            }
-            cur[k] = 0;
-          }
-        }
-      }
-      if (g + kConjGroup < P.nfl) {
-        // Early-out between groups: a word whose running AND is empty in
-        // every lane skips the later groups' loads (uniform branch), and a
-        // lane whose word is empty loads the first live lane's address, so
-        // a 128-byte line with no live word is not fetched.
-#pragma unroll
-        for (int k = 0; k < WPL; ++k) {
-          const bool live = acc[k] != 0;
-          const uint64_t m = __ballot(live);
-          any[k] = m != 0;
-          const uint32_t w = w0 + ws * k;
-          const uint32_t wf = __shfl(w, any[k] ? (int)__builtin_ctzll(m) : 0, 64);
-          wa[k] = live ? w : wf;
-        }
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < WPL; ++k) {
-      po[k] = acc[k];
-      uo[k] = CHK ? (accu[k] & acc[k]) : 0;
-    }
+    eval_conj<CHK, ACCT, WPL, F>(P, w0, ws, wend, in, keep, po, uo, acct);
     return;
   }
   if (EAGER == 4) {
@@ -941,7 +972,7 @@ constexpr uint32_t kDenseRowsPerWord = 16;
 // SCAN: the words come from the rows' exact predicates (scan_words), not
 // from bitmaps.
 template <bool CHK, bool ACCT, int WPL, int EAGER, int E = 2, bool DIRECT = false,
-          bool SCAN = false>
+          bool SCAN = false, class F = RuntimeShape>
 __device__ __forceinline__ void run_wave_range(const DevPrg &P, const DevOp &O,
                                                uint32_t nr_grp, uint32_t w0,
                                                uint32_t ws, uint32_t wend,
@@ -956,7 +987,7 @@ __device__ __forceinline__ void run_wave_range(const DevPrg &P, const DevOp &O,
 #pragma unroll
     for (int k = 0; k < WPL; ++k) u[k] = 0;
   } else {
-    eval_words<CHK, ACCT, EAGER, WPL>(P, w0 + lane, ws, wend, p, u, acct);
+    eval_words<CHK, ACCT, EAGER, WPL, F>(P, w0 + lane, ws, wend, p, u, acct);
   }
   if (DIRECT) {
     // The ablation without compaction (abla::method1, primitive_abla.cuh:

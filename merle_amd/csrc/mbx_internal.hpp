@@ -166,6 +166,10 @@ struct DevOp {
 
 // Validation + conversion; returns MBX_OK or an error (with message).
 int make_dev_prg(const mbx_vprg *p, DevPrg *out);
+// Compile-time shape code of a conjunction of single-column clauses (the
+// dedicated kernels' template argument; mbx_prog_shape documents the
+// encoding), 0 for any other program.
+uint32_t conj_shape_code(const DevPrg &P);
 // Division of any 64-bit x by an invariant d >= 1 as
 //   t = mulhi64(m, x);  q = (t + ((x - t) >> sh1)) >> sh2
 // (Granlund and Montgomery, "Division by Invariant Integers using

@@ -157,6 +157,7 @@ const char *mbx_strerror(int s) {
   case MBX_ERR_CAPACITY: return "capacity exceeded";
   case MBX_ERR_NODEVICE: return "no usable GPU";
   case MBX_ERR_ADMISSION: return "refused by memory admission";
+  case MBX_ERR_UNSUPPORTED: return "no dedicated kernel for the program's shape";
   default: return "unknown status";
   }
 }

@@ -677,6 +677,13 @@ int mbx_prog_grpby(mbx_ctx *ctx, const mbx_vprg *prg, const mbx_rowop *op,
     if (!st && e != hipSuccess) st = hip_fail(e, "base_fuse ablation");
     return st;
   }
+  if (flags & MBX_PLAN_DEDICATED) {
+    if ((flags & MBX_PLAN_MASK) != MBX_PLAN_FUSED ||
+        (flags & (MBX_PLAN_DIRECT | MBX_PLAN_BASEFUSE)))
+      return fail(MBX_ERR_INVALID,
+                  "MBX_PLAN_DEDICATED is a form of the fused plan alone");
+    return launch_dedicated(ctx, q);
+  }
   switch (flags & MBX_PLAN_MASK) {
   case MBX_PLAN_FUSED: return launch_fused(ctx, q, nullptr);
   case MBX_PLAN_MATERIALISE: return run_materialise(ctx, q);
@@ -711,10 +718,15 @@ int mbx_prog_grpby_noalloc(mbx_ctx *ctx, const mbx_vprg *prg,
   if (flags & MBX_PLAN_BASEFUSE)
     return fail(MBX_ERR_INVALID,
                 "prog_grpby_noalloc: the base_fuse ablation needs scratch");
+  if ((flags & MBX_PLAN_DEDICATED) &&
+      (plan != MBX_PLAN_FUSED || (flags & MBX_PLAN_DIRECT)))
+    return fail(MBX_ERR_INVALID,
+                "MBX_PLAN_DEDICATED is a form of the fused plan alone");
   if (!(flags & MBX_ACCUMULATE)) {
     st = launch_fill_u64(ctx->stream, d_out, 0, nr_grp);
     if (st) return st;
   }
+  if (flags & MBX_PLAN_DEDICATED) return launch_dedicated(ctx, q);
   if (plan == MBX_PLAN_FUSED) return launch_fused(ctx, q, nullptr);
   if (plan == MBX_PLAN_SCAN) return launch_scan(ctx, q);
   const size_t nw = q.prg.nwords;

@@ -46,6 +46,9 @@ size_t noalloc_max_rows(size_t scratch_bytes);
 int prepare(mbx_ctx *ctx, const mbx_vprg *prg, const mbx_rowop *op,
             uint32_t nr_grp, int64_t *d_out, QueryPlan *q);
 int launch_fused(mbx_ctx *ctx, const QueryPlan &q, const Acct *acct);
+// The fused plan through the kernel built for the program's shape
+// (MBX_PLAN_DEDICATED); MBX_ERR_UNSUPPORTED when there is none.
+int launch_dedicated(mbx_ctx *ctx, const QueryPlan &q);
 int launch_scan(mbx_ctx *ctx, const QueryPlan &q);
 int launch_stage1(mbx_ctx *ctx, const DevPrg &P, uint64_t *po, uint64_t *uo);
 int launch_stage2(mbx_ctx *ctx, const uint64_t *po, const uint64_t *uo,

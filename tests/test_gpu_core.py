@@ -283,6 +283,15 @@ def test_errors(dev):
     prg.factsz = 1 << 31
     assert dev.lib.mbx_prog_grpby(dev.ctx, C.byref(prg), C.byref(op), 1,
                                   out.ptr, 0) == abi.ERR_CAPACITY
+    # the dedicated plan: a shape without a kernel (one clause of one
+    # middle bin) is refused before any launch, as is its use with any plan
+    # other than the fused one
+    prg.factsz = n
+    assert dev.lib.mbx_prog_grpby(dev.ctx, C.byref(prg), C.byref(op), 1,
+                                  out.ptr, abi.PLAN_DEDICATED) == abi.ERR_UNSUPPORTED
+    assert dev.lib.mbx_prog_grpby(
+        dev.ctx, C.byref(prg), C.byref(op), 1, out.ptr,
+        abi.PLAN_MATERIALISE | abi.PLAN_DEDICATED) == abi.ERR_INVALID
     v.free()
     out.free()
 

@@ -293,6 +293,61 @@ def _prg(ncols, instrs):
     return p
 
 
+def test_prog_shape_codes(lib):
+    """mbx_prog_shape: clause k of a conjunction of single-column clauses at
+    bits 6k..6k+5 as middle bins | left << 3 | right << 4 | negated << 5,
+    in the order the program reads them; 0 for other programs."""
+    def prg_of(cols, ins):
+        p = abi.Vprg()
+        p.factsz = 1000
+        nxt = [0x1000]
+
+        def ptr():
+            nxt[0] += 0x1000
+            return nxt[0]
+        for c, (nm, l, r) in enumerate(cols):
+            for k in range(nm):
+                p.cols[c].middle[k] = ptr()
+            if l:
+                p.cols[c].leftmost = ptr()
+            if r:
+                p.cols[c].rightmost = ptr()
+        for i, (o, d, s1) in enumerate(ins):
+            p.instrs[i].opcode, p.instrs[i].dstreg, p.instrs[i].src1 = o, d, s1
+        return p
+
+    def shape(p):
+        code, built = C.c_uint32(), C.c_int()
+        assert lib.mbx_prog_shape(C.byref(p), C.byref(code), C.byref(built)) == abi.OK
+        return code.value, built.value
+
+    ORM, ANDM = abi.ORM, abi.ANDM
+    # a AND b AND c, clause order = instruction order
+    p = prg_of([(1, 0, 0), (0, 1, 1), (2, 0, 1)],
+               [(ORM, 0, 0), (ANDM, 0, 1), (ANDM, 0, 2)])
+    assert shape(p) == (1 | (24 << 6) | (18 << 12), 1)   # SSB11 Sparse's shape
+    p = prg_of([(1, 0, 0), (0, 1, 1), (2, 0, 1)],
+               [(ORM, 0, 2), (ANDM, 0, 0), (ANDM, 0, 1)])
+    assert shape(p)[0] == 18 | (1 << 6) | (24 << 12)
+    # a AND b AND NOT c: the negated clause carries bit 5
+    p = prg_of([(1, 0, 0), (4, 0, 0), (2, 0, 0)],
+               [(ORM, 0, 0), (ANDM, 0, 1), (ORM, 1, 2), (abi.NOT, 1, 1),
+                (abi.AND, 0, 1)])
+    assert shape(p)[0] == 1 | (4 << 6) | ((2 | 32) << 12)
+    # four clauses of four middle bins: a conjunction, no kernel built
+    p = prg_of([(4, 0, 0)] * 4,
+               [(ORM, 0, 0), (ANDM, 0, 1), (ANDM, 0, 2), (ANDM, 0, 3)])
+    assert shape(p) == (4 | (4 << 6) | (4 << 12) | (4 << 18), 0)
+    # (a OR b) AND c: a clause of two columns -- no shape code
+    p = prg_of([(1, 0, 0), (1, 0, 0), (1, 0, 0)],
+               [(ORM, 0, 0), (ORM, 0, 1), (ANDM, 0, 2)])
+    assert shape(p) == (0, 0)
+    # not a conjunction (a OR (b AND c)) -- no shape code
+    p = prg_of([(1, 0, 0), (1, 0, 0), (1, 0, 0)],
+               [(ORM, 1, 1), (ANDM, 1, 2), (ORM, 0, 0), (abi.OR, 0, 1)])
+    assert shape(p) == (0, 0)
+
+
 def test_plan_features_and_candidates(lib):
     """The cost-model features price a program by what it selects: the
     program run over the column densities as independent probabilities

@@ -12,8 +12,8 @@ library's loader and the 13 flights (ssb/cpuref.c:244-302) are checked:
   are built with create_bin (FK projection for customer city and part
   mfgr); each flight runs its Sparse, Medium and Dense programs of
   ssb/fixedlayout.cu (col::from_bin / from_bin2, edge bins -> candidate
-  checks) as fused, eager, materialise and scan plans; every result equals
-  the oracle's.
+  checks) as fused, eager, materialise and scan plans and through its
+  dedicated kernel; every result equals the oracle's.
 """
 import ctypes as C
 import os
@@ -138,6 +138,41 @@ def test_fixed_plans_cover_every_range(lib):
                 assert all(b[j + 1] == b[k] for j, k in zip(covered, covered[1:]))
 
 
+def test_fixed_programs_have_dedicated_kernels(lib):
+    """Every SSB fixed-layout program, as written, is a conjunction of
+    single-column clauses whose shape has a dedicated kernel
+    (mbx_prog_shape, host only); the shape code spells out the program's
+    bins: SSB11 Sparse = date (1 middle bin) AND discount (left and right
+    edge bins) AND quantity (2 middle bins and a right edge bin)."""
+    fake = {}
+
+    def ptr(key):
+        return fake.setdefault(key, 0x100000 * (len(fake) + 1))
+
+    for f in ssbq.FLIGHTS:
+        for fp in ssbq.fixed_plans(f):
+            spec = ssbq.fixed_spec(f, fp)
+            cols = []
+            for cn, lo, hi, lvl in fp.cols:
+                d, (mid, l, r) = ssbq.fixed_bins(cn, lo, hi, lvl)
+                cols.append(S.BitmapCol([ptr((cn, d, j)) for j in mid],
+                                        None if l is None else ptr((cn, d, l)),
+                                        None if r is None else ptr((cn, d, r))))
+            prg = S.to_vprg(cols, 6_000_000, spec.instrs)
+            code, built = C.c_uint32(), C.c_int()
+            check(lib.mbx_prog_shape(C.byref(prg), C.byref(code), C.byref(built)))
+            assert code.value and built.value == 1, (f.name, fp.label, hex(code.value))
+            # one clause per program column, in instruction order, the
+            # negated one (flight 3's date < d) last
+            clauses = [(code.value >> (6 * k)) & 63 for k in range(4)]
+            assert sum(1 for c in clauses if c) == len(fp.cols)
+            negated = any(op == abi.NOT for op, _, _ in spec.instrs)
+            assert [bool(c & 32) for c in clauses if c][-1] == negated
+            assert sum(bool(c & 32) for c in clauses) == int(negated)
+            if f.name == "SSB11" and fp.label == "Sparse":
+                assert clauses[:3] == [1, 8 | 16, 2 | 16]
+
+
 @pytest.mark.gpu
 def test_dbgen_flights_on_gpu(dev, ssb_host):
     """Every flight under its three fixed-layout programs and four plans."""
@@ -177,14 +212,15 @@ def test_dbgen_flights_on_gpu(dev, ssb_host):
                                         None if r is None else ptrs[r]))
             prg = S.to_vprg(cols, s.nrows, spec.instrs)
             for plan in (abi.PLAN_FUSED, abi.PLAN_FUSED | abi.PLAN_EAGER,
-                         abi.PLAN_MATERIALISE, abi.PLAN_SCAN):
+                         abi.PLAN_MATERIALISE, abi.PLAN_SCAN,
+                         abi.PLAN_FUSED | abi.PLAN_DEDICATED):
                 check(dev.lib.mbx_prog_grpby(dev.ctx, C.byref(prg), C.byref(op),
                                              spec.nr_grp, out.ptr, plan))
                 dev.sync()
                 np.testing.assert_array_equal(out.host(spec.nr_grp), want,
                                               err_msg=f"{q.name} {fp.label} plan {plan}")
                 nplans += 1
-    assert nplans == 3 * 4 * len(ssbq.FLIGHTS)
+    assert nplans == 3 * 5 * len(ssbq.FLIGHTS)
     out.free()
     for p in allocs:
         check(dev.lib.mbx_free(dev.ctx, p))
@@ -202,8 +238,9 @@ def test_demo_driver_report(ssb_host, tmp_path_factory):
     the synthetic suite (perfect / many-OR / candidate-check recipes on
     seeded Zipf columns), checks each plan against the scan plan or the other
     method (exit status 0) and prints the five-block report.  The sums it
-    writes for every SSB row and plan equal the oracle's straight loops
-    (s1ref..s4ref restated) on the same data."""
+    writes for every SSB row and plan (the dedicated kernels' included)
+    equal the oracle's straight loops (s1ref..s4ref restated) on the same
+    data."""
     demo = os.path.join(ROOT, "merle_amd", "lib", "mbx_demo")
     if not os.path.exists(demo):
         pytest.skip("mbx_demo not built")
@@ -229,8 +266,11 @@ def test_demo_driver_report(ssb_host, tmp_path_factory):
                                           err_msg=f"{name} {bt} {plan}")
             seen.add((name, bt, plan))
     bts = ("Sparse", "Medium", "Dense", "Perfect", "AllAlig", "MisAlig")
+    # the fixed layouts also run their dedicated kernels (every planned
+    # shape of these programs on this data is built)
     assert seen == {(q.name, bt, p) for q in ssbq.FLIGHTS for bt in bts
-                    for p in ("join", "ours", "stg")}
+                    for p in ("join", "ours", "stg")} | \
+        {(q.name, bt, "ded") for q in ssbq.FLIGHTS for bt in bts[:3]}
     # method 1 / method 2 (synth/cpuref.c:194-205): 7 selectivities
     meth = blocks[3].splitlines()
     assert meth[0].split("\t") == ["Seletiv", "Method1", "Method2"]
@@ -258,7 +298,9 @@ def test_demo_driver_report(ssb_host, tmp_path_factory):
                         ("Perfect", "AllAlig", "MisAlig"))[i % 3]
         assert len(f) == 10 and float(f[3]) > 0
         assert all(float(x) > 0 for x in f[4:9])   # ablations and stages
-        assert f[9] == "0"                         # no dedicated kernels
+        # dedicated kernels on the fixed layouts, none on the controlled
+        # ones (as the reference's report)
+        assert (float(f[9]) > 0) if fixed else f[9] == "0"
         # DOWORK times the scan on the fixed layouts' Sparse row only
         # (ssb/lambdas.cuh:157); the controlled rows print 0 there
         assert (float(f[2]) > 0) if fixed and i % 3 == 0 else f[2] == "0"
