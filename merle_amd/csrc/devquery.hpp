@@ -611,6 +611,137 @@ __device__ __forceinline__ uint64_t key_div(const DevKey &kp, uint64_t x) {
   return (t + ((x - t) >> kp.sh1)) >> kp.sh2;
 }
 
+// Group key and value of one row from its fetched sources v (key parts,
+// then measures a and b); ok is false for a dropped row (a source past its
+// dimension, or a key outside [0, nr_grp)).
+__device__ __forceinline__ void keyval_of(const DevOp &O,
+                                          const uint64_t (&v)[MBX_MAXKEYS + 2],
+                                          bool vok, uint32_t nr_grp,
+                                          uint32_t &key, int64_t &val,
+                                          bool &ok) {
+  int64_t k = 0;
+#pragma unroll
+  for (int i = 0; i < MBX_MAXKEYS; ++i)
+    if (i < (int)O.nkeys) {
+      const DevKey &kp = O.keys[i];
+      const uint64_t x = v[i];
+      uint64_t f = x;
+      if (kp.kind == MBX_KEY_DIV) {
+        f = key_div(kp, x);
+      } else if (kp.kind == MBX_KEY_BUCKET) {
+        f = 0;
+        for (uint32_t j = 0; j < kp.nb; ++j) f += kp.bounds[j] <= x;
+      }
+      k += kp.stride * ((int64_t)f - kp.base);
+    }
+  const uint64_t a = v[3], b = v[4];
+  val = O.measure == MBX_MEAS_MUL   ? (int64_t)(a * b)
+        : O.measure == MBX_MEAS_SUB ? (int64_t)a - (int64_t)b
+                                    : (int64_t)a;
+  key = (uint32_t)k;
+  ok = vok && k >= 0 && k < (int64_t)nr_grp;
+}
+
+// Index of a source for fact row r: r itself, or its foreign key.  Every
+// lane loads (the branch is on the source, which is wave-uniform).
+__device__ __forceinline__ uint32_t fetch_index(const DevSrc &s, uint32_t r) {
+  return s.fk ? s.fk[r] : r;
+}
+
+// Value of a source at index ix; ok = false past the dimension, where the
+// lane reads entry 0 instead (dropped), so the load is never conditional on
+// the lane.  A source with an empty dimension reads nothing.
+__device__ __forceinline__ uint64_t fetch_value(const DevSrc &s, uint32_t ix,
+                                                bool &ok) {
+  if (s.fk && s.dimsz == 0) {
+    ok = false;
+    return 0;
+  }
+  ok = s.fk == nullptr || ix < s.dimsz;
+  return ld_elem(s.data, s.eb, ok ? ix : 0u, false);
+}
+
+// The row op of E three-valued entries in one or two rounds of loads
+// whatever the number of sources -- the foreign keys of every source, then
+// every value; one round when no source goes through a foreign key: the
+// predicate values an uncertain row's re-check reads are fetched for every
+// entry together with its key parts and measures, where process_entries
+// re-checks each uncertain row inside a lane-dependent branch (its loads
+// waited for there) before the key fetch: up to E + 1 rounds.  An invalid
+// entry reads row 0 and is dropped; a certain row reads its predicate
+// values for nothing.  The columns in P.known are not read.  Used by the
+// word-by-word row op of dense ranges (an edge bin over a skewed column's
+// head: most rows uncertain), where those extra loads are coalesced.
+template <int E>
+__device__ __forceinline__ void rows_fetch_chk(const DevPrg &P, const DevOp &O,
+                                               const uint32_t *ent,
+                                               const bool *valid,
+                                               uint32_t nr_grp, uint32_t *key,
+                                               int64_t *val, bool *ok) {
+  constexpr int NS = MBX_MAXKEYS + 2;   // key parts, then measure a, b
+  const bool has_a = O.measure != MBX_MEAS_COUNT;
+  const bool has_b = O.measure == MBX_MEAS_MUL || O.measure == MBX_MEAS_SUB;
+  const uint32_t need = ~P.known & ((1u << O.npreds) - 1u);
+  uint32_t row[E];
+  bool unc[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    row[e] = valid[e] ? (ent[e] & 0x7fffffffu) : 0u;
+    unc[e] = valid[e] && (ent[e] >> 31) != 0;
+  }
+  uint32_t pix[E][MBX_MAXCOLS], kix[E][NS];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+#pragma unroll
+    for (int c = 0; c < MBX_MAXCOLS; ++c)
+      if ((need >> c) & 1u) pix[e][c] = fetch_index(O.pred[c], row[e]);
+#pragma unroll
+    for (int i = 0; i < MBX_MAXKEYS; ++i)
+      if (i < (int)O.nkeys) kix[e][i] = fetch_index(O.keys[i].src, row[e]);
+    if (has_a) kix[e][3] = fetch_index(O.ma, row[e]);
+    if (has_b) kix[e][4] = fetch_index(O.mb, row[e]);
+  }
+  uint64_t pv[E][MBX_MAXCOLS], kv[E][NS];
+  bool pok[E][MBX_MAXCOLS], vok[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+#pragma unroll
+    for (int c = 0; c < MBX_MAXCOLS; ++c)
+      if ((need >> c) & 1u) pv[e][c] = fetch_value(O.pred[c], pix[e][c], pok[e][c]);
+    vok[e] = valid[e];
+#pragma unroll
+    for (int i = 0; i < MBX_MAXKEYS; ++i)
+      if (i < (int)O.nkeys) {
+        bool o;
+        kv[e][i] = fetch_value(O.keys[i].src, kix[e][i], o);
+        vok[e] = vok[e] && o;
+      }
+    kv[e][3] = 1;
+    kv[e][4] = 0;
+    if (has_a) {
+      bool o;
+      kv[e][3] = fetch_value(O.ma, kix[e][3], o);
+      vok[e] = vok[e] && o;
+    }
+    if (has_b) {
+      bool o;
+      kv[e][4] = fetch_value(O.mb, kix[e][4], o);
+      vok[e] = vok[e] && o;
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    uint32_t m = P.known_val;
+#pragma unroll
+    for (int c = 0; c < MBX_MAXCOLS; ++c)
+      if ((need >> c) & 1u)
+        m |= (pok[e][c] && pv[e][c] >= O.lo[c] && pv[e][c] < O.hi[c]) ? 1u << c
+                                                                     : 0u;
+    const bool pass = !unc[e] || ((P.truth >> m) & 1u) != 0;
+    keyval_of(O, kv[e], vok[e] && pass, nr_grp, key[e], val[e], ok[e]);
+  }
+}
+
 // {value, group} of E selected rows at once (primitive.cuh:10-15); ok[e] is
 // false for a dropped row (fk out of range or key outside [0, nr_grp)).
 template <bool ACCT, int E>
@@ -825,7 +956,9 @@ __device__ __forceinline__ void sink_wave(const GroupSink &sink, uint32_t key,
 
 // List entries: row id, MSB = uncertain (primitive.cuh:406-407).  E entries
 // per lane share their memory round trips.
-template <bool CHK, bool ACCT, int E>
+// MERGED (three-valued, timed path only): rows_fetch_chk's single set of
+// loads for re-checks and keys.
+template <bool CHK, bool ACCT, int E, bool MERGED = false>
 __device__ __forceinline__ void process_entries(const DevPrg &P,
                                                 const DevOp &O,
                                                 uint32_t nr_grp,
@@ -833,19 +966,23 @@ __device__ __forceinline__ void process_entries(const DevPrg &P,
                                                 const bool *valid,
                                                 const GroupSink &sink,
                                                 const Acct *acct) {
-  uint32_t row[E];
-  bool live[E];
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-    row[e] = ent[e] & 0x7fffffffu;
-    live[e] = valid[e];
-    if (CHK && live[e] && (ent[e] >> 31))
-      live[e] = row_passes<ACCT>(P, O, row[e], acct);
-  }
   uint32_t key[E];
   int64_t val[E];
   bool ok[E];
-  rows_keyval<ACCT, E>(O, row, live, nr_grp, key, val, ok, acct);
+  if constexpr (MERGED && CHK && !ACCT) {
+    rows_fetch_chk<E>(P, O, ent, valid, nr_grp, key, val, ok);
+  } else {
+    uint32_t row[E];
+    bool live[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      row[e] = ent[e] & 0x7fffffffu;
+      live[e] = valid[e];
+      if (CHK && live[e] && (ent[e] >> 31))
+        live[e] = row_passes<ACCT>(P, O, row[e], acct);
+    }
+    rows_keyval<ACCT, E>(O, row, live, nr_grp, key, val, ok, acct);
+  }
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     if (ACCT) {
@@ -968,11 +1105,15 @@ constexpr uint32_t kWaveList = kListCap / 4;
 // uncertain (an edge bin over a skewed column's head) that serial fill cost
 // more than the row op itself.
 constexpr uint32_t kDenseRowsPerWord = 16;
+// Words per round of the dense path's three-valued row op (rows_fetch_chk).
+constexpr int kDenseChkE = 2;
 
 // SCAN: the words come from the rows' exact predicates (scan_words), not
 // from bitmaps.
+// DMERGE: the dense path's three-valued row op takes rows_fetch_chk, one
+// word at a time (kernels with the registers to spare for it).
 template <bool CHK, bool ACCT, int WPL, int EAGER, int E = 2, bool DIRECT = false,
-          bool SCAN = false, class F = RuntimeShape>
+          bool SCAN = false, class F = RuntimeShape, bool DMERGE = true>
 __device__ __forceinline__ void run_wave_range(const DevPrg &P, const DevOp &O,
                                                uint32_t nr_grp, uint32_t w0,
                                                uint32_t ws, uint32_t wend,
@@ -1034,15 +1175,19 @@ __device__ __forceinline__ void run_wave_range(const DevPrg &P, const DevOp &O,
       }
       wave_lds_sync();
     }
+    // three-valued: one word at a time, its re-check and key loads in one
+    // set (rows_fetch_chk)
+    constexpr bool kMerge = DMERGE && CHK && !ACCT;
+    constexpr int ED = kMerge ? kDenseChkE : E;
 #pragma unroll
     for (int k = 0; k < WPL; ++k) {
       const uint32_t wk = w0 + ws * k;   // lane s holds word wk + s
-      for (uint32_t s = 0; s < 64u; s += E) {
-        uint32_t ent[E];
-        bool valid[E];
+      for (uint32_t s = 0; s < 64u; s += ED) {
+        uint32_t ent[ED];
+        bool valid[ED];
         bool anyv = false;
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
+        for (int e = 0; e < ED; ++e) {
           const uint32_t src = s + (uint32_t)e;
           const uint64_t pw = kStage ? sw[(NV * k) * 64 + src]
                                      : __shfl(p[k], (int)src, 64);
@@ -1056,7 +1201,8 @@ __device__ __forceinline__ void run_wave_range(const DevPrg &P, const DevOp &O,
           anyv = anyv || valid[e];
         }
         if (__ballot(anyv) == 0) continue;
-        process_entries<CHK, ACCT, E>(P, O, nr_grp, ent, valid, sink, acct);
+        process_entries<CHK, ACCT, ED, kMerge>(P, O, nr_grp, ent, valid, sink,
+                                               acct);
       }
     }
     if (kStage) wave_lds_sync();   // the list space is rewritten next range
