@@ -1110,10 +1110,8 @@ constexpr int kDenseChkE = 2;
 
 // SCAN: the words come from the rows' exact predicates (scan_words), not
 // from bitmaps.
-// DMERGE: the dense path's three-valued row op takes rows_fetch_chk, one
-// word at a time (kernels with the registers to spare for it).
 template <bool CHK, bool ACCT, int WPL, int EAGER, int E = 2, bool DIRECT = false,
-          bool SCAN = false, class F = RuntimeShape, bool DMERGE = true>
+          bool SCAN = false, class F = RuntimeShape>
 __device__ __forceinline__ void run_wave_range(const DevPrg &P, const DevOp &O,
                                                uint32_t nr_grp, uint32_t w0,
                                                uint32_t ws, uint32_t wend,
@@ -1175,9 +1173,9 @@ __device__ __forceinline__ void run_wave_range(const DevPrg &P, const DevOp &O,
       }
       wave_lds_sync();
     }
-    // three-valued: one word at a time, its re-check and key loads in one
-    // set (rows_fetch_chk)
-    constexpr bool kMerge = DMERGE && CHK && !ACCT;
+    // three-valued: kDenseChkE words at a time, their re-check and key
+    // loads in one set (rows_fetch_chk)
+    constexpr bool kMerge = CHK && !ACCT;
     constexpr int ED = kMerge ? kDenseChkE : E;
 #pragma unroll
     for (int k = 0; k < WPL; ++k) {
