@@ -47,8 +47,8 @@ __device__ __forceinline__ void batch_item(const BatchQ &q, uint32_t w_begin,
   constexpr int W = kBWPL;
   const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   for (uint32_t w0 = w_begin + wv * 64; w0 < w_end; w0 += 4 * 64 * W)
-    run_wave_range<CHK, false, W, EAGER>(q.prg, q.op, q.nr_grp, w0, 4 * 64,
-                                         w_end, sink, s_list, nullptr);
+    run_wave_range<CHK, false, W, EAGER, 2, false, false, RuntimeShapeCached>(
+        q.prg, q.op, q.nr_grp, w0, 4 * 64, w_end, sink, s_list, nullptr);
 }
 
 // CONJ: the class-0 kernel (eager conjunctions, LDS table); else every form.

@@ -109,7 +109,15 @@ constexpr int kOpGroup = 4;
 // Bitmaps per load group of the conjunction form (EAGER 3).
 constexpr int kConjGroup = 4;
 
-template <bool ACCT, bool CHK, int WPL, int NC>
+// A bitmap word: streaming (non-temporal) for plans that read each word once,
+// cached where other queries read the same lines soon after (the batched
+// launch, whose items of one row range run back to back on one XCD).
+template <bool NT>
+__device__ __forceinline__ uint64_t ld_word(const uint64_t *p) {
+  return NT ? ld_stream(p) : *p;
+}
+
+template <bool ACCT, bool CHK, int WPL, int NC, bool NT = true>
 __device__ __forceinline__ void load_ops(const DevPrg &P, uint32_t e0,
                                          uint32_t e1, uint32_t w0, uint32_t ws,
                                          const bool (&ld)[WPL],
@@ -138,7 +146,7 @@ __device__ __forceinline__ void load_ops(const DevPrg &P, uint32_t e0,
         for (int k = 0; k < WPL; ++k)
           if (any[k]) {
             if (ACCT && ld[k]) mark<true>(acct, b + wa[k]);
-            v[j][k] = ld_stream(b + wa[k]);
+            v[j][k] = ld_word<NT>(b + wa[k]);
           }
       }
     }
@@ -209,6 +217,7 @@ __device__ __forceinline__ void apply_ins(uint32_t op, uint32_t d, uint32_t s,
 // compile-time constants (the dedicated kernels, whose shape is a template
 // argument).
 struct RuntimeShape {
+  static constexpr bool kNT = true;   // bitmap words: streaming loads
   static __device__ __forceinline__ uint32_t nfl(const DevPrg &P) { return P.nfl; }
   static __device__ __forceinline__ bool edge(const DevPrg &P, uint32_t e) {
     return P.fledge[e] != 0;
@@ -219,6 +228,10 @@ struct RuntimeShape {
   static __device__ __forceinline__ bool neg(const DevPrg &P, uint32_t e) {
     return P.flneg[e] != 0;
   }
+};
+// The same, bitmap words through cached loads (the batched launch).
+struct RuntimeShapeCached : RuntimeShape {
+  static constexpr bool kNT = false;
 };
 
 // EAGER 3: a conjunction of clauses (P.conj).  in[k] / keep[k]: word k is
@@ -272,7 +285,7 @@ __device__ __forceinline__ void eval_conj(const DevPrg &P, uint32_t w0,
         for (int k = 0; k < WPL; ++k)
           if (any[k]) {
             if (ACCT && in[k]) mark<true>(acct, b + wa[k]);
-            v[j][k] = ld_stream(b + wa[k]);
+            v[j][k] = ld_word<F::kNT>(b + wa[k]);
           }
       }
     }
@@ -387,7 +400,7 @@ __device__ __forceinline__ void eval_words(const DevPrg &P, uint32_t w0,
 #pragma unroll
         for (int k = 0; k < WPL; ++k)
           ld[k] = in[k] && (op == MBX_ORM || gr[k][d % 3] != 0);
-        load_ops<ACCT, CHK, WPL, 1>(P, P.beg[s], P.end[s], w0, ws, ld, cs, us,
+        load_ops<ACCT, CHK, WPL, 1, F::kNT>(P, P.beg[s], P.end[s], w0, ws, ld, cs, us,
                                     acct);
       }
 #pragma unroll
@@ -431,7 +444,7 @@ __device__ __forceinline__ void eval_words(const DevPrg &P, uint32_t w0,
         for (int k = 0; k < WPL; ++k)
           if (any[k]) {
             if (ACCT && in[k]) mark<true>(acct, b + wa[k]);
-            v[e][k] = ld_stream(b + wa[k]);
+            v[e][k] = ld_word<F::kNT>(b + wa[k]);
           }
       }
     }
@@ -466,7 +479,7 @@ __device__ __forceinline__ void eval_words(const DevPrg &P, uint32_t w0,
     for (uint32_t c = 0; c < MBX_MAXCOLS; ++c)
 #pragma unroll
       for (int k = 0; k < WPL; ++k) pc[c][k] = pu[c][k] = 0;
-    load_ops<ACCT, CHK, WPL, MBX_MAXCOLS>(P, 0, P.nfl, w0, ws, in, pc, pu,
+    load_ops<ACCT, CHK, WPL, MBX_MAXCOLS, F::kNT>(P, 0, P.nfl, w0, ws, in, pc, pu,
                                           acct);
 #pragma unroll
     for (int k = 0; k < WPL; ++k) {
@@ -500,7 +513,7 @@ __device__ __forceinline__ void eval_words(const DevPrg &P, uint32_t w0,
 #pragma unroll
       for (int k = 0; k < WPL; ++k)
         ld[k] = in[k] && (op == MBX_ORM || rget(p0[k], p1[k], p2[k], d) != 0);
-      load_ops<ACCT, CHK, WPL, 1>(P, P.beg[s], P.end[s], w0, ws, ld, cs, us,
+      load_ops<ACCT, CHK, WPL, 1, F::kNT>(P, P.beg[s], P.end[s], w0, ws, ld, cs, us,
                                   acct);
     }
 #pragma unroll
