@@ -234,6 +234,56 @@ enum mbx_flags {
                                 (see mbx_prog_shape) */
 };
 
+/* ---------------------------------------------- WAH compressed bitmaps */
+/* The compressed-bitmap baseline the reference's report sets beside the
+ * fused path (merle/wahGpuCmprs.cu, merle/wahGpuDirect.cu,
+ * ssb/wah-demo.cu); the WAH format is the reference's, byte for byte.
+ * Groups: 31 rows per uint32, row 31g+j at bit 30-j of group g, bit 31
+ * zero; mbx_wah_groups_of(N) = ceil(N/31) groups, rows >= N zero (the
+ * reference's dbjoinFlatWah makes floor(N/31) and drops the last rows).
+ * WAH words: bit 31 set = a fill (bit 30 = the fill bit, bits 0-29 = its
+ * groups), else a literal holding one group.  All arrays are device uint32
+ * and at most 2^30 - 1 words or groups long. */
+size_t mbx_wah_groups_of(uint32_t nrows);
+/* dbjoinFlatWah (wahGpuCmprs.cu:247-320): groups of [lo <= v < hi] over a
+ * column or a dimension attribute through foreign keys (keys >= dimsz
+ * select nothing). */
+int mbx_wah_select(mbx_ctx *ctx, const mbx_colref *src, uint32_t nrows,
+                   uint64_t lo, uint64_t hi, uint32_t *groups);
+/* The groups of a bitmap of this library (bit b of word k = row 32k+b). */
+int mbx_wah_groups(mbx_ctx *ctx, const uint32_t *bitmap, uint32_t nrows,
+                   uint32_t *groups);
+/* wahCompress (wahGpuCmprs.cu:215-245): each maximal run of all-zero or
+ * all-one groups becomes one fill, every other group a literal, and a
+ * trailing zero run is dropped.  out holds up to ngroups words; *nout gets
+ * the count (the call waits for its kernels). */
+int mbx_wah_compress(mbx_ctx *ctx, const uint32_t *groups, size_t ngroups,
+                     uint32_t *out, size_t *nout);
+/* wahCntExcScan (wahGpuCmprs.cu:6-14): excl[i] = groups before word i,
+ * excl[n] = all groups (n + 1 entries).  Waits for its kernels (pool
+ * scratch). */
+int mbx_wah_scan(mbx_ctx *ctx, const uint32_t *wah, size_t n, uint32_t *excl);
+/* wahAndNo1 (wahGpuDirect.cu:8-34): up AND down in down's word structure
+ * (out has ndown words): a 0-fill of down stays, a literal (or a 1-fill of
+ * one group) is ANDed with the up word covering its group -- groups past up
+ * read zero -- and a zero result becomes a 0-fill of one group.  down must
+ * hold no 1-fill longer than one group (mbx_wah_longest_1fill <= 1).  The
+ * reference's SSB driver passes exclusive scans where the function expects
+ * inclusive ones (ssb/wah-demo.cu:48-61), so its lookup can name the word
+ * after a covering fill; here the covering word is always the one used. */
+int mbx_wah_and_no1(mbx_ctx *ctx, const uint32_t *up, const uint32_t *up_excl,
+                    size_t nup, const uint32_t *down,
+                    const uint32_t *down_excl, size_t ndown, uint32_t *out);
+/* wahDecomp (wahGpuCmprs.cu:37-51): the first ngroups groups of the words,
+ * zero past their last group. */
+int mbx_wah_decompress(mbx_ctx *ctx, const uint32_t *wah, const uint32_t *excl,
+                       size_t n, uint32_t *groups, size_t ngroups);
+/* wahLongest1Fill (merle/host/wahConv.c:6-20): the longest run of one
+ * groups held by consecutive 1-fill words (a 0x7fffffff literal counts as
+ * one).  Waits for its kernel. */
+int mbx_wah_longest_1fill(mbx_ctx *ctx, const uint32_t *wah, size_t n,
+                          uint64_t *longest);
+
 /* Program + group-by in one call.  d_out[nr_grp] int64 on the device.
  * The fused plan never materialises a bitmap. */
 int mbx_prog_grpby(mbx_ctx *ctx, const mbx_vprg *prg, const mbx_rowop *op,

@@ -151,6 +151,14 @@ PROTOTYPES = {
     "mbx_create_select": (_i32, [_vp, _vp, _u32, _u32, _u32, _u32, _P(_vp)]),
     "mbx_create_join": (_i32, [_vp, _vp, _vp, _u32, _u32, _u32, _u32, _u32,
                                _P(_vp)]),
+    "mbx_wah_groups_of": (_sz, [_u32]),
+    "mbx_wah_select": (_i32, [_vp, _P(ColRef), _u32, _u64, _u64, _vp]),
+    "mbx_wah_groups": (_i32, [_vp, _vp, _u32, _vp]),
+    "mbx_wah_compress": (_i32, [_vp, _vp, _sz, _vp, _P(_sz)]),
+    "mbx_wah_scan": (_i32, [_vp, _vp, _sz, _vp]),
+    "mbx_wah_and_no1": (_i32, [_vp, _vp, _vp, _sz, _vp, _vp, _sz, _vp]),
+    "mbx_wah_decompress": (_i32, [_vp, _vp, _vp, _sz, _vp, _sz]),
+    "mbx_wah_longest_1fill": (_i32, [_vp, _vp, _sz, _P(_u64)]),
     "mbx_prog_grpby": (_i32, [_vp, _P(Vprg), _P(RowOp), _u32, _vp, _u32]),
     "mbx_prog_grpby_noalloc": (_i32, [_vp, _P(Vprg), _P(RowOp), _u32, _vp,
                                       _u32]),

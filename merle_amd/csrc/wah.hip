@@ -1,0 +1,524 @@
+// wah.hip -- the WAH compressed-bitmap baseline on gfx950: the format and
+// operations the reference's report compares the fused path against
+// (merle/wahGpuCmprs.cu, merle/wahGpuDirect.cu, ssb/wah-demo.cu).
+//
+// Group layout: 31 rows per uint32, row 31g+j at bit 30-j of group g, bit 31
+// zero (dbjoinFlatWah, wahGpuCmprs.cu:247-320); ceil(N/31) groups, rows >= N
+// zero.  WAH words: bit 31 set = a fill (bit 30 = the fill bit, bits 0-29 =
+// groups), else a literal holding one group.
+//
+// Kernels: select (a predicate straight into groups: each wave ballots 31
+// row blocks of 64, stages the 31 masks in LDS and lane g cuts group g out
+// of them), groups (the same from a 64-row-word bitmap), compress (keep
+// flags -> tile sums -> tile offsets -> ordered scatter -> fill lengths),
+// count scan (the same three-pass tile scan), AND in the down operand's
+// word structure, decompress, and the longest 1-fill.  The AND and the
+// decompression give each thread 16 consecutive items: one binary search
+// for its first item, then a forward walk.
+#include <algorithm>
+#include <cstring>
+
+#include "devfuncs.hpp"
+#include "mbx_internal.hpp"
+
+namespace mbx {
+
+constexpr uint32_t kWahFill = 0x80000000u;
+constexpr uint32_t kWahOne = 0x40000000u;
+constexpr uint32_t kWahCnt = 0x3fffffffu;
+constexpr uint32_t kWahRows = 0x7fffffffu;
+// a tile of the scans: 256 threads x kWahItems consecutive items
+constexpr uint32_t kWahItems = 16;
+constexpr uint32_t kWahTile = 256 * kWahItems;
+
+__device__ __forceinline__ uint32_t wah_count(uint32_t w) {
+  return (w & kWahFill) ? (w & kWahCnt) : 1u;
+}
+__device__ __forceinline__ uint32_t wah_rows(uint32_t w) {
+  if (w & kWahFill) return (w & kWahOne) ? kWahRows : 0u;
+  return w & kWahRows;
+}
+// 31 rows with row j at bit j -> row j at bit 30 - j
+__device__ __forceinline__ uint32_t wah_order(uint32_t x) {
+  return __builtin_bitreverse32(x & kWahRows) >> 1;
+}
+
+// Exclusive scan over the 256 threads of a workgroup; *total = the sum.
+__device__ __forceinline__ uint32_t wah_block_scan(uint32_t x, uint32_t *s_w,
+                                                   uint32_t *total) {
+  const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
+  const uint32_t incl = wave_incl_scan(x);
+  if (lane == 63) s_w[wv] = incl;
+  __syncthreads();
+  uint32_t before = 0, sum = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < 4; ++j) {
+    const uint32_t t = s_w[j];
+    before += j < wv ? t : 0;
+    sum += t;
+  }
+  __syncthreads();
+  *total = sum;
+  return before + incl - x;
+}
+
+// ------------------------------------------------------------ groups
+// Wave w covers rows [1984w, 1984w + 1984): 31 ballots of 64 rows, then
+// lane g holds group 64w + g = bits 31g..31g+30 of the 1984 ballot bits.
+__global__ __launch_bounds__(256) void wah_select_kernel(DevSrc src,
+                                                         uint32_t nrows,
+                                                         uint64_t lo,
+                                                         uint64_t hi,
+                                                         uint32_t *groups,
+                                                         uint32_t ngroups) {
+  __shared__ uint64_t s_m[4][32];
+  const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
+  const uint64_t wave = (uint64_t)blockIdx.x * 4 + wv;
+  const uint64_t r0 = wave * 1984u;
+  uint64_t v[31];
+  bool ok[31];
+  if (src.fk && src.dimsz == 0) {   // no dimension row: nothing selected
+#pragma unroll
+    for (int b = 0; b < 31; ++b) {
+      ok[b] = false;
+      v[b] = 0;
+    }
+  } else if (src.fk) {
+    uint32_t k[31];
+#pragma unroll
+    for (int b = 0; b < 31; ++b) {
+      const uint64_t row = r0 + 64u * b + lane;
+      k[b] = row < nrows ? src.fk[row] : 0xffffffffu;
+    }
+#pragma unroll
+    for (int b = 0; b < 31; ++b) {
+      ok[b] = k[b] < src.dimsz;
+      v[b] = ld_elem(src.data, src.eb, ok[b] ? k[b] : 0u, false);
+    }
+  } else {
+#pragma unroll
+    for (int b = 0; b < 31; ++b) {
+      const uint64_t row = r0 + 64u * b + lane;
+      ok[b] = row < nrows;
+      v[b] = ld_elem(src.data, src.eb, ok[b] ? row : 0u, false);
+    }
+  }
+#pragma unroll
+  for (int b = 0; b < 31; ++b) {
+    const uint64_t m = __ballot(ok[b] && v[b] >= lo && v[b] < hi);
+    if (lane == 0) s_m[wv][b] = m;
+  }
+  if (lane == 0) s_m[wv][31] = 0;
+  wave_lds_sync();
+  const uint32_t bit = 31u * lane, w = bit >> 6, sh = bit & 63u;
+  uint64_t x = s_m[wv][w] >> sh;
+  if (sh > 33) x |= s_m[wv][w + 1] << (64 - sh);
+  const uint64_t g = wave * 64u + lane;
+  if (g < ngroups) groups[g] = wah_order((uint32_t)x);
+}
+
+// From a bitmap (uint32 words, bit b of word k = row 32k+b): group g is
+// bits 31g..31g+30, within two words.
+__global__ __launch_bounds__(256) void wah_groups_kernel(
+    const uint32_t *__restrict__ bm, uint32_t nrows, uint32_t *groups,
+    uint32_t ngroups) {
+  const uint32_t nw = (uint32_t)(((uint64_t)nrows + 31) / 32);
+  for (uint64_t g = (uint64_t)blockIdx.x * 256 + threadIdx.x; g < ngroups;
+       g += (uint64_t)gridDim.x * 256) {
+    const uint64_t bit = 31u * g;
+    const uint32_t w = (uint32_t)(bit >> 5), sh = (uint32_t)(bit & 31u);
+    uint32_t x = bm[w] >> sh;
+    if (sh > 1 && w + 1 < nw) x |= bm[w + 1] << (32 - sh);
+    const uint64_t left = (uint64_t)nrows - bit;   // rows of this group < N
+    if (left < 31) x &= (1u << left) - 1u;
+    groups[g] = wah_order(x);
+  }
+}
+
+// ---------------------------------------------------------- compress
+// A group is kept when it is a literal or the last of its fill run; the
+// last group only when it is not zero, so a trailing zero run leaves
+// nothing (wahCompress, wahGpuCmprs.cu:215-245).
+__device__ __forceinline__ bool wah_keep(const uint32_t *groups, uint32_t g,
+                                         uint32_t ng) {
+  const uint32_t v = groups[g] & kWahRows;
+  if (g + 1 == ng) return v != 0;
+  if (v != 0 && v != kWahRows) return true;
+  return (groups[g + 1] & kWahRows) != v;
+}
+
+__global__ __launch_bounds__(256) void wah_keep_count_kernel(
+    const uint32_t *__restrict__ groups, uint32_t ng, uint32_t *tile_sums) {
+  __shared__ uint32_t s_w[4];
+  const uint32_t g0 = blockIdx.x * kWahTile + threadIdx.x * kWahItems;
+  uint32_t c = 0;
+#pragma unroll
+  for (uint32_t i = 0; i < kWahItems; ++i)
+    c += (g0 + i < ng && wah_keep(groups, g0 + i, ng)) ? 1u : 0u;
+  uint32_t total;
+  (void)wah_block_scan(c, s_w, &total);
+  if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+// Exclusive scan of ntiles tile sums in place by one workgroup;
+// tile_sums[ntiles] = the total.
+__global__ __launch_bounds__(256) void wah_tiles_scan_kernel(
+    uint32_t *tile_sums, uint32_t ntiles) {
+  __shared__ uint32_t s_w[4];
+  uint32_t carry = 0;
+  for (uint32_t b = 0; b < ntiles; b += 256) {
+    const uint32_t i = b + threadIdx.x;
+    const uint32_t x = i < ntiles ? tile_sums[i] : 0u;
+    uint32_t total;
+    const uint32_t ex = wah_block_scan(x, s_w, &total);
+    if (i < ntiles) tile_sums[i] = carry + ex;
+    carry += total;
+  }
+  if (threadIdx.x == 0) tile_sums[ntiles] = carry;
+}
+
+// Kept groups in order: out[pos] = the literal or the fill's kind, and
+// ends[pos] = g + 1, the end of the run the word closes.
+__global__ __launch_bounds__(256) void wah_scatter_kernel(
+    const uint32_t *__restrict__ groups, uint32_t ng,
+    const uint32_t *__restrict__ tile_offs, uint32_t *out, uint32_t *ends) {
+  __shared__ uint32_t s_w[4];
+  const uint32_t g0 = blockIdx.x * kWahTile + threadIdx.x * kWahItems;
+  uint32_t keep = 0;
+#pragma unroll
+  for (uint32_t i = 0; i < kWahItems; ++i)
+    if (g0 + i < ng && wah_keep(groups, g0 + i, ng)) keep |= 1u << i;
+  uint32_t total;
+  uint32_t pos = tile_offs[blockIdx.x] +
+                 wah_block_scan((uint32_t)__popc(keep), s_w, &total);
+#pragma unroll
+  for (uint32_t i = 0; i < kWahItems; ++i) {
+    if (!((keep >> i) & 1u)) continue;
+    const uint32_t v = groups[g0 + i] & kWahRows;
+    out[pos] = v == 0 ? kWahFill : v == kWahRows ? (kWahFill | kWahOne) : v;
+    ends[pos] = g0 + i + 1;
+    ++pos;
+  }
+}
+
+// A fill's length: its run is every group since the previous kept word.
+__global__ __launch_bounds__(256) void wah_fill_len_kernel(
+    uint32_t *out, const uint32_t *__restrict__ ends,
+    const uint32_t *__restrict__ count) {
+  const uint32_t n = *count;
+  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n;
+       i += gridDim.x * 256) {
+    const uint32_t w = out[i];
+    if (w & kWahFill) out[i] = w | (ends[i] - (i ? ends[i - 1] : 0u));
+  }
+}
+
+// ------------------------------------------------------- count scan
+__global__ __launch_bounds__(256) void wah_count_sum_kernel(
+    const uint32_t *__restrict__ wah, uint32_t n, uint32_t *tile_sums) {
+  __shared__ uint32_t s_w[4];
+  const uint32_t i0 = blockIdx.x * kWahTile + threadIdx.x * kWahItems;
+  uint32_t c = 0;
+#pragma unroll
+  for (uint32_t i = 0; i < kWahItems; ++i)
+    c += i0 + i < n ? wah_count(wah[i0 + i]) : 0u;
+  uint32_t total;
+  (void)wah_block_scan(c, s_w, &total);
+  if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(256) void wah_count_scan_kernel(
+    const uint32_t *__restrict__ wah, uint32_t n,
+    const uint32_t *__restrict__ tile_offs, uint32_t *excl) {
+  __shared__ uint32_t s_w[4];
+  const uint32_t i0 = blockIdx.x * kWahTile + threadIdx.x * kWahItems;
+  uint32_t c[kWahItems], s = 0;
+#pragma unroll
+  for (uint32_t i = 0; i < kWahItems; ++i) {
+    c[i] = i0 + i < n ? wah_count(wah[i0 + i]) : 0u;
+    s += c[i];
+  }
+  uint32_t total;
+  uint32_t acc = tile_offs[blockIdx.x] + wah_block_scan(s, s_w, &total);
+#pragma unroll
+  for (uint32_t i = 0; i < kWahItems; ++i) {
+    if (i0 + i < n) excl[i0 + i] = acc;
+    acc += c[i];
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 255)
+    excl[n] = tile_offs[gridDim.x];
+}
+
+// ------------------------------------------------------ AND, decompress
+// The word of (excl, n) holding group g: excl[u] <= g < excl[u + 1]; n when
+// g is past the last group.
+__device__ __forceinline__ uint32_t wah_find(const uint32_t *__restrict__ excl,
+                                             uint32_t n, uint32_t g) {
+  if (n == 0 || g >= excl[n]) return n;
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const uint32_t mid = lo + ((hi - lo) >> 1);
+    if (excl[mid] <= g) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// wahAndNo1 (wahGpuDirect.cu:8-34): down's word structure; a 0-fill stays,
+// a literal (a 1-fill of one group reads as one) is ANDed with the up word
+// covering its group (past up: 0), a zero result becomes a 0-fill of one.
+__global__ __launch_bounds__(256) void wah_and_no1_kernel(
+    const uint32_t *__restrict__ up, const uint32_t *__restrict__ up_excl,
+    uint32_t nup, const uint32_t *__restrict__ down,
+    const uint32_t *__restrict__ down_excl, uint32_t ndown, uint32_t *out) {
+  const uint32_t d0 = (blockIdx.x * 256 + threadIdx.x) * kWahItems;
+  if (d0 >= ndown) return;
+  uint32_t u = wah_find(up_excl, nup, down_excl[d0]);
+  for (uint32_t d = d0; d < d0 + kWahItems && d < ndown; ++d) {
+    uint32_t dw = down[d];
+    if (dw == (kWahFill | kWahOne | 1u)) dw = kWahRows;
+    uint32_t o;
+    if (dw & kWahFill) {
+      o = dw;
+    } else {
+      const uint32_t g = down_excl[d];
+      while (u < nup && up_excl[u + 1] <= g) ++u;
+      o = dw & (u < nup ? wah_rows(up[u]) : 0u);
+      if (o == 0) o = kWahFill | 1u;
+    }
+    out[d] = o;
+  }
+}
+
+// wahDecomp (wahGpuCmprs.cu:37-51): groups of the words, zero past them.
+__global__ __launch_bounds__(256) void wah_decompress_kernel(
+    const uint32_t *__restrict__ wah, const uint32_t *__restrict__ excl,
+    uint32_t n, uint32_t *groups, uint32_t ng) {
+  const uint32_t g0 = (blockIdx.x * 256 + threadIdx.x) * kWahItems;
+  if (g0 >= ng) return;
+  const uint32_t total = n ? excl[n] : 0u;
+  uint32_t u = wah_find(excl, n, g0);
+  for (uint32_t g = g0; g < g0 + kWahItems && g < ng; ++g) {
+    uint32_t v = 0;
+    if (g < total) {
+      while (u + 1 < n && excl[u + 1] <= g) ++u;
+      v = wah_rows(wah[u]);
+    }
+    groups[g] = v;
+  }
+}
+
+// wahLongest1Fill (merle/host/wahConv.c:6-20): a thread starts at each word
+// that opens a run of consecutive one fills and walks it.
+__device__ __forceinline__ bool wah_is_one(uint32_t w) {
+  return w == kWahRows || (w >> 30) == 3u;
+}
+__global__ __launch_bounds__(256) void wah_longest_1fill_kernel(
+    const uint32_t *__restrict__ wah, uint32_t n, unsigned long long *best) {
+  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n;
+       i += gridDim.x * 256) {
+    if (!wah_is_one(wah[i]) || (i > 0 && wah_is_one(wah[i - 1]))) continue;
+    unsigned long long run = 0;
+    for (uint32_t j = i; j < n && wah_is_one(wah[j]); ++j)
+      run += wah[j] == kWahRows ? 1u : (wah[j] & kWahCnt);
+    atomicMax(best, run);
+  }
+}
+
+// ------------------------------------------------------------- host
+static unsigned grid_of(size_t items, size_t per_block) {
+  const size_t g = ceil_div(items, per_block);
+  return g ? (unsigned)g : 1u;
+}
+
+// Pool scratch freed after the context's stream has drained.
+struct Scratch {
+  mbx_ctx *ctx;
+  void *p = nullptr;
+  explicit Scratch(mbx_ctx *c) : ctx(c) {}
+  ~Scratch() {
+    if (p) {
+      (void)hipStreamSynchronize(ctx->stream);
+      ctx->pool->free(p);
+    }
+  }
+};
+
+static int wah_check(mbx_ctx *ctx, const void *p, const char *what) {
+  if (!ctx) return fail(MBX_ERR_INVALID, std::string(what) + ": null ctx");
+  if (reinterpret_cast<uintptr_t>(p) % 4)
+    return fail(MBX_ERR_ALIGN, std::string(what) + ": misaligned array");
+  return MBX_OK;
+}
+
+}  // namespace mbx
+
+using namespace mbx;
+
+extern "C" {
+
+size_t mbx_wah_groups_of(uint32_t nrows) { return ((size_t)nrows + 30) / 31; }
+
+int mbx_wah_select(mbx_ctx *ctx, const mbx_colref *src, uint32_t nrows,
+                   uint64_t lo, uint64_t hi, uint32_t *groups) {
+  if (!src || !groups) return fail(MBX_ERR_INVALID, "wah_select: null");
+  int st = wah_check(ctx, groups, "wah_select");
+  if (st) return st;
+  st = check_src(*src, "wah_select");
+  if (st) return st;
+  const uint32_t ng = (uint32_t)mbx_wah_groups_of(nrows);
+  if (!ng) return MBX_OK;
+  DevSrc d;
+  d.data = src->data;
+  d.fk = src->fk;
+  d.dimsz = src->dimsz;
+  d.eb = src->elem_bytes;
+  hipLaunchKernelGGL(wah_select_kernel, dim3(grid_of(ng, 256)), dim3(256), 0,
+                     ctx->stream, d, nrows, lo, hi, groups, ng);
+  MBX_HIP(hipGetLastError());
+  return MBX_OK;
+}
+
+int mbx_wah_groups(mbx_ctx *ctx, const uint32_t *bitmap, uint32_t nrows,
+                   uint32_t *groups) {
+  if (!bitmap || !groups) return fail(MBX_ERR_INVALID, "wah_groups: null");
+  int st = wah_check(ctx, groups, "wah_groups");
+  if (!st) st = wah_check(ctx, bitmap, "wah_groups");
+  if (st) return st;
+  const size_t ng = mbx_wah_groups_of(nrows);
+  if (!ng) return MBX_OK;
+  const unsigned grid =
+      (unsigned)std::min<size_t>(ceil_div(ng, 256), (size_t)ctx->num_cus * 16);
+  hipLaunchKernelGGL(wah_groups_kernel, dim3(grid), dim3(256), 0, ctx->stream,
+                     bitmap, nrows, groups, (uint32_t)ng);
+  MBX_HIP(hipGetLastError());
+  return MBX_OK;
+}
+
+int mbx_wah_compress(mbx_ctx *ctx, const uint32_t *groups, size_t ngroups,
+                     uint32_t *out, size_t *nout) {
+  if (!groups || !out || !nout) return fail(MBX_ERR_INVALID, "wah_compress: null");
+  int st = wah_check(ctx, groups, "wah_compress");
+  if (!st) st = wah_check(ctx, out, "wah_compress");
+  if (st) return st;
+  *nout = 0;
+  if (ngroups == 0) return MBX_OK;
+  if (ngroups > kWahCnt)
+    return fail(MBX_ERR_CAPACITY, "wah_compress: more groups than a fill can count");
+  const uint32_t ng = (uint32_t)ngroups;
+  const uint32_t ntiles = (uint32_t)ceil_div(ng, kWahTile);
+  Scratch s(ctx);
+  st = ctx->pool->alloc(((size_t)ntiles + 1) * 4 + (size_t)ng * 4 + 256, &s.p);
+  if (st) return st;
+  uint32_t *tiles = static_cast<uint32_t *>(s.p);
+  uint32_t *ends = tiles + ((ntiles + 1 + 63) / 64) * 64;
+  hipLaunchKernelGGL(wah_keep_count_kernel, dim3(ntiles), dim3(256), 0,
+                     ctx->stream, groups, ng, tiles);
+  hipLaunchKernelGGL(wah_tiles_scan_kernel, dim3(1), dim3(256), 0, ctx->stream,
+                     tiles, ntiles);
+  hipLaunchKernelGGL(wah_scatter_kernel, dim3(ntiles), dim3(256), 0,
+                     ctx->stream, groups, ng, tiles, out, ends);
+  hipLaunchKernelGGL(wah_fill_len_kernel,
+                     dim3((unsigned)std::min<size_t>(ceil_div(ng, 256),
+                                                     (size_t)ctx->num_cus * 16)),
+                     dim3(256), 0, ctx->stream, out, ends, tiles + ntiles);
+  MBX_HIP(hipGetLastError());
+  uint32_t k = 0;
+  MBX_HIP(hipMemcpyAsync(&k, tiles + ntiles, 4, hipMemcpyDeviceToHost,
+                         ctx->stream));
+  MBX_HIP(hipStreamSynchronize(ctx->stream));
+  *nout = k;
+  return MBX_OK;
+}
+
+int mbx_wah_scan(mbx_ctx *ctx, const uint32_t *wah, size_t n, uint32_t *excl) {
+  if ((!wah && n) || !excl) return fail(MBX_ERR_INVALID, "wah_scan: null");
+  int st = wah_check(ctx, excl, "wah_scan");
+  if (!st && n) st = wah_check(ctx, wah, "wah_scan");
+  if (st) return st;
+  if (n > kWahCnt) return fail(MBX_ERR_CAPACITY, "wah_scan: too many words");
+  if (n == 0) return launch_fill_u32(ctx->stream, excl, 0, 1);
+  const uint32_t ntiles = (uint32_t)ceil_div(n, kWahTile);
+  Scratch s(ctx);
+  st = ctx->pool->alloc(((size_t)ntiles + 1) * 4, &s.p);
+  if (st) return st;
+  uint32_t *tiles = static_cast<uint32_t *>(s.p);
+  hipLaunchKernelGGL(wah_count_sum_kernel, dim3(ntiles), dim3(256), 0,
+                     ctx->stream, wah, (uint32_t)n, tiles);
+  hipLaunchKernelGGL(wah_tiles_scan_kernel, dim3(1), dim3(256), 0, ctx->stream,
+                     tiles, ntiles);
+  hipLaunchKernelGGL(wah_count_scan_kernel, dim3(ntiles), dim3(256), 0,
+                     ctx->stream, wah, (uint32_t)n, tiles, excl);
+  MBX_HIP(hipGetLastError());
+  return MBX_OK;
+}
+
+int mbx_wah_and_no1(mbx_ctx *ctx, const uint32_t *up, const uint32_t *up_excl,
+                    size_t nup, const uint32_t *down,
+                    const uint32_t *down_excl, size_t ndown, uint32_t *out) {
+  if (!up_excl || !down_excl || !out || (!up && nup) || (!down && ndown))
+    return fail(MBX_ERR_INVALID, "wah_and_no1: null");
+  for (const void *p : {(const void *)up, (const void *)up_excl,
+                        (const void *)down, (const void *)down_excl,
+                        (const void *)out}) {
+    int st = wah_check(ctx, p, "wah_and_no1");
+    if (st) return st;
+  }
+  if (nup > kWahCnt || ndown > kWahCnt)
+    return fail(MBX_ERR_CAPACITY, "wah_and_no1: too many words");
+  if (ndown == 0) return MBX_OK;
+  hipLaunchKernelGGL(wah_and_no1_kernel,
+                     dim3(grid_of(ceil_div(ndown, kWahItems), 256)), dim3(256),
+                     0, ctx->stream, up, up_excl, (uint32_t)nup, down,
+                     down_excl, (uint32_t)ndown, out);
+  MBX_HIP(hipGetLastError());
+  return MBX_OK;
+}
+
+int mbx_wah_decompress(mbx_ctx *ctx, const uint32_t *wah, const uint32_t *excl,
+                       size_t n, uint32_t *groups, size_t ngroups) {
+  if (!excl || !groups || (!wah && n))
+    return fail(MBX_ERR_INVALID, "wah_decompress: null");
+  int st = wah_check(ctx, groups, "wah_decompress");
+  if (!st) st = wah_check(ctx, excl, "wah_decompress");
+  if (!st && n) st = wah_check(ctx, wah, "wah_decompress");
+  if (st) return st;
+  if (n > kWahCnt || ngroups > 0xffffffffull - kWahItems * 256ull)
+    return fail(MBX_ERR_CAPACITY, "wah_decompress: too many words or groups");
+  if (ngroups == 0) return MBX_OK;
+  hipLaunchKernelGGL(wah_decompress_kernel,
+                     dim3(grid_of(ceil_div(ngroups, kWahItems), 256)), dim3(256),
+                     0, ctx->stream, wah, excl, (uint32_t)n, groups,
+                     (uint32_t)ngroups);
+  MBX_HIP(hipGetLastError());
+  return MBX_OK;
+}
+
+int mbx_wah_longest_1fill(mbx_ctx *ctx, const uint32_t *wah, size_t n,
+                          uint64_t *longest) {
+  if (!longest || (!wah && n)) return fail(MBX_ERR_INVALID, "wah_longest_1fill: null");
+  int st = n ? wah_check(ctx, wah, "wah_longest_1fill")
+             : (ctx ? MBX_OK : fail(MBX_ERR_INVALID, "wah_longest_1fill: null ctx"));
+  if (st) return st;
+  *longest = 0;
+  if (n == 0) return MBX_OK;
+  if (n > kWahCnt) return fail(MBX_ERR_CAPACITY, "wah_longest_1fill: too many words");
+  Scratch s(ctx);
+  st = ctx->pool->alloc(256, &s.p);
+  if (st) return st;
+  st = launch_fill_u32(ctx->stream, s.p, 0, 2);
+  if (st) return st;
+  hipLaunchKernelGGL(wah_longest_1fill_kernel,
+                     dim3((unsigned)std::min<size_t>(ceil_div(n, 256),
+                                                     (size_t)ctx->num_cus * 16)),
+                     dim3(256), 0, ctx->stream, wah, (uint32_t)n,
+                     static_cast<unsigned long long *>(s.p));
+  MBX_HIP(hipGetLastError());
+  unsigned long long v = 0;
+  MBX_HIP(hipMemcpyAsync(&v, s.p, 8, hipMemcpyDeviceToHost, ctx->stream));
+  MBX_HIP(hipStreamSynchronize(ctx->stream));
+  *longest = v;
+  return MBX_OK;
+}
+
+}  // extern "C"

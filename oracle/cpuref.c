@@ -528,3 +528,147 @@ double ref_mlp_loss_grad(const float *x, const float *t, int n, int din,
   free(pre); free(dy); free(gw1); free(gbb1); free(gw2); free(gbb2);
   return loss * scale;
 }
+
+/* ------------------------------------------------------------------------ */
+/* WAH compressed-bitmap baseline                                           */
+/* ------------------------------------------------------------------------ */
+#define WAH_FILL 0x80000000u
+#define WAH_ONE 0x40000000u
+#define WAH_CNT 0x3fffffffu
+#define WAH_ROWS 0x7fffffffu
+
+static inline size_t wah_ngroups(uint32_t n) { return ((size_t)n + 30) / 31; }
+
+/* dbjoinFlatWah (wahGpuCmprs.cu:247-320): lane 30-j of a warp tests row
+ * base+j, so the ballot puts the group's first row at bit 30. */
+void ref_wah_select(const ref_src *src, uint32_t n, uint64_t lo, uint64_t hi,
+                    uint32_t *groups) {
+  const size_t G = wah_ngroups(n);
+  #pragma omp parallel for schedule(static)
+  for (long g = 0; g < (long)G; ++g) {
+    uint32_t w = 0;
+    for (uint32_t j = 0; j < 31; ++j) {
+      const size_t row = (size_t)g * 31 + j;
+      if (row >= n) break;
+      uint64_t v;
+      if (fetch(src, row, &v) && v >= lo && v < hi) w |= 1u << (30 - j);
+    }
+    groups[g] = w;
+  }
+}
+
+void ref_wah_groups(const uint32_t *bitmap, uint32_t n, uint32_t *groups) {
+  const size_t G = wah_ngroups(n);
+  #pragma omp parallel for schedule(static)
+  for (long g = 0; g < (long)G; ++g) {
+    uint32_t w = 0;
+    for (uint32_t j = 0; j < 31; ++j) {
+      const size_t row = (size_t)g * 31 + j;
+      if (row >= n) break;
+      if ((bitmap[row >> 5] >> (row & 31)) & 1u) w |= 1u << (30 - j);
+    }
+    groups[g] = w;
+  }
+}
+
+/* wahCompress (wahGpuCmprs.cu:215-245): a group is kept when it is a
+ * literal, or the last of its fill run; the last group only when it is not
+ * zero (so a trailing zero run leaves nothing).  A kept fill carries the
+ * groups since the previous kept group. */
+size_t ref_wah_compress(const uint32_t *groups, size_t ng, uint32_t *out) {
+  size_t k = 0, prev_end = 0;
+  for (size_t i = 0; i < ng; ++i) {
+    const uint32_t v = groups[i] & WAH_ROWS;
+    const int fill = v == 0 || v == WAH_ROWS;
+    int keep;
+    if (i + 1 == ng) keep = v != 0;
+    else keep = !fill || (groups[i + 1] & WAH_ROWS) != v;
+    if (!keep) continue;
+    if (fill)
+      out[k++] = WAH_FILL | (v ? WAH_ONE : 0u) | (uint32_t)(i + 1 - prev_end);
+    else
+      out[k++] = v;
+    prev_end = i + 1;
+  }
+  return k;
+}
+
+static inline uint32_t wah_count(uint32_t w) { return (w & WAH_FILL) ? (w & WAH_CNT) : 1u; }
+/* the 31 rows a word stands for in each of its groups */
+static inline uint32_t wah_rows(uint32_t w) {
+  if (w & WAH_FILL) return (w & WAH_ONE) ? WAH_ROWS : 0u;
+  return w & WAH_ROWS;
+}
+
+void ref_wah_scan(const uint32_t *wah, size_t n, uint32_t *excl) {
+  uint32_t acc = 0;
+  for (size_t i = 0; i < n; ++i) {
+    excl[i] = acc;
+    acc += wah_count(wah[i]);
+  }
+  excl[n] = acc;
+}
+
+/* the word of (wah, excl, n) holding group g; n when g is past the end */
+static size_t wah_find(const uint32_t *excl, size_t n, uint32_t g) {
+  if (n == 0 || g >= excl[n]) return n;
+  size_t lo = 0, hi = n;   /* excl[lo] <= g < excl[hi] */
+  while (hi - lo > 1) {
+    const size_t mid = lo + (hi - lo) / 2;
+    if (excl[mid] <= g) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+/* wahAndNo1 (wahGpuDirect.cu:8-34) with the up word taken as the one that
+ * covers the down word's group.  The reference finds it by a lower-bound
+ * search of inclusive scans (its parameters upIncScan / downIncScan), but
+ * its SSB driver passes the exclusive scans of wahCntExcScan
+ * (ssb/wah-demo.cu:48-61), so there the search names the word after a
+ * covering fill that starts before the group. */
+void ref_wah_and_no1(const uint32_t *up, const uint32_t *up_excl, size_t nup,
+                     const uint32_t *down, const uint32_t *down_excl,
+                     size_t ndown, uint32_t *out) {
+  #pragma omp parallel for schedule(static)
+  for (long d = 0; d < (long)ndown; ++d) {
+    uint32_t dw = down[d];
+    if (dw == (WAH_FILL | WAH_ONE | 1u)) dw = WAH_ROWS;
+    uint32_t o;
+    if (dw & WAH_FILL) {
+      o = dw;
+    } else {
+      const size_t u = wah_find(up_excl, nup, down_excl[d]);
+      o = dw & (u < nup ? wah_rows(up[u]) : 0u);
+      if (o == 0) o = WAH_FILL | 1u;
+    }
+    out[d] = o;
+  }
+}
+
+void ref_wah_decompress(const uint32_t *wah, const uint32_t *excl, size_t n,
+                        uint32_t *groups, size_t ng) {
+  size_t g = 0;
+  for (size_t i = 0; i < n && g < ng; ++i) {
+    const uint32_t c = wah_count(wah[i]), r = wah_rows(wah[i]);
+    for (uint32_t k = 0; k < c && g < ng; ++k) groups[g++] = r;
+  }
+  (void)excl;
+  for (; g < ng; ++g) groups[g] = 0;
+}
+
+uint64_t ref_wah_longest_1fill(const uint32_t *wah, size_t n) {
+  uint64_t best = 0, cur = 0;
+  for (size_t i = 0; i < n; ++i) {
+    uint32_t w = wah[i];
+    if (w == WAH_ROWS) w = WAH_FILL | WAH_ONE | 1u;
+    if ((w >> 30) == 3u) {
+      cur += w & WAH_CNT;
+      if (cur > best) best = cur;
+    } else {
+      cur = 0;
+    }
+  }
+  return best;
+}
+

@@ -140,6 +140,41 @@ void ref_synth(const void *fa1, const void *fa2, uint32_t bitwidth,
                uint64_t fa1hi, uint64_t fa2lo, uint64_t fa2hi, uint64_t da1lo,
                uint64_t da1hi, int64_t grpout[256]);
 
+/* ---- WAH compressed-bitmap baseline (merle/wahGpuCmprs.cu,
+ *      merle/wahGpuDirect.cu, merle/host/wahConv.c) ------------------------ */
+/* Group layout (dbjoinFlatWah, wahGpuCmprs.cu:247-320): 31 rows per uint32,
+ * row 31g+j at bit 30-j of group g, bit 31 zero; ceil(n/31) groups, rows
+ * >= n zero (the reference makes floor(n/31) groups and drops the tail). */
+void ref_wah_select(const ref_src *src, uint32_t n, uint64_t lo, uint64_t hi,
+                    uint32_t *groups);
+/* The same groups from a bitmap of this layout family (uint32 words, bit b
+ * of word k = row 32k+b). */
+void ref_wah_groups(const uint32_t *bitmap, uint32_t n, uint32_t *groups);
+/* WAH words: bit 31 set = fill (bit 30 = fill bit, bits 0-29 = groups),
+ * else a literal of 31 rows.  wahCompress (wahGpuCmprs.cu:215-245): each
+ * maximal run of all-zero or all-one groups -> one fill, every other group a
+ * literal, a trailing zero run dropped.  Returns the number of words. */
+size_t ref_wah_compress(const uint32_t *groups, size_t ng, uint32_t *out);
+/* wahCntExcScan (wahGpuCmprs.cu:6-14): excl[i] = groups before word i,
+ * excl[n] = all groups. */
+void ref_wah_scan(const uint32_t *wah, size_t n, uint32_t *excl);
+/* wahAndNo1 (wahGpuDirect.cu:8-34): the AND in `down`'s word structure: a
+ * 0-fill of down stays, a literal (or 1-fill of one group) is ANDed with
+ * the up word covering its group (groups past up read 0), a zero result
+ * becomes a 0-fill of one group.  down must hold no 1-fill longer than one
+ * group.  out has ndown words. */
+void ref_wah_and_no1(const uint32_t *up, const uint32_t *up_excl, size_t nup,
+                     const uint32_t *down, const uint32_t *down_excl,
+                     size_t ndown, uint32_t *out);
+/* wahDecomp (wahGpuCmprs.cu:37-51): the groups of the words, then zero
+ * groups up to ng. */
+void ref_wah_decompress(const uint32_t *wah, const uint32_t *excl, size_t n,
+                        uint32_t *groups, size_t ng);
+/* wahLongest1Fill (merle/host/wahConv.c:6-20): the longest run of one
+ * groups held by consecutive 1-fill words (a 0x7fffffff literal counts as
+ * a 1-fill of one group). */
+uint64_t ref_wah_longest_1fill(const uint32_t *wah, size_t n);
+
 /* ---- cost-model MLP (fp32 forward/backward, double accumulation) --------- */
 /* y = W2 relu(W1 x + b1) + b2 ; x [n,din], W1 [dh,din], W2 [dout,dh].
  * loss = mean over n*dout of (y - t)^2.  Gradients are written if non-NULL.

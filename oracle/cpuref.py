@@ -92,6 +92,20 @@ def lib():
         L.ref_synth.restype = None
         L.ref_mlp_loss_grad.argtypes = [vp, vp, i32, i32, i32, i32] + [vp] * 9
         L.ref_mlp_loss_grad.restype = C.c_double
+        L.ref_wah_select.argtypes = [P(RefSrc), u32, u64, u64, vp]
+        L.ref_wah_select.restype = None
+        L.ref_wah_groups.argtypes = [vp, u32, vp]
+        L.ref_wah_groups.restype = None
+        L.ref_wah_compress.argtypes = [vp, sz, vp]
+        L.ref_wah_compress.restype = sz
+        L.ref_wah_scan.argtypes = [vp, sz, vp]
+        L.ref_wah_scan.restype = None
+        L.ref_wah_and_no1.argtypes = [vp, vp, sz, vp, vp, sz, vp]
+        L.ref_wah_and_no1.restype = None
+        L.ref_wah_decompress.argtypes = [vp, vp, sz, vp, sz]
+        L.ref_wah_decompress.restype = None
+        L.ref_wah_longest_1fill.argtypes = [vp, sz]
+        L.ref_wah_longest_1fill.restype = u64
         _lib = L
     return _lib
 
@@ -230,3 +244,63 @@ def mlp_loss_grad(x, t, W1, b1, W2, b2):
         *[np.ascontiguousarray(a, np.float32).ctypes.data for a in (W1, b1, W2, b2)],
         *[a.ctypes.data for a in g], y.ctypes.data)
     return loss, g, y
+
+
+# ---------------------------------------------------------------- WAH
+def wah_ngroups(n):
+    """31-row groups of an n-row bitmap."""
+    return (n + 30) // 31
+
+
+def wah_select(col, lo, hi, fk=None, dimsz=0):
+    """Groups (31 rows per uint32, first row at bit 30) of [lo <= v < hi]."""
+    n = len(fk) if fk is not None else len(col)
+    src = raw_src(col, fk, dimsz)
+    out = np.zeros(wah_ngroups(n), np.uint32)
+    lib().ref_wah_select(C.byref(src), n, lo, hi, _ptr(out))
+    return out
+
+
+def wah_groups(bitmap, n):
+    out = np.zeros(wah_ngroups(n), np.uint32)
+    lib().ref_wah_groups(_ptr(np.ascontiguousarray(bitmap, np.uint32)), n,
+                         _ptr(out))
+    return out
+
+
+def wah_compress(groups):
+    g = np.ascontiguousarray(groups, np.uint32)
+    out = np.zeros(max(1, len(g)), np.uint32)
+    k = lib().ref_wah_compress(_ptr(g), len(g), _ptr(out))
+    return out[:k].copy()
+
+
+def wah_scan(wah):
+    w = np.ascontiguousarray(wah, np.uint32)
+    out = np.zeros(len(w) + 1, np.uint32)
+    lib().ref_wah_scan(_ptr(w), len(w), _ptr(out))
+    return out
+
+
+def wah_and_no1(up, down):
+    up = np.ascontiguousarray(up, np.uint32)
+    down = np.ascontiguousarray(down, np.uint32)
+    su, sd = wah_scan(up), wah_scan(down)
+    out = np.zeros(max(1, len(down)), np.uint32)
+    lib().ref_wah_and_no1(_ptr(up), _ptr(su), len(up), _ptr(down), _ptr(sd),
+                          len(down), _ptr(out))
+    return out[:len(down)].copy()
+
+
+def wah_decompress(wah, ng):
+    w = np.ascontiguousarray(wah, np.uint32)
+    s = wah_scan(w)
+    out = np.zeros(max(1, ng), np.uint32)
+    lib().ref_wah_decompress(_ptr(w), _ptr(s), len(w), _ptr(out), ng)
+    return out[:ng].copy()
+
+
+def wah_longest_1fill(wah):
+    w = np.ascontiguousarray(wah, np.uint32)
+    return int(lib().ref_wah_longest_1fill(_ptr(w), len(w)))
+

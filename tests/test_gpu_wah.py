@@ -1,0 +1,213 @@
+"""WAH compressed-bitmap baseline on the GPU (merle/wahGpuCmprs.cu,
+merle/wahGpuDirect.cu) through the C ABI, bit-exact against the oracle
+(oracle/cpuref.c, whose WAH functions are pinned in test_oracle.py)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from merle_amd import abi
+from merle_amd.host import check
+from oracle import cpuref
+
+pytestmark = pytest.mark.gpu
+
+O, ONES = 0x80000000, 0x7fffffff
+
+
+def _runs(rng, ng, p_lit=0.3, max_run=40, p_one=0.3):
+    out = []
+    while len(out) < ng:
+        r = rng.random()
+        if r < p_lit:
+            out.append(int(rng.integers(1, ONES)))
+        elif r < p_lit + (1 - p_lit) * p_one:
+            out += [ONES] * int(rng.integers(1, max_run))
+        else:
+            out += [0] * int(rng.integers(1, max_run))
+    return np.array(out[:ng], np.uint32)
+
+
+def _colref(arr, fk=None, dimsz=0):
+    r = abi.ColRef()
+    r.data = arr.ptr
+    r.elem_bytes = arr.dtype.itemsize
+    r.fk = fk.ptr if fk is not None else None
+    r.dimsz = dimsz
+    return r
+
+
+def _compress(dev, groups_dev, ng):
+    out = dev.alloc(4 * max(ng, 1), np.uint32, max(ng, 1))
+    k = C.c_size_t()
+    check(dev.lib.mbx_wah_compress(dev.ctx, groups_dev.ptr, ng, out.ptr,
+                                   C.byref(k)))
+    return out, k.value
+
+
+def _scan(dev, wah_dev, n):
+    ex = dev.alloc(4 * (n + 1), np.uint32, n + 1)
+    check(dev.lib.mbx_wah_scan(dev.ctx, wah_dev.ptr if n else None, n, ex.ptr))
+    return ex
+
+
+@pytest.mark.parametrize("n", [1, 30, 31, 32, 63, 64, 1983, 1984, 1985, 70001,
+                               300007])
+@pytest.mark.parametrize("dt", [np.uint8, np.uint16, np.uint32, np.uint64])
+def test_wah_select_and_groups(dev, n, dt):
+    """Groups of a predicate (direct and through foreign keys, keys past
+    the dimension included) and of the library's own bitmaps == oracle."""
+    rng = np.random.default_rng(n + np.dtype(dt).itemsize)
+    v = rng.integers(0, 200, n).astype(dt)
+    lo, hi = 30, 90
+    ng = int(dev.lib.mbx_wah_groups_of(n))
+    assert ng == (n + 30) // 31
+    dv = dev.upload(v)
+    g = dev.alloc(4 * ng, np.uint32, ng)
+    check(dev.lib.mbx_wah_select(dev.ctx, C.byref(_colref(dv)), n, lo, hi, g.ptr))
+    want = cpuref.wah_select(v, lo, hi)
+    np.testing.assert_array_equal(g.host(), want)
+    # through a foreign key
+    dim = rng.integers(0, 200, 777).astype(dt)
+    fk = rng.integers(0, 800, n).astype(np.uint32)
+    ddim, dfk = dev.upload(dim), dev.upload(fk)
+    check(dev.lib.mbx_wah_select(dev.ctx, C.byref(_colref(ddim, dfk, 777)), n,
+                                 lo, hi, g.ptr))
+    np.testing.assert_array_equal(g.host(), cpuref.wah_select(dim, lo, hi, fk=fk,
+                                                              dimsz=777))
+    # from a bitmap of the library
+    if dt == np.uint32:
+        bm = C.c_void_p()
+        check(dev.lib.mbx_create_select(dev.ctx, dv.ptr, 4, n, lo, hi,
+                                        C.byref(bm)))
+        g2 = dev.alloc(4 * ng, np.uint32, ng)
+        check(dev.lib.mbx_wah_groups(dev.ctx, bm.value, n, g2.ptr))
+        np.testing.assert_array_equal(g2.host(), want)
+        check(dev.lib.mbx_free(dev.ctx, bm.value))
+        g2.free()
+    for a in (dv, g, ddim, dfk):
+        a.free()
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_wah_compress_scan_decompress(dev, seed):
+    """compress (run structure, trailing zero run dropped), the count scan
+    and decompression == oracle; decompression to more groups pads zeros,
+    to fewer truncates."""
+    rng = np.random.default_rng(seed)
+    for ng in (1, 2, 17, 4095, 4096, 4097, 100_003, int(rng.integers(5e5, 2e6))):
+        x = _runs(rng, ng, p_lit=float(rng.choice([0.0, 0.02, 0.3, 0.9, 1.0])),
+                  p_one=float(rng.choice([0.0, 0.3])))
+        if seed == 0 and ng == 17:
+            x[:] = 0
+        dx = dev.upload(x)
+        out, k = _compress(dev, dx, ng)
+        want = cpuref.wah_compress(x)
+        assert k == len(want), (ng, k, len(want))
+        got = out.host(k)
+        np.testing.assert_array_equal(got, want)
+        ex = _scan(dev, out, k)
+        np.testing.assert_array_equal(ex.host(), cpuref.wah_scan(want))
+        for cap in (ng, ng + 100, max(1, ng // 2)):
+            dec = dev.alloc(4 * cap, np.uint32, cap)
+            check(dev.lib.mbx_wah_decompress(dev.ctx, out.ptr, ex.ptr, k, dec.ptr,
+                                             cap))
+            np.testing.assert_array_equal(dec.host(),
+                                          cpuref.wah_decompress(want, cap))
+            dec.free()
+        for a in (dx, out, ex):
+            a.free()
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_wah_and_no1(dev, seed):
+    """up AND down in down's word structure == oracle (which decompresses
+    to the verbatim AND), including a down literal inside an up 0-fill that
+    starts before it and down words past the end of up."""
+    rng = np.random.default_rng(100 + seed)
+    cases = [(np.array([O | 5, 0x7ffffffe], np.uint32),
+              np.array([O | 2, 0x12345, 0x33, 0x44, 0x55, 0x66, 0x77], np.uint32))]
+    for _ in range(5):
+        ng = int(rng.choice([50, 5000, 300_000]))
+        a = _runs(rng, ng, p_lit=float(rng.choice([0.05, 0.5])))
+        b = _runs(rng, int(ng * rng.choice([0.5, 1.0, 1.5])),
+                  p_lit=float(rng.choice([0.05, 0.5])), p_one=0.0)
+        cases.append((cpuref.wah_compress(a), cpuref.wah_compress(b)))
+    for up, down in cases:
+        if len(down) == 0 or len(up) == 0:
+            continue
+        du, dd = dev.upload(up), dev.upload(down)
+        su, sd = _scan(dev, du, len(up)), _scan(dev, dd, len(down))
+        out = dev.alloc(4 * len(down), np.uint32, len(down))
+        check(dev.lib.mbx_wah_and_no1(dev.ctx, du.ptr, su.ptr, len(up), dd.ptr,
+                                      sd.ptr, len(down), out.ptr))
+        np.testing.assert_array_equal(out.host(), cpuref.wah_and_no1(up, down))
+        for x in (du, dd, su, sd, out):
+            x.free()
+
+
+def test_wah_longest_1fill(dev):
+    """Consecutive one fills add up; a 0x7fffffff literal counts as one."""
+    rng = np.random.default_rng(7)
+    arrays = [np.array([0xc0000003, 0xc0000002, 5, ONES, 0xc0000004, O | 9,
+                        0xc0000001], np.uint32)]
+    for _ in range(6):
+        w = rng.choice(np.array([0xc0000001, 0xc0000007, ONES, O | 3, 12345],
+                                np.uint32), int(rng.integers(1, 100_000)))
+        arrays.append(w.astype(np.uint32))
+    for w in arrays:
+        dw = dev.upload(w)
+        got = C.c_uint64()
+        check(dev.lib.mbx_wah_longest_1fill(dev.ctx, dw.ptr, len(w),
+                                            C.byref(got)))
+        assert got.value == cpuref.wah_longest_1fill(w)
+        dw.free()
+
+
+def test_wah_full_size_chain(dev):
+    """120M rows (3,870,968 groups): two predicates' groups, compressed,
+    ANDed, scanned and decompressed == the oracle's groups of the AND."""
+    n = 120_000_000
+    rng = np.random.default_rng(11)
+    a = rng.integers(0, 1000, n, dtype=np.uint32)
+    b = rng.integers(0, 1000, n, dtype=np.uint32)
+    ng = (n + 30) // 31
+    da, db = dev.upload(a), dev.upload(b)
+    ga, gb = (dev.alloc(4 * ng, np.uint32, ng) for _ in range(2))
+    check(dev.lib.mbx_wah_select(dev.ctx, C.byref(_colref(da)), n, 0, 10, ga.ptr))
+    check(dev.lib.mbx_wah_select(dev.ctx, C.byref(_colref(db)), n, 100, 400, gb.ptr))
+    wa, ka = _compress(dev, ga, ng)
+    wb, kb = _compress(dev, gb, ng)
+    want_a = cpuref.wah_compress(cpuref.wah_select(a, 0, 10))
+    want_b = cpuref.wah_compress(cpuref.wah_select(b, 100, 400))
+    np.testing.assert_array_equal(wa.host(ka), want_a)
+    np.testing.assert_array_equal(wb.host(kb), want_b)
+    lf = C.c_uint64()
+    check(dev.lib.mbx_wah_longest_1fill(dev.ctx, wa.ptr, ka, C.byref(lf)))
+    assert lf.value <= 1
+    sa, sb = _scan(dev, wa, ka), _scan(dev, wb, kb)
+    r = dev.alloc(4 * ka, np.uint32, ka)
+    check(dev.lib.mbx_wah_and_no1(dev.ctx, wb.ptr, sb.ptr, kb, wa.ptr, sa.ptr, ka,
+                                  r.ptr))
+    sr = _scan(dev, r, ka)
+    dec = dev.alloc(4 * ng, np.uint32, ng)
+    check(dev.lib.mbx_wah_decompress(dev.ctx, r.ptr, sr.ptr, ka, dec.ptr, ng))
+    want = cpuref.wah_select(a, 0, 10) & cpuref.wah_select(b, 100, 400)
+    np.testing.assert_array_equal(dec.host(), want)
+    for x in (da, db, ga, gb, wa, wb, sa, sb, r, sr, dec):
+        x.free()
+
+
+def test_wah_errors(dev):
+    """Misaligned arrays and sizes past what a fill can count are refused
+    before any launch."""
+    g = dev.alloc(4096, np.uint32, 1024)
+    k = C.c_size_t()
+    assert dev.lib.mbx_wah_compress(dev.ctx, g.ptr + 2, 10, g.ptr, C.byref(k)) \
+        == abi.ERR_ALIGN
+    assert dev.lib.mbx_wah_compress(dev.ctx, g.ptr, 1 << 30, g.ptr, C.byref(k)) \
+        == abi.ERR_CAPACITY
+    assert dev.lib.mbx_wah_scan(dev.ctx, g.ptr, 1 << 30, g.ptr) == abi.ERR_CAPACITY
+    assert dev.lib.mbx_wah_and_no1(dev.ctx, None, g.ptr, 5, g.ptr, g.ptr, 5,
+                                   g.ptr) == abi.ERR_INVALID
+    g.free()

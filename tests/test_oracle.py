@@ -351,3 +351,176 @@ def test_golden_fixtures():
         s.add(k, z[k])
     got, _ = cpuref.query(spec, s)
     np.testing.assert_array_equal(got, z["sums"])
+
+
+# ------------------------------------------------------------------ WAH
+# The WAH baseline (merle/wahGpuCmprs.cu, merle/wahGpuDirect.cu).  Pinned by
+# the group layout's closed form, hand-worked known answers, round trips,
+# the verbatim AND, and -- where the reference's CPU codec
+# (merle/host/wahCpu.c, wahConv.c) is built into oracle/_ref -- by that
+# codec's own outputs.
+WAHREF = os.path.join(os.path.dirname(cpuref.__file__), "_ref", "libwahref.so")
+
+
+def _wahref():
+    import ctypes as C
+    if not os.path.exists(WAHREF):
+        pytest.skip("oracle/_ref/libwahref.so not built (needs the reference tree)")
+    L = C.CDLL(WAHREF)
+    vp, sz = C.c_void_p, C.c_size_t
+    L.wahCmprsCPU.argtypes = [vp, sz, vp]
+    L.wahCmprsCPU.restype = sz
+    L.wahAndCPU.argtypes = [vp] * 6
+    L.wahAndCPU.restype = vp
+    L.wahLongest1Fill.argtypes = [vp, sz]
+    L.wahLongest1Fill.restype = sz
+    return L
+
+
+def _runs(rng, ng, p_lit=0.3, max_run=40, p_one=0.3):
+    """Groups made of zero runs, one runs and random literals."""
+    out = []
+    while len(out) < ng:
+        r = rng.random()
+        if r < p_lit:
+            out.append(int(rng.integers(1, 0x7fffffff)))
+        elif r < p_lit + (1 - p_lit) * p_one:
+            out += [0x7fffffff] * int(rng.integers(1, max_run))
+        else:
+            out += [0] * int(rng.integers(1, max_run))
+    return np.array(out[:ng], np.uint32)
+
+
+@pytest.mark.parametrize("n", [1, 30, 31, 32, 61, 62, 63, 64, 65, 1000, 12345])
+def test_wah_groups_closed_form(n):
+    """Row 31g+j is bit 30-j of group g (dbjoinFlatWah's ballot,
+    wahGpuCmprs.cu:256-300), ceil(n/31) groups, bit 31 and rows >= n zero;
+    the groups of a bitmap equal those of the predicate."""
+    rng = np.random.default_rng(n)
+    v = rng.integers(0, 100, n).astype(np.uint32)
+    lo, hi = 20, 70
+    g = cpuref.wah_select(v, lo, hi)
+    assert len(g) == (n + 30) // 31
+    want = np.zeros(len(g), np.uint32)
+    for row in range(n):
+        if lo <= v[row] < hi:
+            want[row // 31] |= np.uint32(1 << (30 - row % 31))
+    np.testing.assert_array_equal(g, want)
+    assert not np.any(g >> 31)
+    bm = cpuref.create_join(v, lo, hi)
+    np.testing.assert_array_equal(cpuref.wah_groups(bm, n), want)
+    # through a foreign key, keys past the dimension select nothing
+    dim = rng.integers(0, 100, 50).astype(np.uint32)
+    fk = rng.integers(0, 60, n).astype(np.uint32)
+    gj = cpuref.wah_select(dim, lo, hi, fk=fk, dimsz=50)
+    wj = np.zeros(len(g), np.uint32)
+    for row in range(n):
+        if fk[row] < 50 and lo <= dim[fk[row]] < hi:
+            wj[row // 31] |= np.uint32(1 << (30 - row % 31))
+    np.testing.assert_array_equal(gj, wj)
+
+
+def test_wah_compress_known_answers():
+    """wahCompress (wahGpuCmprs.cu:215-245) worked by hand: a run of zero
+    or one groups is one fill carrying its length, a single one a fill of
+    one, literals stay, a trailing zero run is dropped, the last group stays
+    when it is not zero."""
+    L, O = 0x7fffffff, 0x80000000
+    cases = [
+        ([0, 0, 0, 5, L, L, 0, 7, 0, 0], [O | 3, 5, 0xc0000002, O | 1, 7]),
+        ([5], [5]), ([0], []), ([L], [0xc0000001]), ([0, 0, L], [O | 2, 0xc0000001]),
+        ([L, L, L, 0], [0xc0000003]), ([9, 9, 9], [9, 9, 9]),
+        ([0, 3, 0, 3, 0], [O | 1, 3, O | 1, 3]), ([], []),
+    ]
+    for groups, want in cases:
+        got = cpuref.wah_compress(np.array(groups, np.uint32))
+        assert list(got) == want, (groups, list(map(hex, got)))
+        # scan: groups before each word, then all groups
+        s = cpuref.wah_scan(got)
+        cnt = [w & 0x3fffffff if w >> 31 else 1 for w in want]
+        assert list(s) == list(np.concatenate([[0], np.cumsum(cnt)]).astype(int))
+
+
+def test_wah_compress_round_trip_and_reference_codec():
+    """decompress(compress(x)) == x (zero groups past the last word); the
+    compression equals the reference's CPU codec (wahCmprsCPU,
+    merle/host/wahConv.c:36-57) up to the trailing zero fill that the GPU
+    compression drops."""
+    rng = np.random.default_rng(5)
+    ref = None
+    try:
+        ref = _wahref()
+    except pytest.skip.Exception:
+        pass
+    for trial in range(40):
+        ng = int(rng.integers(1, 3000))
+        x = _runs(rng, ng, p_lit=float(rng.choice([0.02, 0.3, 0.9])))
+        w = cpuref.wah_compress(x)
+        np.testing.assert_array_equal(cpuref.wah_decompress(w, ng), x)
+        s = cpuref.wah_scan(w)
+        nz = np.nonzero(x)[0]
+        assert s[-1] == (nz[-1] + 1 if len(nz) else 0)
+        if ref is not None:
+            buf = np.full(ng + 2, 0xffffffff, np.uint32)
+            ref.wahCmprsCPU(x.ctypes.data, ng, buf.ctypes.data)
+            k = int(np.argmax(buf == 0xffffffff))
+            r = buf[:k]
+            if len(r) and r[-1] >> 30 == 2:
+                r = r[:-1]
+            np.testing.assert_array_equal(w, r)
+
+
+def test_wah_and_no1_against_verbatim_and():
+    """wahAndNo1 (wahGpuDirect.cu:8-34): down's words kept, each literal
+    ANDed with the up word covering its group; decompressed it is the AND
+    of the decompressed inputs.  A down literal inside an up zero fill that
+    starts before it reads zero (a search that took the next up word would
+    AND it with a literal).  The reference's CPU AND (wahAndCPU,
+    merle/host/wahCpu.c:23-78, fully merged output) decompresses to the
+    same groups."""
+    O = 0x80000000
+    up = np.array([O | 5, 0x7ffffffe], np.uint32)
+    down = np.array([O | 2, 0x12345, 0x33, 0x44, 0x55], np.uint32)
+    got = cpuref.wah_and_no1(up, down)
+    assert list(got) == [O | 2, O | 1, O | 1, O | 1, 0x55 & 0x7ffffffe]
+    rng = np.random.default_rng(9)
+    ref = None
+    try:
+        ref = _wahref()
+    except pytest.skip.Exception:
+        pass
+    for trial in range(40):
+        ng = int(rng.integers(1, 2000))
+        a = _runs(rng, ng, p_lit=float(rng.choice([0.05, 0.5])))
+        b = _runs(rng, ng, p_lit=float(rng.choice([0.05, 0.5])), p_one=0.0)
+        b[b == 0x7fffffff] = 0x7ffffffe   # down: no fill of ones
+        wa, wb = cpuref.wah_compress(a), cpuref.wah_compress(b)
+        assert cpuref.wah_longest_1fill(wb) <= 1
+        r = cpuref.wah_and_no1(wa, wb)
+        assert len(r) == len(wb)
+        np.testing.assert_array_equal(cpuref.wah_decompress(r, ng), a & b)
+        if ref is not None:
+            pa = np.concatenate([wa, [0, 0]]).astype(np.uint32)
+            pb = np.concatenate([wb, [0, 0]]).astype(np.uint32)
+            out = np.zeros(len(wa) + len(wb) + 4, np.uint32)
+            if len(wa) == 0 or len(wb) == 0:
+                continue
+            end = ref.wahAndCPU(pa.ctypes.data, pa.ctypes.data + 4 * len(wa),
+                                pb.ctypes.data, pb.ctypes.data + 4 * len(wb),
+                                out.ctypes.data, out.ctypes.data + 4 * len(out))
+            k = (end - out.ctypes.data) // 4
+            np.testing.assert_array_equal(cpuref.wah_decompress(out[:k], ng), a & b)
+
+
+def test_wah_longest_1fill_reference_codec():
+    """wahLongest1Fill (merle/host/wahConv.c:6-20): consecutive one fills
+    add up, a 0x7fffffff literal counts as a fill of one."""
+    w = np.array([0xc0000003, 0xc0000002, 5, 0x7fffffff, 0xc0000004, 0x80000009,
+                  0xc0000001], np.uint32)
+    assert cpuref.wah_longest_1fill(w) == 5
+    ref = _wahref()
+    rng = np.random.default_rng(3)
+    for _ in range(30):
+        x = _runs(rng, int(rng.integers(1, 2000)))
+        w = cpuref.wah_compress(x)
+        assert cpuref.wah_longest_1fill(w) == ref.wahLongest1Fill(w.ctypes.data, len(w))
