@@ -19,8 +19,11 @@
 // index creation and the SSB block in full (BinType Sparse / Medium / Dense
 // / Perfect / AllAlig / MisAlig), the method and synthetic blocks in full
 // when synth_rows > 0 (seeded columns of that many rows, see synth_suite),
-// the WAH / RTScan block with its header and `nan` values (WAH and RTScan
-// are not built).  SSB columns: Join = the bitmap-free scan (Sparse rows
+// the WAH / RTScan block with each flight's WAH index-only time (its
+// predicates' WAH bitmaps ANDed, the result decompressed; RTScan has no
+// counterpart and reads nan), and the index block's WAH columns (every
+// bin's WAH bitmap built and compressed, and its size).  SSB columns: Join
+// = the bitmap-free scan (Sparse rows
 // only, as DOWORK), Ours = fused under the library's static plan, PrgStg1 /
 // Stg2 / Stg3 = the three materialise stages, Basefus = the base_fuse
 // ablation (method 1 with the program's registers in device memory),
@@ -562,6 +565,47 @@ double now_ms() {
       .count();
 }
 
+// One predicate as a WAH bitmap (device words and their count scan).
+struct Wah {
+  uint32_t *w = nullptr, *ex = nullptr;
+  size_t n = 0;
+  uint64_t longest1 = 0;
+};
+
+// The predicate's groups (mbx_wah_select, the analogue of dbjoinFlatWah)
+// compressed into a pool buffer, then scanned; `groups` is ng words of
+// scratch.  False on a library error (message printed).
+bool wah_of(mbx_ctx *ctx, const mbx_colref &src, uint32_t nrows, uint64_t lo,
+            uint64_t hi, uint32_t *groups, Wah &out) {
+  const size_t ng = mbx_wah_groups_of(nrows);
+  size_t k = 0;
+  uint32_t *tmp = nullptr;
+  if (mbx_wah_select(ctx, &src, nrows, lo, hi, groups) != MBX_OK ||
+      mbx_alloc(ctx, 4 * (ng ? ng : 1), (void **)&tmp) != MBX_OK ||
+      mbx_wah_compress(ctx, groups, ng, tmp, &k) != MBX_OK) {
+    std::fprintf(stderr, "wah: %s\n", mbx_last_error());
+    return false;
+  }
+  out.n = k;
+  if (mbx_alloc(ctx, 4 * (k ? k : 1), (void **)&out.w) != MBX_OK ||
+      mbx_alloc(ctx, 4 * (k + 1), (void **)&out.ex) != MBX_OK ||
+      (k && mbx_memcpy_d2d(ctx, out.w, tmp, 4 * k) != MBX_OK) ||
+      mbx_wah_scan(ctx, out.w, k, out.ex) != MBX_OK ||
+      mbx_wah_longest_1fill(ctx, out.w, k, &out.longest1) != MBX_OK) {
+    std::fprintf(stderr, "wah: %s\n", mbx_last_error());
+    return false;
+  }
+  mbx_sync(ctx);
+  mbx_free(ctx, tmp);
+  return true;
+}
+
+void wah_free(mbx_ctx *ctx, Wah &w) {
+  if (w.w) mbx_free(ctx, w.w);
+  if (w.ex) mbx_free(ctx, w.ex);
+  w = Wah();
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -602,15 +646,106 @@ int main(int argc, char **argv) {
   t.nrows = (uint32_t)t.c[DATE].n;
   mbx_timer *tm = nullptr;
   mbx_timer_create(ctx, &tm);
+  auto timed = [&](auto &&fn) {
+    float ms = 0, sum = 0;
+    for (int r = 0; r < reps; ++r) {
+      mbx_timer_start(ctx, tm);
+      if (fn() != MBX_OK) {
+        std::fprintf(stderr, "launch: %s\n", mbx_last_error());
+        std::exit(2);
+      }
+      mbx_timer_stop(ctx, tm, &ms);
+      sum += ms;
+    }
+    return sum / reps;
+  };
+
+  // ------------------------------------------ block 0: the WAH baseline
+  // ssb/wah-demo.cu:24-292: each flight's predicates (its exact ranges, as
+  // the Perfect layout below) as WAH bitmaps, built and scanned untimed;
+  // timed per rep, index only: the predicates ANDed left to right
+  // (mbx_wah_and_no1, the left operand as `up`, so each right operand must
+  // hold no long 1-fill -- else the two swap, and a flight where neither
+  // can be `down` prints nan), the result scanned and decompressed.
+  // RTScan (OptiX ray tracing) has no counterpart here: nan.
+  const size_t ngroups = mbx_wah_groups_of(t.nrows);
+  uint32_t *wgroups = nullptr, *wres[2] = {nullptr, nullptr}, *wex = nullptr;
+  if (mbx_alloc(ctx, 4 * ngroups, (void **)&wgroups) != MBX_OK ||
+      mbx_alloc(ctx, 4 * ngroups, (void **)&wres[0]) != MBX_OK ||
+      mbx_alloc(ctx, 4 * ngroups, (void **)&wres[1]) != MBX_OK ||
+      mbx_alloc(ctx, 4 * (ngroups + 1), (void **)&wex) != MBX_OK) {
+    std::fprintf(stderr, "alloc: %s\n", mbx_last_error());
+    return 2;
+  }
+  std::vector<float> wah_ms;
+  std::vector<std::vector<uint32_t>> wah_res;
+  for (const Flight &f : kFlights) {
+    mbx_rowop op;
+    std::vector<Pred> pr = flight_op(f, t, op);
+    if (f.kind == 1 && pr[2].lo == 0) pr[2].lo = 1;   // quantity >= 1
+    if (f.kind == 4 && f.a[6] <= D920101 && f.a[7] >= D990101) pr.pop_back();
+    std::vector<Wah> w(pr.size());
+    for (size_t i = 0; i < pr.size(); ++i)
+      if (!wah_of(ctx, t.ref(pr[i].src), t.nrows, pr[i].lo, pr[i].hi, wgroups,
+                  w[i]))
+        return 2;
+    // operand order: acc = w[0]; each next operand is `down` unless it has
+    // a long 1-fill and the running result does not
+    bool ok = true;
+    std::vector<std::pair<int, bool>> steps;   // (operand, operand is down)
+    uint64_t acc1 = w[0].longest1;
+    for (size_t i = 1; i < w.size() && ok; ++i) {
+      if (w[i].longest1 <= 1) steps.push_back({(int)i, true});
+      else if (acc1 <= 1) steps.push_back({(int)i, false});
+      else ok = false;
+      acc1 = 1;   // a result keeps down's words: no long 1-fill
+    }
+    float ms = NAN;
+    if (ok) {
+      ms = timed([&] {
+        const uint32_t *cur = w[0].w, *cur_ex = w[0].ex;
+        size_t cur_n = w[0].n;
+        int buf = 0;
+        for (const auto &st : steps) {
+          const Wah &o = w[st.first];
+          const uint32_t *up = st.second ? cur : o.w;
+          const uint32_t *up_ex = st.second ? cur_ex : o.ex;
+          const size_t nup = st.second ? cur_n : o.n;
+          const uint32_t *dn = st.second ? o.w : cur;
+          const uint32_t *dn_ex = st.second ? o.ex : cur_ex;
+          const size_t ndn = st.second ? o.n : cur_n;
+          int rc = mbx_wah_and_no1(ctx, up, up_ex, nup, dn, dn_ex, ndn, wres[buf]);
+          if (rc != MBX_OK) return rc;
+          cur = wres[buf];
+          cur_ex = dn_ex;   // the result keeps down's word structure
+          cur_n = ndn;
+          buf ^= 1;
+        }
+        int rc = mbx_wah_scan(ctx, cur, cur_n, wex);
+        if (rc != MBX_OK) return rc;
+        return mbx_wah_decompress(ctx, cur, wex, cur_n, wgroups, ngroups);
+      });
+    }
+    wah_ms.push_back(ms);
+    // the result's groups, checked against the Perfect program's bitmap in
+    // block 2
+    wah_res.emplace_back(ok ? ngroups : 0);
+    if (ok) mbx_memcpy_d2h(ctx, wah_res.back().data(), wgroups, 4 * ngroups);
+    for (Wah &x : w) wah_free(ctx, x);
+  }
+  std::printf("Case\tWAH\tRTScan\n");
+  for (size_t i = 0; i < wah_ms.size(); ++i)
+    std::printf("%s\t%.4f\tnan\n", kFlights[i].name, wah_ms[i]);
 
   // ---------------------------------------------- block 1: index creation
+  // My: create_bin of the column's bins; WAH: each bin's groups
+  // (mbx_wah_select) compressed, one bin at a time, and the compressed size
+  // (ssb/bmpidx-create.cu:56-116)
   std::vector<uint *> bins[2][NSRC];
-  std::printf("Case\tWAH\tRTScan\n");
-  for (const Flight &f : kFlights) std::printf("%s\tnan\tnan\n", f.name);
   std::printf("\nColumn\tMy(ms)\tWAH(ms)\tMy(MB)\tWAH(MB)\n");
   const char *kSet[2] = {"Sparse", "Dense"};
   for (int d = 0; d < 2; ++d) {
-    double tot_ms = 0, tot_mb = 0;
+    double tot_ms = 0, tot_mb = 0, tot_wms = 0, tot_wmb = 0;
     for (int s = 0; s < NSRC; ++s) {
       const std::vector<size_t> &b = kBounds[d][s];
       const size_t nb = b.size() - 1;
@@ -629,11 +764,28 @@ int main(int argc, char **argv) {
         return 2;
       }
       const double mb = (double)bytes * nb / (1 << 20);
+      const double w0 = now_ms();
+      size_t wah_words = 0;
+      for (size_t j = 0; j < nb; ++j) {
+        size_t k = 0;
+        if (mbx_wah_select(ctx, &r, t.nrows, lo[j], hi[j], wgroups) != MBX_OK ||
+            mbx_wah_compress(ctx, wgroups, ngroups, wres[0], &k) != MBX_OK) {
+          std::fprintf(stderr, "wah %s: %s\n", kSrcName[s], mbx_last_error());
+          return 2;
+        }
+        wah_words += k;
+      }
+      mbx_sync(ctx);
+      const double wms = now_ms() - w0, wmb = 4.0 * wah_words / (1 << 20);
       tot_ms += ms;
       tot_mb += mb;
-      std::printf("%s_%s\t%.2f\t0\t%.2f\t0\n", kSrcName[s], kSet[d], ms, mb);
+      tot_wms += wms;
+      tot_wmb += wmb;
+      std::printf("%s_%s\t%.2f\t%.2f\t%.2f\t%.2f\n", kSrcName[s], kSet[d], ms,
+                  wms, mb, wmb);
     }
-    std::printf("Total_%s\t%.2f\t0\t%.2f\t0\n", kSet[d], tot_ms, tot_mb);
+    std::printf("Total_%s\t%.2f\t%.2f\t%.2f\t%.2f\n", kSet[d], tot_ms, tot_wms,
+                tot_mb, tot_wmb);
   }
 
   // ------------------------------------------------------ block 2: SSB
@@ -650,19 +802,6 @@ int main(int argc, char **argv) {
   }
   int bad = 0;
   std::vector<int64_t> h_want(1024), h_got(1024);
-  auto timed = [&](auto &&fn) {
-    float ms = 0, sum = 0;
-    for (int r = 0; r < reps; ++r) {
-      mbx_timer_start(ctx, tm);
-      if (fn() != MBX_OK) {
-        std::fprintf(stderr, "launch: %s\n", mbx_last_error());
-        std::exit(2);
-      }
-      mbx_timer_stop(ctx, tm, &ms);
-      sum += ms;
-    }
-    return sum / reps;
-  };
   // Basefus: the base_fuse ablation (method 1, the program's registers in
   // device memory); NfuStg1: the first materialise stage with its registers
   // in device memory (the reference's _stg1 with idxbuf).  Both under the
@@ -852,6 +991,19 @@ int main(int argc, char **argv) {
       const float s1 = timed([&] {
         return mbx_prog_eval(ctx, &ap, poss, p.is_nochk() ? nullptr : unc);
       });
+      if (kind == 0 && !wah_res[&f - kFlights].empty()) {
+        // the WAH baseline's result (block 0) == this program's bitmap
+        std::vector<uint32_t> gh(ngroups);
+        if (mbx_wah_groups(ctx, poss, t.nrows, wgroups) != MBX_OK ||
+            mbx_memcpy_d2h(ctx, gh.data(), wgroups, 4 * ngroups) != MBX_OK) {
+          std::fprintf(stderr, "wah check: %s\n", mbx_last_error());
+          return 2;
+        }
+        if (gh != wah_res[&f - kFlights]) {
+          std::fprintf(stderr, "%s: WAH result differs from the bitmap\n", f.name);
+          ++bad;
+        }
+      }
       const float s2 = timed([&] {
         return mbx_compact(ctx, poss, p.is_nochk() ? nullptr : unc, t.nrows,
                            rows, count);

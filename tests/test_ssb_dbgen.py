@@ -251,11 +251,24 @@ def test_demo_driver_report(ssb_host, tmp_path_factory):
     assert r.returncode == 0, r.stderr
     blocks = r.stdout.strip("\n").split("\n\n")
     assert len(blocks) == 5
-    # block 0: WAH and RTScan are not built -- the values are nan
+    # block 0: each flight's WAH index-only time (its predicates' WAH
+    # bitmaps ANDed and the result decompressed, checked by the driver
+    # against the Perfect program's bitmap); RTScan has no counterpart: nan
     wah = blocks[0].splitlines()
     assert wah[0].split("\t") == ["Case", "WAH", "RTScan"]
     assert [row.split("\t")[0] for row in wah[1:]] == [q.name for q in ssbq.FLIGHTS]
-    assert all(row.split("\t")[1:] == ["nan", "nan"] for row in wah[1:])
+    assert all(float(row.split("\t")[1]) > 0 and row.split("\t")[2] == "nan"
+               for row in wah[1:])
+    # block 1: every bin set's creation time and size, for the library's
+    # bitmaps and for their WAH compression
+    idx = blocks[1].splitlines()
+    assert idx[0].split("\t") == ["Column", "My(ms)", "WAH(ms)", "My(MB)", "WAH(MB)"]
+    assert len(idx) == 1 + 2 * 7
+    for row in idx[1:]:
+        f = row.split("\t")
+        assert len(f) == 5 and all(float(x) > 0 for x in f[1:]), row
+        # compressed bitmaps are never larger than the plain ones here
+        assert float(f[4]) <= float(f[3]) * 1.05, row
     # every SSB row's sums (scan, fused, materialise) == the straight loops
     want = {q.name: _oracle(ssb_host, q) for q in ssbq.FLIGHTS}
     seen = set()
