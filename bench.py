@@ -402,6 +402,151 @@ def run_serving(dev, nreq=1000, rates=(2000.0, 20000.0), budget=256 << 20,
     }
 
 
+def run_wah(dev, sch, specs, qs, fused_us, reps=20):
+    """The WAH baseline (merle/wahGpu*.cu, ssb/wah-demo.cu) on the config-2
+    queries: each predicate's WAH bitmap (groups of the predicate through
+    its foreign key, compressed, scanned; untimed), then the index-only step
+    the reference's report times -- the predicates ANDed left to right (the
+    operand with more words as the AND's down side), the result scanned and
+    decompressed -- per query (median of `reps`).  A
+    negated clause (d_year != 6) is its complement range.  The decompressed
+    result is checked against the groups of the fused program's bitmap.
+    The roofline is the decompression kernel's (the largest writer), its
+    launches back to back under one event pair."""
+    from merle_amd import abi, configs, spec as S
+    from merle_amd.host import Timer, check
+    n = sch.nrows
+    ng = int(dev.lib.mbx_wah_groups_of(n))
+    groups = dev.alloc(4 * ng, np.uint32, ng)
+    res = [dev.alloc(4 * ng, np.uint32, ng) for _ in range(2)]
+    rex = dev.alloc(4 * (ng + 1), np.uint32, ng + 1)
+    tmp = dev.alloc(4 * ng, np.uint32, ng)
+    nw = ((n + 63) // 64) * 2
+    poss = dev.alloc(4 * nw, np.uint32, nw)
+
+    def wah_of(src, lo, hi):
+        ref = S.colref(sch, src)
+        check(dev.lib.mbx_wah_select(dev.ctx, C.byref(ref), n, lo, hi, groups.ptr))
+        k = C.c_size_t()
+        check(dev.lib.mbx_wah_compress(dev.ctx, groups.ptr, ng, tmp.ptr, C.byref(k)))
+        w = dev.alloc(4 * max(1, k.value), np.uint32, k.value)
+        ex = dev.alloc(4 * (k.value + 1), np.uint32, k.value + 1)
+        if k.value:
+            check(dev.lib.mbx_memcpy_d2d(dev.ctx, w.ptr, tmp.ptr, 4 * k.value))
+        check(dev.lib.mbx_wah_scan(dev.ctx, w.ptr, k.value, ex.ptr))
+        lf = C.c_uint64()
+        check(dev.lib.mbx_wah_longest_1fill(dev.ctx, w.ptr, k.value, C.byref(lf)))
+        return w, ex, k.value, lf.value
+
+    def ranges(spec):
+        neg, loaded = set(), {}
+        for op, d, c in spec.instrs:
+            if op == abi.ORM:
+                loaded[d] = c
+            elif op == abi.NOT:
+                neg.add(loaded[d])
+        out = []
+        for i, p in enumerate(spec.preds):
+            lo, hi = p.lo, p.hi
+            if i in neg:
+                card = configs.C2_ATTRS[p.src.col][2]
+                assert lo == 0 or hi == card, "complement is not one range"
+                lo, hi = (hi, card) if lo == 0 else (0, lo)
+            out.append((p.src, lo, hi))
+        return out
+
+    per_q, sizes, identical, steps = {}, {}, True, []
+    t = Timer(dev)
+    for s, (sp, prg, op, out) in zip(specs, qs):
+        ws = [wah_of(src, lo, hi) for src, lo, hi in ranges(s)]
+        sizes[s.name] = [w[2] for w in ws]
+        # each AND's result keeps its down operand's words: down is the
+        # operand with more words (its literals then find the up words
+        # covering them staged in LDS), unless it holds a long 1-fill
+        order = [(0, None)]
+        acc1, acc_n = ws[0][3], ws[0][2]
+        for i in range(1, len(ws)):
+            cand = [x for x in ((ws[i][2], True, ws[i][3]), (acc_n, False, acc1))
+                    if x[2] <= 1]
+            assert cand, "no operand can be the AND's down side"
+            down = max(cand)[1]
+            order.append((i, down))
+            acc1, acc_n = 1, (ws[i][2] if down else acc_n)
+
+        def step():
+            cur, cur_ex, cur_n, b = ws[0][0].ptr, ws[0][1].ptr, ws[0][2], 0
+            for i, down in order[1:]:
+                w, ex, k, _ = ws[i]
+                up, upx, nup = (cur, cur_ex, cur_n) if down else (w.ptr, ex.ptr, k)
+                dn, dnx, ndn = (w.ptr, ex.ptr, k) if down else (cur, cur_ex, cur_n)
+                check(dev.lib.mbx_wah_and_no1(dev.ctx, up, upx, nup, dn, dnx, ndn,
+                                              res[b].ptr))
+                cur, cur_ex, cur_n, b = res[b].ptr, dnx, ndn, b ^ 1
+            check(dev.lib.mbx_wah_scan(dev.ctx, cur, cur_n, rex.ptr))
+            check(dev.lib.mbx_wah_decompress(dev.ctx, cur, rex.ptr, cur_n,
+                                             groups.ptr, ng))
+            return cur, cur_n
+
+        cur, cur_n = step()
+        dev.sync()
+        got = groups.host()
+        check(dev.lib.mbx_prog_eval(dev.ctx, C.byref(prg), poss.ptr, None))
+        check(dev.lib.mbx_wah_groups(dev.ctx, poss.ptr, n, groups.ptr))
+        identical &= bool(np.array_equal(got, groups.host()))
+        times = []
+        for _ in range(reps):
+            t.start()
+            step()
+            times.append(t.stop() * 1e3)
+        per_q[s.name] = round(float(np.median(times)), 1)
+        # keep the result for the isolated decompression roofline
+        keep = dev.alloc(4 * max(1, cur_n), np.uint32, cur_n)
+        if cur_n:
+            check(dev.lib.mbx_memcpy_d2d(dev.ctx, keep.ptr, cur, 4 * cur_n))
+        kex = dev.alloc(4 * (cur_n + 1), np.uint32, cur_n + 1)
+        check(dev.lib.mbx_wah_scan(dev.ctx, keep.ptr, cur_n, kex.ptr))
+        steps.append((keep, kex, cur_n))
+        for w, ex, _, _ in ws:
+            w.free()
+            ex.free()
+    # decompression in isolation: every query's result back to back
+    dev.sync()
+    t.start()
+    for _ in range(reps):
+        for keep, kex, k in steps:
+            check(dev.lib.mbx_wah_decompress(dev.ctx, keep.ptr, kex.ptr, k,
+                                             groups.ptr, ng))
+    ms = t.stop()
+    t.close()
+    launch_us = ms * 1e3 / (reps * len(steps))
+    alg = sum(4 * (k + k + 1 + ng) for _, _, k in steps) / len(steps)
+    for keep, kex, _ in steps:
+        keep.free()
+        kex.free()
+    for a in [groups, tmp, rex, poss] + res:
+        a.free()
+    wah_sum = sum(per_q.values())
+    return {
+        "workload": f"the 12 config-2 queries over {n:,} rows: each "
+                    "predicate's WAH bitmap (31-row groups, compressed); timed: "
+                    "the index-only step (ANDs, count scan, decompression)",
+        "index_only_us": per_q,
+        "index_only_sum_us": round(wah_sum, 1),
+        "fused_query_sum_us": round(sum(fused_us), 1),
+        "fused_vs_wah_index_only": round(wah_sum / sum(fused_us), 2),
+        "wah_words": sizes,
+        "plain_words_per_bitmap": ng,
+        "identical_to_fused_bitmaps": identical,
+        "roofline_decompress": {
+            "bound": "hbm", "achieved": round(alg / launch_us / 1e3, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(alg / launch_us / 1e3 / HBM_PEAK_GBS, 4),
+            "alg_bytes_per_launch": int(alg), "launch_us": round(launch_us, 2),
+            "note": "4 B per WAH word and count-scan entry read, 4 B per group "
+                    "written; launches back to back under one event pair"},
+    }
+
+
 def run_learned(dev, nprog=20000, neval=200):
     """Config 4: runtimes of random programs' candidate plans (evaluation
     orders x fuse / eager / materialise) measured by the engine, the
@@ -667,6 +812,9 @@ def main():
         result["parity_on_cpu_sample"] = ok
         log(f"cpu {cb['value']} G rows/s on {cb['cores']} threads; parity {ok}")
     if not args.no_extras:
+        log("WAH baseline on the config-2 queries")
+        result["wah"] = run_wah(dev, sch, specs, qs, [q * 1e3 for q in qms])
+        log(f"  {result['wah']}")
         outbuf.free()
         for a in slabs + [c.dev for c in sch.cols.values()]:
             a.free()

@@ -664,9 +664,9 @@ int main(int argc, char **argv) {
   // ssb/wah-demo.cu:24-292: each flight's predicates (its exact ranges, as
   // the Perfect layout below) as WAH bitmaps, built and scanned untimed;
   // timed per rep, index only: the predicates ANDed left to right
-  // (mbx_wah_and_no1, the left operand as `up`, so each right operand must
-  // hold no long 1-fill -- else the two swap, and a flight where neither
-  // can be `down` prints nan), the result scanned and decompressed.
+  // (mbx_wah_and_no1; the down side, which must hold no long 1-fill, is
+  // the operand with more words, and a flight where neither operand can
+  // be down prints nan), the result scanned and decompressed.
   // RTScan (OptiX ray tracing) has no counterpart here: nan.
   const size_t ngroups = mbx_wah_groups_of(t.nrows);
   uint32_t *wgroups = nullptr, *wres[2] = {nullptr, nullptr}, *wex = nullptr;
@@ -689,15 +689,23 @@ int main(int argc, char **argv) {
       if (!wah_of(ctx, t.ref(pr[i].src), t.nrows, pr[i].lo, pr[i].hi, wgroups,
                   w[i]))
         return 2;
-    // operand order: acc = w[0]; each next operand is `down` unless it has
-    // a long 1-fill and the running result does not
+    // operand order: acc = w[0]; the result of each AND keeps its down
+    // operand's words, so down is the operand with more words (its
+    // literals then find the up words covering them staged in LDS) unless
+    // it holds a long 1-fill
     bool ok = true;
     std::vector<std::pair<int, bool>> steps;   // (operand, operand is down)
     uint64_t acc1 = w[0].longest1;
+    size_t acc_n = w[0].n;
     for (size_t i = 1; i < w.size() && ok; ++i) {
-      if (w[i].longest1 <= 1) steps.push_back({(int)i, true});
-      else if (acc1 <= 1) steps.push_back({(int)i, false});
-      else ok = false;
+      const bool next_ok = w[i].longest1 <= 1, acc_ok = acc1 <= 1;
+      if (!next_ok && !acc_ok) {
+        ok = false;
+        break;
+      }
+      const bool down = next_ok && (!acc_ok || w[i].n >= acc_n);
+      steps.push_back({(int)i, down});
+      if (down) acc_n = w[i].n;
       acc1 = 1;   // a result keeps down's words: no long 1-fill
     }
     float ms = NAN;

@@ -13,8 +13,8 @@
 // flags -> tile sums -> tile offsets -> ordered scatter -> fill lengths),
 // count scan (the same three-pass tile scan), AND in the down operand's
 // word structure, decompress, and the longest 1-fill.  The AND and the
-// decompression give each thread 16 consecutive items: one binary search
-// for its first item, then a forward walk.
+// decompression stage, per wave, the other operand's words that cover the
+// wave's groups in LDS and search them there.
 #include <algorithm>
 #include <cstring>
 
@@ -264,6 +264,75 @@ __device__ __forceinline__ uint32_t wah_find(const uint32_t *__restrict__ excl,
   return lo;
 }
 
+// A wave's items (down words of the AND, groups of the decompression) are
+// kWahSpan consecutive ones, lane l taking items l, l + 64, ... so that
+// every load and store is coalesced.  The words of the other operand that
+// cover the wave's groups (at most kWahSpan of them, else the AND looks
+// each one up in device memory) are staged in LDS: their group offsets and
+// their 31 rows; an item then costs a 10-step binary search in LDS.
+constexpr uint32_t kWahSpan = 1024;
+
+struct WahStage {
+  uint32_t ex[4][kWahSpan + 1];
+  uint32_t rows[4][kWahSpan];
+};
+
+// wah_find by the whole wave (g wave-uniform, below excl[n]): each round
+// the 64 lanes test 64 evenly spaced words of the range and the range
+// shrinks to the span between the last one at or before g and the next, so
+// a few million words take 4 dependent rounds instead of 22.
+__device__ __forceinline__ uint32_t wah_wave_find(
+    const uint32_t *__restrict__ excl, uint32_t n, uint32_t g) {
+  uint32_t lo = 0, hi = n;   // excl[lo] <= g < excl[hi]
+  while (hi - lo > 1) {
+    const uint32_t stride = (hi - lo + 63) / 64;
+    const uint32_t idx = lo + lane_id() * stride;
+    const bool ok = idx < hi && excl[idx] <= g;   // a prefix of the lanes
+    const uint64_t b = __ballot(ok);
+    const uint32_t last = 63u - (uint32_t)__builtin_clzll(b);   // lane 0 is ok
+    const uint32_t nlo = lo + last * stride;
+    hi = min(hi, nlo + stride);
+    lo = nlo;
+  }
+  return lo;
+}
+
+// Stages the words of (wah, excl, n) covering groups [g0, g1] (g1 < the
+// total; both wave-uniform); returns the first word, *cnt the number
+// staged, or false (and nothing staged) when more than kWahSpan words
+// cover them.
+__device__ __forceinline__ bool wah_stage(const uint32_t *__restrict__ wah,
+                                          const uint32_t *__restrict__ excl,
+                                          uint32_t n, uint32_t g0, uint32_t g1,
+                                          uint32_t *s_ex, uint32_t *s_rows,
+                                          uint32_t *ulo, uint32_t *cnt) {
+  const uint32_t lo = wah_wave_find(excl, n, g0);
+  const uint32_t hi = wah_wave_find(excl, n, g1);
+  const uint32_t c = hi - lo + 1;
+  *ulo = lo;
+  *cnt = c;
+  if (c > kWahSpan) return false;
+  for (uint32_t j = lane_id(); j < c; j += 64) {
+    s_ex[j] = excl[lo + j];
+    s_rows[j] = wah_rows(wah[lo + j]);
+  }
+  if (lane_id() == 0) s_ex[c] = excl[lo + c];   // lo + c = hi + 1 <= n
+  return true;
+}
+
+// The staged word covering group g: last j with s_ex[j] <= g.
+__device__ __forceinline__ uint32_t wah_staged(const uint32_t *s_ex,
+                                               const uint32_t *s_rows,
+                                               uint32_t cnt, uint32_t g) {
+  uint32_t lo = 0, hi = cnt;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (s_ex[mid] <= g) lo = mid;
+    else hi = mid;
+  }
+  return s_rows[lo];
+}
+
 // wahAndNo1 (wahGpuDirect.cu:8-34): down's word structure; a 0-fill stays,
 // a literal (a 1-fill of one group reads as one) is ANDed with the up word
 // covering its group (past up: 0), a zero result becomes a 0-fill of one.
@@ -271,19 +340,32 @@ __global__ __launch_bounds__(256) void wah_and_no1_kernel(
     const uint32_t *__restrict__ up, const uint32_t *__restrict__ up_excl,
     uint32_t nup, const uint32_t *__restrict__ down,
     const uint32_t *__restrict__ down_excl, uint32_t ndown, uint32_t *out) {
-  const uint32_t d0 = (blockIdx.x * 256 + threadIdx.x) * kWahItems;
-  if (d0 >= ndown) return;
-  uint32_t u = wah_find(up_excl, nup, down_excl[d0]);
-  for (uint32_t d = d0; d < d0 + kWahItems && d < ndown; ++d) {
+  __shared__ WahStage st;
+  const uint32_t wv = threadIdx.x >> 6, lane = lane_id();
+  const uint64_t d0 = ((uint64_t)blockIdx.x * 4 + wv) * kWahSpan;
+  if (d0 >= ndown) return;   // the whole wave
+  const uint32_t dl = (uint32_t)min<uint64_t>(d0 + kWahSpan - 1, ndown - 1);
+  const uint32_t utot = nup ? up_excl[nup] : 0u;
+  const uint32_t gf = down_excl[d0], gl = down_excl[dl];
+  uint32_t ulo = 0, cnt = 0;
+  bool staged = false;
+  if (gf < utot)
+    staged = wah_stage(up, up_excl, nup, gf, min(gl, utot - 1), st.ex[wv],
+                       st.rows[wv], &ulo, &cnt);
+  wave_lds_sync();
+  for (uint32_t k = 0; k < kWahSpan / 64; ++k) {
+    const uint64_t d = d0 + lane + 64u * k;
+    if (d >= ndown) break;
     uint32_t dw = down[d];
     if (dw == (kWahFill | kWahOne | 1u)) dw = kWahRows;
-    uint32_t o;
-    if (dw & kWahFill) {
-      o = dw;
-    } else {
+    uint32_t o = dw;
+    if (!(dw & kWahFill)) {
       const uint32_t g = down_excl[d];
-      while (u < nup && up_excl[u + 1] <= g) ++u;
-      o = dw & (u < nup ? wah_rows(up[u]) : 0u);
+      uint32_t r = 0;
+      if (g < utot)
+        r = staged ? wah_staged(st.ex[wv], st.rows[wv], cnt, g)
+                   : wah_rows(up[wah_find(up_excl, nup, g)]);
+      o = dw & r;
       if (o == 0) o = kWahFill | 1u;
     }
     out[d] = o;
@@ -291,19 +373,29 @@ __global__ __launch_bounds__(256) void wah_and_no1_kernel(
 }
 
 // wahDecomp (wahGpuCmprs.cu:37-51): groups of the words, zero past them.
+// The kWahSpan groups of a wave are covered by at most kWahSpan words.
 __global__ __launch_bounds__(256) void wah_decompress_kernel(
     const uint32_t *__restrict__ wah, const uint32_t *__restrict__ excl,
     uint32_t n, uint32_t *groups, uint32_t ng) {
-  const uint32_t g0 = (blockIdx.x * 256 + threadIdx.x) * kWahItems;
-  if (g0 >= ng) return;
+  __shared__ WahStage st;
+  const uint32_t wv = threadIdx.x >> 6, lane = lane_id();
+  const uint64_t g0 = ((uint64_t)blockIdx.x * 4 + wv) * kWahSpan;
+  if (g0 >= ng) return;   // the whole wave
   const uint32_t total = n ? excl[n] : 0u;
-  uint32_t u = wah_find(excl, n, g0);
-  for (uint32_t g = g0; g < g0 + kWahItems && g < ng; ++g) {
+  const uint32_t gl = (uint32_t)min<uint64_t>(g0 + kWahSpan - 1, ng - 1);
+  uint32_t ulo = 0, cnt = 0;
+  bool staged = false;   // false only for words that count no group
+  if (g0 < total)
+    staged = wah_stage(wah, excl, n, (uint32_t)g0, min(gl, total - 1),
+                       st.ex[wv], st.rows[wv], &ulo, &cnt);
+  wave_lds_sync();
+  for (uint32_t k = 0; k < kWahSpan / 64; ++k) {
+    const uint64_t g = g0 + lane + 64u * k;
+    if (g >= ng) break;
     uint32_t v = 0;
-    if (g < total) {
-      while (u + 1 < n && excl[u + 1] <= g) ++u;
-      v = wah_rows(wah[u]);
-    }
+    if (g < total)
+      v = staged ? wah_staged(st.ex[wv], st.rows[wv], cnt, (uint32_t)g)
+                 : wah_rows(wah[wah_find(excl, n, (uint32_t)g)]);
     groups[g] = v;
   }
 }
@@ -468,9 +560,9 @@ int mbx_wah_and_no1(mbx_ctx *ctx, const uint32_t *up, const uint32_t *up_excl,
     return fail(MBX_ERR_CAPACITY, "wah_and_no1: too many words");
   if (ndown == 0) return MBX_OK;
   hipLaunchKernelGGL(wah_and_no1_kernel,
-                     dim3(grid_of(ceil_div(ndown, kWahItems), 256)), dim3(256),
-                     0, ctx->stream, up, up_excl, (uint32_t)nup, down,
-                     down_excl, (uint32_t)ndown, out);
+                     dim3(grid_of(ceil_div(ndown, kWahSpan), 4)), dim3(256), 0,
+                     ctx->stream, up, up_excl, (uint32_t)nup, down, down_excl,
+                     (uint32_t)ndown, out);
   MBX_HIP(hipGetLastError());
   return MBX_OK;
 }
@@ -483,12 +575,12 @@ int mbx_wah_decompress(mbx_ctx *ctx, const uint32_t *wah, const uint32_t *excl,
   if (!st) st = wah_check(ctx, excl, "wah_decompress");
   if (!st && n) st = wah_check(ctx, wah, "wah_decompress");
   if (st) return st;
-  if (n > kWahCnt || ngroups > 0xffffffffull - kWahItems * 256ull)
+  if (n > kWahCnt || ngroups > 0xffffffffull)
     return fail(MBX_ERR_CAPACITY, "wah_decompress: too many words or groups");
   if (ngroups == 0) return MBX_OK;
   hipLaunchKernelGGL(wah_decompress_kernel,
-                     dim3(grid_of(ceil_div(ngroups, kWahItems), 256)), dim3(256),
-                     0, ctx->stream, wah, excl, (uint32_t)n, groups,
+                     dim3(grid_of(ceil_div(ngroups, kWahSpan), 4)), dim3(256), 0,
+                     ctx->stream, wah, excl, (uint32_t)n, groups,
                      (uint32_t)ngroups);
   MBX_HIP(hipGetLastError());
   return MBX_OK;
