@@ -598,6 +598,8 @@ def main():
                     help="run only a plain device copy of this many bytes")
     ap.add_argument("--batched-only", action="store_true",
                     help="run only the config-3 batched section (profiling)")
+    ap.add_argument("--wah-only", action="store_true",
+                    help="config 2 and the WAH section alone (profiling)")
     args = ap.parse_args()
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration, max(1, args.steps))
@@ -794,7 +796,7 @@ def main():
                           "records add a few microseconds to each)",
     }
 
-    if not args.no_cpu:
+    if not args.no_cpu and not args.wah_only:
         log("CPU baseline (oracle) on a bounded sample")
         cb, R, cres = cpu_baseline(dev, sch, specs)
         result["cpu_baseline"] = cb
@@ -811,10 +813,11 @@ def main():
             prg.factsz = n
         result["parity_on_cpu_sample"] = ok
         log(f"cpu {cb['value']} G rows/s on {cb['cores']} threads; parity {ok}")
-    if not args.no_extras:
+    if not args.no_extras or args.wah_only:
         log("WAH baseline on the config-2 queries")
         result["wah"] = run_wah(dev, sch, specs, qs, [q * 1e3 for q in qms])
         log(f"  {result['wah']}")
+    if not args.no_extras and not args.wah_only:
         outbuf.free()
         for a in slabs + [c.dev for c in sch.cols.values()]:
             a.free()

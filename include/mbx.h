@@ -260,8 +260,8 @@ int mbx_wah_groups(mbx_ctx *ctx, const uint32_t *bitmap, uint32_t nrows,
 int mbx_wah_compress(mbx_ctx *ctx, const uint32_t *groups, size_t ngroups,
                      uint32_t *out, size_t *nout);
 /* wahCntExcScan (wahGpuCmprs.cu:6-14): excl[i] = groups before word i,
- * excl[n] = all groups (n + 1 entries).  Waits for its kernels (pool
- * scratch). */
+ * excl[n] = all groups (n + 1 entries).  Its tile sums live in pool memory
+ * kept per stream of the context (grown on demand, after a wait). */
 int mbx_wah_scan(mbx_ctx *ctx, const uint32_t *wah, size_t n, uint32_t *excl);
 /* wahAndNo1 (wahGpuDirect.cu:8-34): up AND down in down's word structure
  * (out has ndown words): a 0-fill of down stays, a literal (or a 1-fill of

@@ -88,6 +88,9 @@ struct mbx_ctx {
   std::map<hipStream_t, std::pair<unsigned long long *, uint32_t *>> spreads;
   // mbx_stream_join's event per joined stream (guarded by spread_mu)
   std::map<hipStream_t, hipEvent_t> join_events;
+  // the WAH count scan's tile sums per stream: (pool memory, bytes), grown
+  // on demand (guarded by spread_mu)
+  std::map<hipStream_t, std::pair<void *, size_t>> wah_tiles;
 };
 
 struct mbx_timer {

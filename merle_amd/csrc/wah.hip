@@ -17,6 +17,7 @@
 // wave's groups in LDS and search them there.
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 
 #include "devfuncs.hpp"
 #include "mbx_internal.hpp"
@@ -65,6 +66,14 @@ __device__ __forceinline__ uint32_t wah_block_scan(uint32_t x, uint32_t *s_w,
 // ------------------------------------------------------------ groups
 // Wave w covers rows [1984w, 1984w + 1984): 31 ballots of 64 rows, then
 // lane g holds group 64w + g = bits 31g..31g+30 of the 1984 ballot bits.
+template <typename T>
+__device__ __forceinline__ uint64_t wah_elem(const void *p, uint64_t i) {
+  return static_cast<const T *>(p)[i];
+}
+
+// T: the element type of the column (a load per row with no branch on the
+// width, so the 31 loads of a lane are in flight together).
+template <typename T>
 __global__ __launch_bounds__(256) void wah_select_kernel(DevSrc src,
                                                          uint32_t nrows,
                                                          uint64_t lo,
@@ -93,14 +102,14 @@ __global__ __launch_bounds__(256) void wah_select_kernel(DevSrc src,
 #pragma unroll
     for (int b = 0; b < 31; ++b) {
       ok[b] = k[b] < src.dimsz;
-      v[b] = ld_elem(src.data, src.eb, ok[b] ? k[b] : 0u, false);
+      v[b] = wah_elem<T>(src.data, ok[b] ? k[b] : 0u);
     }
   } else {
 #pragma unroll
     for (int b = 0; b < 31; ++b) {
       const uint64_t row = r0 + 64u * b + lane;
       ok[b] = row < nrows;
-      v[b] = ld_elem(src.data, src.eb, ok[b] ? row : 0u, false);
+      v[b] = wah_elem<T>(src.data, ok[b] ? row : 0u);
     }
   }
 #pragma unroll
@@ -150,11 +159,13 @@ __device__ __forceinline__ bool wah_keep(const uint32_t *groups, uint32_t g,
 __global__ __launch_bounds__(256) void wah_keep_count_kernel(
     const uint32_t *__restrict__ groups, uint32_t ng, uint32_t *tile_sums) {
   __shared__ uint32_t s_w[4];
-  const uint32_t g0 = blockIdx.x * kWahTile + threadIdx.x * kWahItems;
+  const uint32_t base = blockIdx.x * kWahTile;
   uint32_t c = 0;
 #pragma unroll
-  for (uint32_t i = 0; i < kWahItems; ++i)
-    c += (g0 + i < ng && wah_keep(groups, g0 + i, ng)) ? 1u : 0u;
+  for (uint32_t k = 0; k < kWahItems; ++k) {
+    const uint32_t g = base + k * 256 + threadIdx.x;
+    c += (g < ng && wah_keep(groups, g, ng)) ? 1u : 0u;
+  }
   uint32_t total;
   (void)wah_block_scan(c, s_w, &total);
   if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
@@ -214,14 +225,22 @@ __global__ __launch_bounds__(256) void wah_fill_len_kernel(
 }
 
 // ------------------------------------------------------- count scan
+// Tile sums read the words coalesced (word base + k * 256 + thread);
+// the scan itself stages its tile in LDS the same way, then thread t scans
+// its 16 consecutive words there.  LDS index j + j / 16 keeps both access
+// patterns free of bank conflicts (thread t's words start 17t apart).
+__device__ __forceinline__ uint32_t wah_pad(uint32_t j) { return j + (j >> 4); }
+
 __global__ __launch_bounds__(256) void wah_count_sum_kernel(
     const uint32_t *__restrict__ wah, uint32_t n, uint32_t *tile_sums) {
   __shared__ uint32_t s_w[4];
-  const uint32_t i0 = blockIdx.x * kWahTile + threadIdx.x * kWahItems;
+  const uint32_t base = blockIdx.x * kWahTile;
   uint32_t c = 0;
 #pragma unroll
-  for (uint32_t i = 0; i < kWahItems; ++i)
-    c += i0 + i < n ? wah_count(wah[i0 + i]) : 0u;
+  for (uint32_t k = 0; k < kWahItems; ++k) {
+    const uint32_t i = base + k * 256 + threadIdx.x;
+    c += i < n ? wah_count(wah[i]) : 0u;
+  }
   uint32_t total;
   (void)wah_block_scan(c, s_w, &total);
   if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
@@ -231,22 +250,32 @@ __global__ __launch_bounds__(256) void wah_count_scan_kernel(
     const uint32_t *__restrict__ wah, uint32_t n,
     const uint32_t *__restrict__ tile_offs, uint32_t *excl) {
   __shared__ uint32_t s_w[4];
-  const uint32_t i0 = blockIdx.x * kWahTile + threadIdx.x * kWahItems;
-  uint32_t c[kWahItems], s = 0;
+  __shared__ uint32_t s_c[kWahTile + kWahTile / 16];
+  const uint32_t base = blockIdx.x * kWahTile, t = threadIdx.x;
 #pragma unroll
-  for (uint32_t i = 0; i < kWahItems; ++i) {
-    c[i] = i0 + i < n ? wah_count(wah[i0 + i]) : 0u;
-    s += c[i];
+  for (uint32_t k = 0; k < kWahItems; ++k) {
+    const uint32_t j = k * 256 + t, i = base + j;
+    s_c[wah_pad(j)] = i < n ? wah_count(wah[i]) : 0u;
   }
+  __syncthreads();
+  uint32_t sum = 0;
+#pragma unroll
+  for (uint32_t i = 0; i < kWahItems; ++i) sum += s_c[wah_pad(t * kWahItems + i)];
   uint32_t total;
-  uint32_t acc = tile_offs[blockIdx.x] + wah_block_scan(s, s_w, &total);
+  uint32_t acc = tile_offs[blockIdx.x] + wah_block_scan(sum, s_w, &total);
 #pragma unroll
   for (uint32_t i = 0; i < kWahItems; ++i) {
-    if (i0 + i < n) excl[i0 + i] = acc;
-    acc += c[i];
+    const uint32_t p = wah_pad(t * kWahItems + i), v = s_c[p];
+    s_c[p] = acc;
+    acc += v;
   }
-  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 255)
-    excl[n] = tile_offs[gridDim.x];
+  __syncthreads();
+#pragma unroll
+  for (uint32_t k = 0; k < kWahItems; ++k) {
+    const uint32_t j = k * 256 + t, i = base + j;
+    if (i < n) excl[i] = s_c[wah_pad(j)];
+  }
+  if (blockIdx.x == gridDim.x - 1 && t == 255) excl[n] = tile_offs[gridDim.x];
 }
 
 // ------------------------------------------------------ AND, decompress
@@ -267,10 +296,13 @@ __device__ __forceinline__ uint32_t wah_find(const uint32_t *__restrict__ excl,
 // A wave's items (down words of the AND, groups of the decompression) are
 // kWahSpan consecutive ones, lane l taking items l, l + 64, ... so that
 // every load and store is coalesced.  The words of the other operand that
-// cover the wave's groups (at most kWahSpan of them, else the AND looks
-// each one up in device memory) are staged in LDS: their group offsets and
-// their 31 rows; an item then costs a 10-step binary search in LDS.
+// cover the wave's groups are staged in LDS, kWahSpan words at a time
+// (their group offsets and their 31 rows), from the word covering the
+// wave's first group on; an item then costs a 10-step binary search in
+// LDS.  The items' groups increase with the item, so an item the staged
+// words do not reach is reached by a later chunk of words.
 constexpr uint32_t kWahSpan = 1024;
+constexpr uint32_t kWahPerLane = kWahSpan / 64;
 
 struct WahStage {
   uint32_t ex[4][kWahSpan + 1];
@@ -297,30 +329,40 @@ __device__ __forceinline__ uint32_t wah_wave_find(
   return lo;
 }
 
-// Stages the words of (wah, excl, n) covering groups [g0, g1] (g1 < the
-// total; both wave-uniform); returns the first word, *cnt the number
-// staged, or false (and nothing staged) when more than kWahSpan words
-// cover them.
-__device__ __forceinline__ bool wah_stage(const uint32_t *__restrict__ wah,
-                                          const uint32_t *__restrict__ excl,
-                                          uint32_t n, uint32_t g0, uint32_t g1,
-                                          uint32_t *s_ex, uint32_t *s_rows,
-                                          uint32_t *ulo, uint32_t *cnt) {
-  const uint32_t lo = wah_wave_find(excl, n, g0);
-  const uint32_t hi = wah_wave_find(excl, n, g1);
-  const uint32_t c = hi - lo + 1;
-  *ulo = lo;
+// Stages words [w0, w0 + c), c = min(n - w0, kWahSpan), of (wah, excl, n):
+// group offsets, rows, and the offset after the last (returned).  Every
+// load is in flight before the first store (a lane past c reads word w0
+// again, dropped).
+__device__ __forceinline__ uint32_t wah_stage(const uint32_t *__restrict__ wah,
+                                              const uint32_t *__restrict__ excl,
+                                              uint32_t n, uint32_t w0,
+                                              uint32_t *s_ex, uint32_t *s_rows,
+                                              uint32_t *cnt) {
+  const uint32_t c = min(n - w0, kWahSpan);
   *cnt = c;
-  if (c > kWahSpan) return false;
-  for (uint32_t j = lane_id(); j < c; j += 64) {
-    s_ex[j] = excl[lo + j];
-    s_rows[j] = wah_rows(wah[lo + j]);
+  uint32_t e[kWahPerLane], r[kWahPerLane];
+#pragma unroll
+  for (uint32_t k = 0; k < kWahPerLane; ++k) {
+    const uint32_t j = lane_id() + 64u * k, at = w0 + (j < c ? j : 0u);
+    e[k] = excl[at];
+    r[k] = wah[at];
   }
-  if (lane_id() == 0) s_ex[c] = excl[lo + c];   // lo + c = hi + 1 <= n
-  return true;
+  const uint32_t end = excl[w0 + c];   // w0 + c <= n
+  wave_lds_sync();   // the previous chunk's searches are done
+#pragma unroll
+  for (uint32_t k = 0; k < kWahPerLane; ++k) {
+    const uint32_t j = lane_id() + 64u * k;
+    if (j < c) {
+      s_ex[j] = e[k];
+      s_rows[j] = wah_rows(r[k]);
+    }
+  }
+  if (lane_id() == 0) s_ex[c] = end;
+  wave_lds_sync();
+  return end;
 }
 
-// The staged word covering group g: last j with s_ex[j] <= g.
+// The staged word covering group g (s_ex[0] <= g < s_ex[cnt]).
 __device__ __forceinline__ uint32_t wah_staged(const uint32_t *s_ex,
                                                const uint32_t *s_rows,
                                                uint32_t cnt, uint32_t g) {
@@ -336,6 +378,7 @@ __device__ __forceinline__ uint32_t wah_staged(const uint32_t *s_ex,
 // wahAndNo1 (wahGpuDirect.cu:8-34): down's word structure; a 0-fill stays,
 // a literal (a 1-fill of one group reads as one) is ANDed with the up word
 // covering its group (past up: 0), a zero result becomes a 0-fill of one.
+// A word is stored as soon as it is known.
 __global__ __launch_bounds__(256) void wah_and_no1_kernel(
     const uint32_t *__restrict__ up, const uint32_t *__restrict__ up_excl,
     uint32_t nup, const uint32_t *__restrict__ down,
@@ -344,36 +387,57 @@ __global__ __launch_bounds__(256) void wah_and_no1_kernel(
   const uint32_t wv = threadIdx.x >> 6, lane = lane_id();
   const uint64_t d0 = ((uint64_t)blockIdx.x * 4 + wv) * kWahSpan;
   if (d0 >= ndown) return;   // the whole wave
-  const uint32_t dl = (uint32_t)min<uint64_t>(d0 + kWahSpan - 1, ndown - 1);
   const uint32_t utot = nup ? up_excl[nup] : 0u;
-  const uint32_t gf = down_excl[d0], gl = down_excl[dl];
-  uint32_t ulo = 0, cnt = 0;
-  bool staged = false;
-  if (gf < utot)
-    staged = wah_stage(up, up_excl, nup, gf, min(gl, utot - 1), st.ex[wv],
-                       st.rows[wv], &ulo, &cnt);
-  wave_lds_sync();
-  for (uint32_t k = 0; k < kWahSpan / 64; ++k) {
+  // the wave's down words and their groups, every load in flight at once
+  // (a lane past the end reads word d0 again, dropped)
+  uint32_t dwv[kWahPerLane], gv[kWahPerLane];
+#pragma unroll
+  for (uint32_t k = 0; k < kWahPerLane; ++k) {
+    const uint64_t d = d0 + lane + 64u * k;
+    const uint64_t at = d < ndown ? d : d0;
+    const uint32_t dw = down[at];
+    gv[k] = down_excl[at];
+    dwv[k] = dw == (kWahFill | kWahOne | 1u) ? kWahRows : dw;
+  }
+  // the up words from the one covering the wave's first group
+  const uint32_t g_first = __shfl(gv[0], 0, 64);   // item 0 of lane 0
+  uint32_t w0 = 0, cnt = 0, end = 0, pend = 0;
+  if (g_first < utot) {
+    w0 = wah_wave_find(up_excl, nup, g_first);
+    end = wah_stage(up, up_excl, nup, w0, st.ex[wv], st.rows[wv], &cnt);
+  }
+  for (uint32_t k = 0; k < kWahPerLane; ++k) {
     const uint64_t d = d0 + lane + 64u * k;
     if (d >= ndown) break;
-    uint32_t dw = down[d];
-    if (dw == (kWahFill | kWahOne | 1u)) dw = kWahRows;
+    const uint32_t dw = dwv[k], g = gv[k];
     uint32_t o = dw;
     if (!(dw & kWahFill)) {
-      const uint32_t g = down_excl[d];
-      uint32_t r = 0;
-      if (g < utot)
-        r = staged ? wah_staged(st.ex[wv], st.rows[wv], cnt, g)
-                   : wah_rows(up[wah_find(up_excl, nup, g)]);
-      o = dw & r;
-      if (o == 0) o = kWahFill | 1u;
+      if (g < utot && g >= end) {
+        pend |= 1u << k;
+        continue;
+      }
+      const uint32_t v =
+          g < utot ? dw & wah_staged(st.ex[wv], st.rows[wv], cnt, g) : 0u;
+      o = v ? v : (kWahFill | 1u);
     }
     out[d] = o;
+  }
+  while (__ballot(pend != 0)) {
+    w0 += cnt;
+    end = wah_stage(up, up_excl, nup, w0, st.ex[wv], st.rows[wv], &cnt);
+    for (uint32_t k = 0; k < kWahPerLane; ++k) {
+      if (((pend >> k) & 1u) && gv[k] < end) {
+        const uint32_t v = dwv[k] & wah_staged(st.ex[wv], st.rows[wv], cnt, gv[k]);
+        out[d0 + lane + 64u * k] = v ? v : (kWahFill | 1u);
+        pend &= ~(1u << k);
+      }
+    }
   }
 }
 
 // wahDecomp (wahGpuCmprs.cu:37-51): groups of the words, zero past them.
-// The kWahSpan groups of a wave are covered by at most kWahSpan words.
+// A group is stored as soon as a chunk reaches it (the first chunk reaches
+// every group unless some word counts no group).
 __global__ __launch_bounds__(256) void wah_decompress_kernel(
     const uint32_t *__restrict__ wah, const uint32_t *__restrict__ excl,
     uint32_t n, uint32_t *groups, uint32_t ng) {
@@ -382,21 +446,31 @@ __global__ __launch_bounds__(256) void wah_decompress_kernel(
   const uint64_t g0 = ((uint64_t)blockIdx.x * 4 + wv) * kWahSpan;
   if (g0 >= ng) return;   // the whole wave
   const uint32_t total = n ? excl[n] : 0u;
-  const uint32_t gl = (uint32_t)min<uint64_t>(g0 + kWahSpan - 1, ng - 1);
-  uint32_t ulo = 0, cnt = 0;
-  bool staged = false;   // false only for words that count no group
-  if (g0 < total)
-    staged = wah_stage(wah, excl, n, (uint32_t)g0, min(gl, total - 1),
-                       st.ex[wv], st.rows[wv], &ulo, &cnt);
-  wave_lds_sync();
-  for (uint32_t k = 0; k < kWahSpan / 64; ++k) {
+  uint32_t w0 = 0, cnt = 0, end = 0, pend = 0;
+  if (g0 < total) {
+    w0 = wah_wave_find(excl, n, (uint32_t)g0);
+    end = wah_stage(wah, excl, n, w0, st.ex[wv], st.rows[wv], &cnt);
+  }
+  for (uint32_t k = 0; k < kWahPerLane; ++k) {
     const uint64_t g = g0 + lane + 64u * k;
     if (g >= ng) break;
-    uint32_t v = 0;
-    if (g < total)
-      v = staged ? wah_staged(st.ex[wv], st.rows[wv], cnt, (uint32_t)g)
-                 : wah_rows(wah[wah_find(excl, n, (uint32_t)g)]);
-    groups[g] = v;
+    if (g < total && g >= end) {
+      pend |= 1u << k;
+      continue;
+    }
+    groups[g] = g < total ? wah_staged(st.ex[wv], st.rows[wv], cnt, (uint32_t)g)
+                          : 0u;
+  }
+  while (__ballot(pend != 0)) {
+    w0 += cnt;
+    end = wah_stage(wah, excl, n, w0, st.ex[wv], st.rows[wv], &cnt);
+    for (uint32_t k = 0; k < kWahPerLane; ++k) {
+      const uint32_t g = (uint32_t)(g0 + lane + 64u * k);
+      if (((pend >> k) & 1u) && g < end) {
+        groups[g] = wah_staged(st.ex[wv], st.rows[wv], cnt, g);
+        pend &= ~(1u << k);
+      }
+    }
   }
 }
 
@@ -465,8 +539,25 @@ int mbx_wah_select(mbx_ctx *ctx, const mbx_colref *src, uint32_t nrows,
   d.fk = src->fk;
   d.dimsz = src->dimsz;
   d.eb = src->elem_bytes;
-  hipLaunchKernelGGL(wah_select_kernel, dim3(grid_of(ng, 256)), dim3(256), 0,
-                     ctx->stream, d, nrows, lo, hi, groups, ng);
+  const dim3 grid(grid_of(ng, 256)), block(256);
+  switch (d.eb) {
+  case 1:
+    hipLaunchKernelGGL(wah_select_kernel<uint8_t>, grid, block, 0, ctx->stream,
+                       d, nrows, lo, hi, groups, ng);
+    break;
+  case 2:
+    hipLaunchKernelGGL(wah_select_kernel<uint16_t>, grid, block, 0, ctx->stream,
+                       d, nrows, lo, hi, groups, ng);
+    break;
+  case 4:
+    hipLaunchKernelGGL(wah_select_kernel<uint32_t>, grid, block, 0, ctx->stream,
+                       d, nrows, lo, hi, groups, ng);
+    break;
+  default:
+    hipLaunchKernelGGL(wah_select_kernel<uint64_t>, grid, block, 0, ctx->stream,
+                       d, nrows, lo, hi, groups, ng);
+    break;
+  }
   MBX_HIP(hipGetLastError());
   return MBX_OK;
 }
@@ -523,6 +614,28 @@ int mbx_wah_compress(mbx_ctx *ctx, const uint32_t *groups, size_t ngroups,
   return MBX_OK;
 }
 
+// The count scan's tile sums for the context's stream: kept per stream and
+// grown on demand (the old buffer is freed once the stream has drained), so
+// the scan neither allocates nor waits in steady state.
+static int wah_tiles_of(mbx_ctx *ctx, size_t bytes, uint32_t **out) {
+  std::lock_guard<std::mutex> g(ctx->spread_mu);
+  auto &slot = ctx->wah_tiles[ctx->stream];
+  if (slot.second < bytes) {
+    if (slot.first) {
+      MBX_HIP(hipStreamSynchronize(ctx->stream));
+      ctx->pool->free(slot.first);
+      slot = {nullptr, 0};
+    }
+    const size_t want = std::max<size_t>(bytes, 64 << 10);
+    void *p = nullptr;
+    int st = ctx->pool->alloc(want, &p);
+    if (st) return st;
+    slot = {p, want};
+  }
+  *out = static_cast<uint32_t *>(slot.first);
+  return MBX_OK;
+}
+
 int mbx_wah_scan(mbx_ctx *ctx, const uint32_t *wah, size_t n, uint32_t *excl) {
   if ((!wah && n) || !excl) return fail(MBX_ERR_INVALID, "wah_scan: null");
   int st = wah_check(ctx, excl, "wah_scan");
@@ -531,10 +644,9 @@ int mbx_wah_scan(mbx_ctx *ctx, const uint32_t *wah, size_t n, uint32_t *excl) {
   if (n > kWahCnt) return fail(MBX_ERR_CAPACITY, "wah_scan: too many words");
   if (n == 0) return launch_fill_u32(ctx->stream, excl, 0, 1);
   const uint32_t ntiles = (uint32_t)ceil_div(n, kWahTile);
-  Scratch s(ctx);
-  st = ctx->pool->alloc(((size_t)ntiles + 1) * 4, &s.p);
+  uint32_t *tiles = nullptr;
+  st = wah_tiles_of(ctx, ((size_t)ntiles + 1) * 4, &tiles);
   if (st) return st;
-  uint32_t *tiles = static_cast<uint32_t *>(s.p);
   hipLaunchKernelGGL(wah_count_sum_kernel, dim3(ntiles), dim3(256), 0,
                      ctx->stream, wah, (uint32_t)n, tiles);
   hipLaunchKernelGGL(wah_tiles_scan_kernel, dim3(1), dim3(256), 0, ctx->stream,
