@@ -188,26 +188,54 @@ __global__ __launch_bounds__(256) void wah_tiles_scan_kernel(
   if (threadIdx.x == 0) tile_sums[ntiles] = carry;
 }
 
+// LDS index of item j of a tile that is written coalesced (item k * 256 +
+// thread) and read by thread t as its 16 consecutive items: j + j / 16
+// keeps both patterns free of bank conflicts (thread t's items start 17t
+// apart).
+__device__ __forceinline__ uint32_t wah_pad(uint32_t j) { return j + (j >> 4); }
+
 // Kept groups in order: out[pos] = the literal or the fill's kind, and
-// ends[pos] = g + 1, the end of the run the word closes.
+// ends[pos] = g + 1, the end of the run the word closes.  The tile's
+// groups (and the next tile's first) are read coalesced into LDS; thread t
+// then takes its 16 consecutive groups there.
+
 __global__ __launch_bounds__(256) void wah_scatter_kernel(
     const uint32_t *__restrict__ groups, uint32_t ng,
     const uint32_t *__restrict__ tile_offs, uint32_t *out, uint32_t *ends) {
   __shared__ uint32_t s_w[4];
-  const uint32_t g0 = blockIdx.x * kWahTile + threadIdx.x * kWahItems;
+  __shared__ uint32_t s_g[kWahTile + kWahTile / 16 + 1];
+  const uint32_t base = blockIdx.x * kWahTile, t = threadIdx.x;
+#pragma unroll
+  for (uint32_t k = 0; k < kWahItems; ++k) {
+    const uint32_t j = k * 256 + t, g = base + j;
+    s_g[wah_pad(j)] = g < ng ? groups[g] & kWahRows : 0u;
+  }
+  if (t == 0)
+    s_g[wah_pad(kWahTile)] = base + kWahTile < ng ? groups[base + kWahTile] & kWahRows
+                                                  : 0u;
+  __syncthreads();
   uint32_t keep = 0;
 #pragma unroll
-  for (uint32_t i = 0; i < kWahItems; ++i)
-    if (g0 + i < ng && wah_keep(groups, g0 + i, ng)) keep |= 1u << i;
+  for (uint32_t i = 0; i < kWahItems; ++i) {
+    const uint32_t j = t * kWahItems + i, g = base + j;
+    if (g >= ng) break;
+    const uint32_t v = s_g[wah_pad(j)];
+    bool kp;
+    if (g + 1 == ng) kp = v != 0;
+    else if (v != 0 && v != kWahRows) kp = true;
+    else kp = s_g[wah_pad(j + 1)] != v;
+    if (kp) keep |= 1u << i;
+  }
   uint32_t total;
   uint32_t pos = tile_offs[blockIdx.x] +
                  wah_block_scan((uint32_t)__popc(keep), s_w, &total);
 #pragma unroll
   for (uint32_t i = 0; i < kWahItems; ++i) {
     if (!((keep >> i) & 1u)) continue;
-    const uint32_t v = groups[g0 + i] & kWahRows;
+    const uint32_t j = t * kWahItems + i;
+    const uint32_t v = s_g[wah_pad(j)];
     out[pos] = v == 0 ? kWahFill : v == kWahRows ? (kWahFill | kWahOne) : v;
-    ends[pos] = g0 + i + 1;
+    ends[pos] = base + j + 1;
     ++pos;
   }
 }
@@ -227,10 +255,7 @@ __global__ __launch_bounds__(256) void wah_fill_len_kernel(
 // ------------------------------------------------------- count scan
 // Tile sums read the words coalesced (word base + k * 256 + thread);
 // the scan itself stages its tile in LDS the same way, then thread t scans
-// its 16 consecutive words there.  LDS index j + j / 16 keeps both access
-// patterns free of bank conflicts (thread t's words start 17t apart).
-__device__ __forceinline__ uint32_t wah_pad(uint32_t j) { return j + (j >> 4); }
-
+// its 16 consecutive words there.
 __global__ __launch_bounds__(256) void wah_count_sum_kernel(
     const uint32_t *__restrict__ wah, uint32_t n, uint32_t *tile_sums) {
   __shared__ uint32_t s_w[4];
