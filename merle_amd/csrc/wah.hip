@@ -64,13 +64,13 @@ __device__ __forceinline__ uint32_t wah_block_scan(uint32_t x, uint32_t *s_w,
 }
 
 // ------------------------------------------------------------ groups
-// Wave w covers rows [1984w, 1984w + 1984): 31 ballots of 64 rows, then
-// lane g holds group 64w + g = bits 31g..31g+30 of the 1984 ballot bits.
 template <typename T>
 __device__ __forceinline__ uint64_t wah_elem(const void *p, uint64_t i) {
   return static_cast<const T *>(p)[i];
 }
 
+// Wave w covers rows [1984w, 1984w + 1984): 31 ballots of 64 rows, then
+// lane g holds group 64w + g = bits 31g..31g+30 of the 1984 ballot bits.
 // T: the element type of the column (a load per row with no branch on the
 // width, so the 31 loads of a lane are in flight together).
 template <typename T>
