@@ -400,6 +400,59 @@ __device__ __forceinline__ uint32_t wah_staged(const uint32_t *s_ex,
   return s_rows[lo];
 }
 
+// Inclusive max over the wave's lanes.
+__device__ __forceinline__ uint32_t wave_max_scan(uint32_t x) {
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (uint32_t d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(x, d, 64);
+    if (lane >= d) x = max(x, y);
+  }
+  return x;
+}
+
+// The staged word covering each of the kWahSpan groups [gb, gb + kWahSpan)
+// (the staged words start at or before gb and cover the groups asked for):
+// every staged word's index is written at its first group's offset, then a
+// running maximum over the offsets, 64 at a time across the lanes, gives
+// each offset its word.  Offset k * 64 + lane's word is left in own[k] and
+// in s_ex[k * 64 + lane] (s_ex is overwritten).  Independent LDS accesses
+// and shuffles instead of a 10-step dependent search per group.
+__device__ __forceinline__ void wah_owners(uint32_t *s_ex, uint32_t cnt,
+                                           uint32_t gb,
+                                           uint32_t (&own)[kWahPerLane]) {
+  const uint32_t lane = lane_id();
+  uint32_t start[kWahPerLane];
+#pragma unroll
+  for (uint32_t k = 0; k < kWahPerLane; ++k) {
+    const uint32_t j = lane + 64u * k;
+    start[k] = j < cnt ? s_ex[j] : 0xffffffffu;
+  }
+  wave_lds_sync();
+#pragma unroll
+  for (uint32_t k = 0; k < kWahPerLane; ++k) s_ex[lane + 64u * k] = 0;
+  wave_lds_sync();
+#pragma unroll
+  for (uint32_t k = 0; k < kWahPerLane; ++k) {
+    const uint32_t j = lane + 64u * k;
+    // word 0 covers gb and starts at or before it: offset 0 keeps 0; of
+    // words starting at one offset (all but the last count no group) the
+    // last is kept
+    if (j > 0 && j < cnt && start[k] >= gb && start[k] - gb < kWahSpan)
+      atomicMax(&s_ex[start[k] - gb], j);
+  }
+  wave_lds_sync();
+  uint32_t carry = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kWahPerLane; ++k) {
+    const uint32_t x = max(wave_max_scan(s_ex[64u * k + lane]), carry);
+    carry = __shfl(x, 63, 64);
+    own[k] = x;
+    s_ex[64u * k + lane] = x;
+  }
+  wave_lds_sync();
+}
+
 // wahAndNo1 (wahGpuDirect.cu:8-34): down's word structure; a 0-fill stays,
 // a literal (a 1-fill of one group reads as one) is ANDed with the up word
 // covering its group (past up: 0), a zero result becomes a 0-fill of one.
@@ -426,10 +479,31 @@ __global__ __launch_bounds__(256) void wah_and_no1_kernel(
   }
   // the up words from the one covering the wave's first group
   const uint32_t g_first = __shfl(gv[0], 0, 64);   // item 0 of lane 0
+  const uint64_t dl = min<uint64_t>(d0 + kWahSpan, ndown) - 1;
+  const uint32_t g_last = down_excl[dl];            // the wave's last group
   uint32_t w0 = 0, cnt = 0, end = 0, pend = 0;
   if (g_first < utot) {
     w0 = wah_wave_find(up_excl, nup, g_first);
     end = wah_stage(up, up_excl, nup, w0, st.ex[wv], st.rows[wv], &cnt);
+    if (g_last - g_first < kWahSpan && end > min(g_last, utot - 1)) {
+      // every literal's group is within kWahSpan of the first and covered:
+      // the covering words by offset, then one LDS read per literal
+      uint32_t own[kWahPerLane];
+      wah_owners(st.ex[wv], cnt, g_first, own);
+      for (uint32_t k = 0; k < kWahPerLane; ++k) {
+        const uint64_t d = d0 + lane + 64u * k;
+        if (d >= ndown) break;
+        const uint32_t dw = dwv[k], g = gv[k];
+        uint32_t o = dw;
+        if (!(dw & kWahFill)) {
+          const uint32_t v =
+              g < utot ? dw & st.rows[wv][st.ex[wv][g - g_first]] : 0u;
+          o = v ? v : (kWahFill | 1u);
+        }
+        out[d] = o;
+      }
+      return;
+    }
   }
   for (uint32_t k = 0; k < kWahPerLane; ++k) {
     const uint64_t d = d0 + lane + 64u * k;
@@ -475,6 +549,19 @@ __global__ __launch_bounds__(256) void wah_decompress_kernel(
   if (g0 < total) {
     w0 = wah_wave_find(excl, n, (uint32_t)g0);
     end = wah_stage(wah, excl, n, w0, st.ex[wv], st.rows[wv], &cnt);
+    const uint32_t glast = (uint32_t)min<uint64_t>(g0 + kWahSpan, total) - 1;
+    if (end > glast) {
+      // the staged words cover every group of the wave below the total:
+      // the covering word of each group by its offset
+      uint32_t own[kWahPerLane];
+      wah_owners(st.ex[wv], cnt, (uint32_t)g0, own);
+#pragma unroll
+      for (uint32_t k = 0; k < kWahPerLane; ++k) {
+        const uint64_t g = g0 + lane + 64u * k;
+        if (g < ng) groups[g] = g < total ? st.rows[wv][own[k]] : 0u;
+      }
+      return;
+    }
   }
   for (uint32_t k = 0; k < kWahPerLane; ++k) {
     const uint64_t g = g0 + lane + 64u * k;

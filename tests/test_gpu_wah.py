@@ -211,3 +211,27 @@ def test_wah_errors(dev):
     assert dev.lib.mbx_wah_and_no1(dev.ctx, None, g.ptr, 5, g.ptr, g.ptr, 5,
                                    g.ptr) == abi.ERR_INVALID
     g.free()
+
+
+def test_wah_empty_operands(dev):
+    """An empty up operand (an all-zero bitmap compresses to no words)
+    turns every down literal into a 0-fill of one group; an empty down
+    operand writes nothing; an empty WAH decompresses to zeros."""
+    down = np.array([O | 3, 0x1234, ONES, 0xc0000001, 0x55], np.uint32)
+    dd, sd = dev.upload(down), None
+    sd = _scan(dev, dd, len(down))
+    up = dev.alloc(64, np.uint32, 16)
+    su = _scan(dev, up, 0)
+    out = dev.alloc(4 * len(down), np.uint32, len(down))
+    check(dev.lib.mbx_wah_and_no1(dev.ctx, up.ptr, su.ptr, 0, dd.ptr, sd.ptr,
+                                  len(down), out.ptr))
+    assert list(out.host()) == [O | 3, O | 1, O | 1, O | 1, O | 1]
+    assert list(out.host()) == list(cpuref.wah_and_no1(np.zeros(0, np.uint32), down))
+    check(dev.lib.mbx_wah_and_no1(dev.ctx, dd.ptr, sd.ptr, len(down), up.ptr,
+                                  su.ptr, 0, out.ptr))
+    g = dev.alloc(4 * 100, np.uint32, 100)
+    check(dev.lib.mbx_fill_u32(dev.ctx, g.ptr, 7, 100))
+    check(dev.lib.mbx_wah_decompress(dev.ctx, up.ptr, su.ptr, 0, g.ptr, 100))
+    assert not g.host().any()
+    for x in (dd, sd, up, su, out, g):
+        x.free()
