@@ -271,9 +271,16 @@ __global__ __launch_bounds__(256) void wah_count_sum_kernel(
   if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
 }
 
+// Tile counts up to which the scan's tiles sum their predecessors' sums
+// themselves (at most 8 loads a thread) instead of a scan launch between.
+constexpr uint32_t kWahSelfPrefixTiles = 2048;
+
+// kSums: tiles holds the tiles' sums and each tile adds up those before it;
+// otherwise tiles holds their exclusive scan and the total at [gridDim.x].
+template <bool kSums>
 __global__ __launch_bounds__(256) void wah_count_scan_kernel(
     const uint32_t *__restrict__ wah, uint32_t n,
-    const uint32_t *__restrict__ tile_offs, uint32_t *excl) {
+    const uint32_t *__restrict__ tiles, uint32_t *excl) {
   __shared__ uint32_t s_w[4];
   __shared__ uint32_t s_c[kWahTile + kWahTile / 16];
   const uint32_t base = blockIdx.x * kWahTile, t = threadIdx.x;
@@ -282,12 +289,20 @@ __global__ __launch_bounds__(256) void wah_count_scan_kernel(
     const uint32_t j = k * 256 + t, i = base + j;
     s_c[wah_pad(j)] = i < n ? wah_count(wah[i]) : 0u;
   }
-  __syncthreads();
+  uint32_t off;
+  if (kSums) {
+    uint32_t p = 0;
+    for (uint32_t i = t; i < blockIdx.x; i += 256) p += tiles[i];
+    (void)wah_block_scan(p, s_w, &off);   // its barriers order s_c too
+  } else {
+    off = tiles[blockIdx.x];
+    __syncthreads();
+  }
   uint32_t sum = 0;
 #pragma unroll
   for (uint32_t i = 0; i < kWahItems; ++i) sum += s_c[wah_pad(t * kWahItems + i)];
   uint32_t total;
-  uint32_t acc = tile_offs[blockIdx.x] + wah_block_scan(sum, s_w, &total);
+  uint32_t acc = off + wah_block_scan(sum, s_w, &total);
 #pragma unroll
   for (uint32_t i = 0; i < kWahItems; ++i) {
     const uint32_t p = wah_pad(t * kWahItems + i), v = s_c[p];
@@ -300,7 +315,8 @@ __global__ __launch_bounds__(256) void wah_count_scan_kernel(
     const uint32_t j = k * 256 + t, i = base + j;
     if (i < n) excl[i] = s_c[wah_pad(j)];
   }
-  if (blockIdx.x == gridDim.x - 1 && t == 255) excl[n] = tile_offs[gridDim.x];
+  if (blockIdx.x == gridDim.x - 1 && t == 255)
+    excl[n] = kSums ? off + total : tiles[gridDim.x];
 }
 
 // ------------------------------------------------------ AND, decompress
@@ -422,11 +438,18 @@ __device__ __forceinline__ void wah_owners(uint32_t *s_ex, uint32_t cnt,
                                            uint32_t gb,
                                            uint32_t (&own)[kWahPerLane]) {
   const uint32_t lane = lane_id();
-  uint32_t start[kWahPerLane];
+  // each word's start offset from gb, two 16-bit offsets a register;
+  // 0xffff for word 0 (it covers gb and starts at or before it: offset 0
+  // keeps 0), words past cnt and starts outside the span
+  uint32_t offs[kWahPerLane / 2];
 #pragma unroll
   for (uint32_t k = 0; k < kWahPerLane; ++k) {
     const uint32_t j = lane + 64u * k;
-    start[k] = j < cnt ? s_ex[j] : 0xffffffffu;
+    const uint32_t st = j < cnt ? s_ex[j] : 0u;
+    const uint32_t o =
+        j > 0 && j < cnt && st >= gb && st - gb < kWahSpan ? st - gb : 0xffffu;
+    if (k & 1) offs[k >> 1] |= o << 16;
+    else offs[k >> 1] = o;
   }
   wave_lds_sync();
 #pragma unroll
@@ -434,12 +457,10 @@ __device__ __forceinline__ void wah_owners(uint32_t *s_ex, uint32_t cnt,
   wave_lds_sync();
 #pragma unroll
   for (uint32_t k = 0; k < kWahPerLane; ++k) {
-    const uint32_t j = lane + 64u * k;
-    // word 0 covers gb and starts at or before it: offset 0 keeps 0; of
-    // words starting at one offset (all but the last count no group) the
-    // last is kept
-    if (j > 0 && j < cnt && start[k] >= gb && start[k] - gb < kWahSpan)
-      atomicMax(&s_ex[start[k] - gb], j);
+    // of words starting at one offset (all but the last count no group)
+    // the last is kept
+    const uint32_t o = (offs[k >> 1] >> (16u * (k & 1))) & 0xffffu;
+    if (o != 0xffffu) atomicMax(&s_ex[o], lane + 64u * k);
   }
   wave_lds_sync();
   uint32_t carry = 0;
@@ -457,7 +478,8 @@ __device__ __forceinline__ void wah_owners(uint32_t *s_ex, uint32_t cnt,
 // a literal (a 1-fill of one group reads as one) is ANDed with the up word
 // covering its group (past up: 0), a zero result becomes a 0-fill of one.
 // A word is stored as soon as it is known.
-__global__ __launch_bounds__(256) void wah_and_no1_kernel(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+void wah_and_no1_kernel(
     const uint32_t *__restrict__ up, const uint32_t *__restrict__ up_excl,
     uint32_t nup, const uint32_t *__restrict__ down,
     const uint32_t *__restrict__ down_excl, uint32_t ndown, uint32_t *out) {
@@ -761,10 +783,15 @@ int mbx_wah_scan(mbx_ctx *ctx, const uint32_t *wah, size_t n, uint32_t *excl) {
   if (st) return st;
   hipLaunchKernelGGL(wah_count_sum_kernel, dim3(ntiles), dim3(256), 0,
                      ctx->stream, wah, (uint32_t)n, tiles);
-  hipLaunchKernelGGL(wah_tiles_scan_kernel, dim3(1), dim3(256), 0, ctx->stream,
-                     tiles, ntiles);
-  hipLaunchKernelGGL(wah_count_scan_kernel, dim3(ntiles), dim3(256), 0,
-                     ctx->stream, wah, (uint32_t)n, tiles, excl);
+  if (ntiles <= kWahSelfPrefixTiles) {
+    hipLaunchKernelGGL(wah_count_scan_kernel<true>, dim3(ntiles), dim3(256), 0,
+                       ctx->stream, wah, (uint32_t)n, tiles, excl);
+  } else {
+    hipLaunchKernelGGL(wah_tiles_scan_kernel, dim3(1), dim3(256), 0,
+                       ctx->stream, tiles, ntiles);
+    hipLaunchKernelGGL(wah_count_scan_kernel<false>, dim3(ntiles), dim3(256),
+                       0, ctx->stream, wah, (uint32_t)n, tiles, excl);
+  }
   MBX_HIP(hipGetLastError());
   return MBX_OK;
 }

@@ -119,6 +119,24 @@ def test_wah_compress_scan_decompress(dev, seed):
             a.free()
 
 
+@pytest.mark.parametrize("n", [2048 * 4096 - 1, 2048 * 4096, 2048 * 4096 + 1,
+                               9_000_001])
+def test_wah_scan_both_forms(dev, n):
+    """The count scan == oracle on both sides of the tile count up to which
+    each tile sums its predecessors itself (2,048 tiles of 4,096 words);
+    the words mix literals with fills of up to 300 groups."""
+    rng = np.random.default_rng(n)
+    w = rng.integers(1, ONES, n, dtype=np.uint32)
+    fill = rng.random(n) < 0.1
+    w[fill] = (O | (rng.integers(0, 2, n, dtype=np.uint32) << 30)
+               | rng.integers(1, 300, n, dtype=np.uint32))[fill]
+    dw = dev.upload(w)
+    ex = _scan(dev, dw, n)
+    np.testing.assert_array_equal(ex.host(), cpuref.wah_scan(w))
+    dw.free()
+    ex.free()
+
+
 @pytest.mark.parametrize("seed", range(6))
 def test_wah_and_no1(dev, seed):
     """up AND down in down's word structure == oracle (which decompresses
