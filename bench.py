@@ -408,7 +408,8 @@ def run_wah(dev, sch, specs, qs, fused_us, reps=20):
     its foreign key, compressed, scanned; untimed), then the index-only step
     the reference's report times -- the predicates ANDed left to right (the
     operand with more words as the AND's down side), the result scanned and
-    decompressed -- per query (median of `reps`).  A
+    decompressed -- per query (median of `reps`).  Every scan writes a skip
+    table beside it, which the AND and the decompression read.  A
     negated clause (d_year != 6) is its complement range.  The decompressed
     result is checked against the groups of the fused program's bitmap.
     The roofline is the decompression kernel's (the largest writer), its
@@ -420,6 +421,8 @@ def run_wah(dev, sch, specs, qs, fused_us, reps=20):
     groups = dev.alloc(4 * ng, np.uint32, ng)
     res = [dev.alloc(4 * ng, np.uint32, ng) for _ in range(2)]
     rex = dev.alloc(4 * (ng + 1), np.uint32, ng + 1)
+    nsk = int(dev.lib.mbx_wah_skip_len(ng))
+    rsk = dev.alloc(4 * nsk, np.uint32, nsk)
     tmp = dev.alloc(4 * ng, np.uint32, ng)
     nw = ((n + 63) // 64) * 2
     poss = dev.alloc(4 * nw, np.uint32, nw)
@@ -431,12 +434,19 @@ def run_wah(dev, sch, specs, qs, fused_us, reps=20):
         check(dev.lib.mbx_wah_compress(dev.ctx, groups.ptr, ng, tmp.ptr, C.byref(k)))
         w = dev.alloc(4 * max(1, k.value), np.uint32, k.value)
         ex = dev.alloc(4 * (k.value + 1), np.uint32, k.value + 1)
+        sk = dev.alloc(4 * nsk, np.uint32, nsk)
         if k.value:
             check(dev.lib.mbx_memcpy_d2d(dev.ctx, w.ptr, tmp.ptr, 4 * k.value))
-        check(dev.lib.mbx_wah_scan(dev.ctx, w.ptr, k.value, ex.ptr))
+        check(dev.lib.mbx_wah_scan_skip(dev.ctx, w.ptr, k.value, ex.ptr, sk.ptr,
+                                        nsk))
         lf = C.c_uint64()
         check(dev.lib.mbx_wah_longest_1fill(dev.ctx, w.ptr, k.value, C.byref(lf)))
-        return w, ex, k.value, lf.value
+        return w, ex, k.value, lf.value, sk
+
+    def wah_arg(words, excl, k, skip):
+        r = abi.Wah()
+        r.words, r.excl, r.skip, r.n, r.nskip = words, excl, skip, k, nsk
+        return r
 
     def ranges(spec):
         neg, loaded = set(), {}
@@ -473,19 +483,29 @@ def run_wah(dev, sch, specs, qs, fused_us, reps=20):
             order.append((i, down))
             acc1, acc_n = 1, (ws[i][2] if down else acc_n)
 
+        # the step's arguments, built once: an AND's result keeps its down
+        # operand's words, so their count scan and skip table serve it
+        ands = []
+        cur = wah_arg(ws[0][0].ptr, ws[0][1].ptr, ws[0][2], ws[0][4].ptr)
+        b = 0
+        for i, down in order[1:]:
+            w, ex, k, _, sk = ws[i]
+            mine = wah_arg(w.ptr, ex.ptr, k, sk.ptr)
+            up, dn = (cur, mine) if down else (mine, cur)
+            ands.append((up, dn, res[b].ptr))
+            cur = wah_arg(res[b].ptr, dn.excl, dn.n, dn.skip)
+            b ^= 1
+        final = wah_arg(cur.words, rex.ptr, cur.n, rsk.ptr)
+
         def step():
-            cur, cur_ex, cur_n, b = ws[0][0].ptr, ws[0][1].ptr, ws[0][2], 0
-            for i, down in order[1:]:
-                w, ex, k, _ = ws[i]
-                up, upx, nup = (cur, cur_ex, cur_n) if down else (w.ptr, ex.ptr, k)
-                dn, dnx, ndn = (w.ptr, ex.ptr, k) if down else (cur, cur_ex, cur_n)
-                check(dev.lib.mbx_wah_and_no1(dev.ctx, up, upx, nup, dn, dnx, ndn,
-                                              res[b].ptr))
-                cur, cur_ex, cur_n, b = res[b].ptr, dnx, ndn, b ^ 1
-            check(dev.lib.mbx_wah_scan(dev.ctx, cur, cur_n, rex.ptr))
-            check(dev.lib.mbx_wah_decompress(dev.ctx, cur, rex.ptr, cur_n,
-                                             groups.ptr, ng))
-            return cur, cur_n
+            for up, dn, o in ands:
+                check(dev.lib.mbx_wah_and_no1_skip(dev.ctx, C.byref(up),
+                                                   C.byref(dn), o))
+            check(dev.lib.mbx_wah_scan_skip(dev.ctx, final.words, final.n,
+                                            rex.ptr, rsk.ptr, nsk))
+            check(dev.lib.mbx_wah_decompress_skip(dev.ctx, C.byref(final),
+                                                  groups.ptr, ng))
+            return final.words, final.n
 
         cur, cur_n = step()
         dev.sync()
@@ -504,26 +524,31 @@ def run_wah(dev, sch, specs, qs, fused_us, reps=20):
         if cur_n:
             check(dev.lib.mbx_memcpy_d2d(dev.ctx, keep.ptr, cur, 4 * cur_n))
         kex = dev.alloc(4 * (cur_n + 1), np.uint32, cur_n + 1)
-        check(dev.lib.mbx_wah_scan(dev.ctx, keep.ptr, cur_n, kex.ptr))
-        steps.append((keep, kex, cur_n))
-        for w, ex, _, _ in ws:
+        ksk = dev.alloc(4 * nsk, np.uint32, nsk)
+        check(dev.lib.mbx_wah_scan_skip(dev.ctx, keep.ptr, cur_n, kex.ptr,
+                                        ksk.ptr, nsk))
+        steps.append((keep, kex, cur_n, ksk,
+                      wah_arg(keep.ptr, kex.ptr, cur_n, ksk.ptr)))
+        for w, ex, _, _, sk in ws:
             w.free()
             ex.free()
+            sk.free()
     # decompression in isolation: every query's result back to back
     dev.sync()
     t.start()
     for _ in range(reps):
-        for keep, kex, k in steps:
-            check(dev.lib.mbx_wah_decompress(dev.ctx, keep.ptr, kex.ptr, k,
-                                             groups.ptr, ng))
+        for _, _, _, _, arg in steps:
+            check(dev.lib.mbx_wah_decompress_skip(dev.ctx, C.byref(arg),
+                                                  groups.ptr, ng))
     ms = t.stop()
     t.close()
     launch_us = ms * 1e3 / (reps * len(steps))
-    alg = sum(4 * (k + k + 1 + ng) for _, _, k in steps) / len(steps)
-    for keep, kex, _ in steps:
+    alg = sum(4 * (k + k + 1 + ng + nsk) for _, _, k, _, _ in steps) / len(steps)
+    for keep, kex, _, ksk, _ in steps:
+        ksk.free()
         keep.free()
         kex.free()
-    for a in [groups, tmp, rex, poss] + res:
+    for a in [groups, tmp, rex, rsk, poss] + res:
         a.free()
     wah_sum = sum(per_q.values())
     return {
@@ -542,8 +567,9 @@ def run_wah(dev, sch, specs, qs, fused_us, reps=20):
             "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(alg / launch_us / 1e3 / HBM_PEAK_GBS, 4),
             "alg_bytes_per_launch": int(alg), "launch_us": round(launch_us, 2),
-            "note": "4 B per WAH word and count-scan entry read, 4 B per group "
-                    "written; launches back to back under one event pair"},
+            "note": "4 B per WAH word, count-scan and skip-table entry read, "
+                    "4 B per group written; launches back to back under one "
+                    "event pair"},
     }
 
 

@@ -283,6 +283,35 @@ int mbx_wah_decompress(mbx_ctx *ctx, const uint32_t *wah, const uint32_t *excl,
  * one).  Waits for its kernel. */
 int mbx_wah_longest_1fill(mbx_ctx *ctx, const uint32_t *wah, size_t n,
                           uint64_t *longest);
+/* No reference counterpart: a skip table beside the count scan, so that
+ * the AND and the decompression find the word covering a group with one
+ * read instead of a search through the scan.  skip[i] = the word covering
+ * group 1024*i, with bit 31 set when that word starts at the group, for i
+ * below nskip and below ceil(groups / 1024); mbx_wah_skip_len(G) =
+ * ceil(G / 1024) entries cover a bitmap of G groups.  mbx_wah_scan_skip is
+ * mbx_wah_scan plus the table (mbx_wah_scan = a table of no entries). */
+size_t mbx_wah_skip_len(size_t ngroups);
+int mbx_wah_scan_skip(mbx_ctx *ctx, const uint32_t *wah, size_t n,
+                      uint32_t *excl, uint32_t *skip, size_t nskip);
+/* A WAH bitmap on the device: n words, their count scan (n + 1 entries)
+ * and their skip table (nskip entries of mbx_wah_scan_skip, or null and
+ * 0); a table shorter than the bitmap's groups need is completed by
+ * searches. */
+typedef struct mbx_wah {
+  const uint32_t *words;
+  const uint32_t *excl;
+  const uint32_t *skip;
+  size_t n;
+  size_t nskip;
+} mbx_wah;
+/* mbx_wah_and_no1 and mbx_wah_decompress on bitmaps with skip tables, the
+ * same results: the AND takes the down bitmap's groups 1,024 at a time
+ * (the down words starting there, the up words covering them) when down
+ * has a table, else it is mbx_wah_and_no1. */
+int mbx_wah_and_no1_skip(mbx_ctx *ctx, const mbx_wah *up, const mbx_wah *down,
+                         uint32_t *out);
+int mbx_wah_decompress_skip(mbx_ctx *ctx, const mbx_wah *w, uint32_t *groups,
+                            size_t ngroups);
 
 /* Program + group-by in one call.  d_out[nr_grp] int64 on the device.
  * The fused plan never materialises a bitmap. */

@@ -32,6 +32,11 @@ class ColRef(C.Structure):
                 ("dimsz", C.c_uint32), ("elem_bytes", C.c_uint32)]
 
 
+class Wah(C.Structure):
+    _fields_ = [("words", C.c_void_p), ("excl", C.c_void_p), ("skip", C.c_void_p),
+                ("n", C.c_size_t), ("nskip", C.c_size_t)]
+
+
 class Col(C.Structure):
     _fields_ = [("middle", C.c_void_p * MAXBIN_PERCOL),
                 ("leftmost", C.c_void_p), ("rightmost", C.c_void_p)]
@@ -110,7 +115,7 @@ REQ_QUEUED, REQ_ADMITTED, REQ_RUNNING, REQ_DONE, REQ_REJECTED, REQ_CANCELLED, \
 
 NFEAT = 16
 ABI_STRUCTS = [ColRef, Col, Instr, Vprg, Pred, KeyPart, RowOp, Query, Tune, Plan,
-               ServeCfg, ServeStats]
+               ServeCfg, ServeStats, Wah]
 
 _vp, _sz, _u32, _u64, _i32 = C.c_void_p, C.c_size_t, C.c_uint32, C.c_uint64, C.c_int
 _P = C.POINTER
@@ -159,6 +164,10 @@ PROTOTYPES = {
     "mbx_wah_and_no1": (_i32, [_vp, _vp, _vp, _sz, _vp, _vp, _sz, _vp]),
     "mbx_wah_decompress": (_i32, [_vp, _vp, _vp, _sz, _vp, _sz]),
     "mbx_wah_longest_1fill": (_i32, [_vp, _vp, _sz, _P(_u64)]),
+    "mbx_wah_skip_len": (_sz, [_sz]),
+    "mbx_wah_scan_skip": (_i32, [_vp, _vp, _sz, _vp, _vp, _sz]),
+    "mbx_wah_and_no1_skip": (_i32, [_vp, _P(Wah), _P(Wah), _vp]),
+    "mbx_wah_decompress_skip": (_i32, [_vp, _P(Wah), _vp, _sz]),
     "mbx_prog_grpby": (_i32, [_vp, _P(Vprg), _P(RowOp), _u32, _vp, _u32]),
     "mbx_prog_grpby_noalloc": (_i32, [_vp, _P(Vprg), _P(RowOp), _u32, _vp,
                                       _u32]),

@@ -315,7 +315,7 @@ int mbx_abi_sizes(size_t *out, int n) {
                       sizeof(mbx_vprg),   sizeof(mbx_pred),  sizeof(mbx_keypart),
                       sizeof(mbx_rowop),  sizeof(mbx_query), sizeof(mbx_tune),
                       sizeof(mbx_plan), sizeof(mbx_serve_cfg),
-                      sizeof(mbx_serve_stats)};
+                      sizeof(mbx_serve_stats), sizeof(mbx_wah)};
   const int m = (int)(sizeof(s) / sizeof(s[0]));
   for (int i = 0; i < n && i < m; ++i) out[i] = s[i];
   return m;

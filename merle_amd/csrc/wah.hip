@@ -31,6 +31,10 @@ constexpr uint32_t kWahRows = 0x7fffffffu;
 // a tile of the scans: 256 threads x kWahItems consecutive items
 constexpr uint32_t kWahItems = 16;
 constexpr uint32_t kWahTile = 256 * kWahItems;
+// groups between consecutive entries of a skip table: entry i names the
+// word covering group kWahSkip * i, bit 31 set when the word starts there
+constexpr uint32_t kWahSkip = 1024;
+constexpr uint32_t kWahSkipAt = 0x80000000u;
 
 __device__ __forceinline__ uint32_t wah_count(uint32_t w) {
   return (w & kWahFill) ? (w & kWahCnt) : 1u;
@@ -280,7 +284,8 @@ constexpr uint32_t kWahSelfPrefixTiles = 2048;
 template <bool kSums>
 __global__ __launch_bounds__(256) void wah_count_scan_kernel(
     const uint32_t *__restrict__ wah, uint32_t n,
-    const uint32_t *__restrict__ tiles, uint32_t *excl) {
+    const uint32_t *__restrict__ tiles, uint32_t *excl, uint32_t *skip,
+    uint32_t nskip) {
   __shared__ uint32_t s_w[4];
   __shared__ uint32_t s_c[kWahTile + kWahTile / 16];
   const uint32_t base = blockIdx.x * kWahTile, t = threadIdx.x;
@@ -317,6 +322,23 @@ __global__ __launch_bounds__(256) void wah_count_scan_kernel(
   }
   if (blockIdx.x == gridDim.x - 1 && t == 255)
     excl[n] = kSums ? off + total : tiles[gridDim.x];
+  if (!skip) return;
+  // the skip entries whose group lies in the tile's groups [off, off +
+  // total): the last of the tile's words starting at or before it
+  const uint32_t tn = min(n - base, kWahTile);
+  const uint64_t hi = (uint64_t)off + total;
+  const uint64_t iend = min<uint64_t>((hi + kWahSkip - 1) / kWahSkip, nskip);
+  for (uint64_t i = ((uint64_t)off + kWahSkip - 1) / kWahSkip + t; i < iend;
+       i += 256) {
+    const uint32_t g = (uint32_t)(i * kWahSkip);
+    uint32_t a = 0, b = tn;   // s_c[a] <= g (item 0 starts at off <= g)
+    while (b - a > 1) {
+      const uint32_t m = (a + b) >> 1;
+      if (s_c[wah_pad(m)] <= g) a = m;
+      else b = m;
+    }
+    skip[i] = (base + a) | (s_c[wah_pad(a)] == g ? kWahSkipAt : 0u);
+  }
 }
 
 // ------------------------------------------------------ AND, decompress
@@ -342,7 +364,7 @@ __device__ __forceinline__ uint32_t wah_find(const uint32_t *__restrict__ excl,
 // wave's first group on; an item then costs a 10-step binary search in
 // LDS.  The items' groups increase with the item, so an item the staged
 // words do not reach is reached by a later chunk of words.
-constexpr uint32_t kWahSpan = 1024;
+constexpr uint32_t kWahSpan = kWahSkip;
 constexpr uint32_t kWahPerLane = kWahSpan / 64;
 
 struct WahStage {
@@ -368,6 +390,29 @@ __device__ __forceinline__ uint32_t wah_wave_find(
     lo = nlo;
   }
   return lo;
+}
+
+// The word covering group kWahSkip * i (below excl[n]): from the skip table
+// when it holds entry i, else by the wave search.  Wave-uniform i.  A table
+// entry is kept below n, so a table of another bitmap gives wrong words,
+// never reads out of bounds.
+__device__ __forceinline__ uint32_t wah_cover_of(
+    const uint32_t *__restrict__ excl, const uint32_t *__restrict__ skip,
+    uint32_t nskip, uint32_t n, uint32_t i) {
+  if (skip && i < nskip) return min(skip[i] & ~kWahSkipAt, n - 1);
+  return wah_wave_find(excl, n, i * kWahSkip);
+}
+
+// The first word starting at or after group kWahSkip * i (below excl[n]).
+__device__ __forceinline__ uint32_t wah_first_of(
+    const uint32_t *__restrict__ excl, const uint32_t *__restrict__ skip,
+    uint32_t nskip, uint32_t n, uint32_t i) {
+  if (skip && i < nskip) {
+    const uint32_t e = skip[i], u = min(e & ~kWahSkipAt, n - 1);
+    return (e & kWahSkipAt) ? u : u + 1;
+  }
+  const uint32_t u = wah_wave_find(excl, n, i * kWahSkip);
+  return excl[u] == i * kWahSkip ? u : u + 1;
 }
 
 // Stages words [w0, w0 + c), c = min(n - w0, kWahSpan), of (wah, excl, n):
@@ -556,12 +601,96 @@ void wah_and_no1_kernel(
   }
 }
 
+// wahAndNo1 over bitmaps with skip tables: a wave takes the down words
+// starting in one window of kWahSkip groups (at most kWahSkip words when
+// each counts a group) and stages the up words covering the window (at
+// most kWahSkip likewise) from the up table's entry, so one staging and
+// the owner scan resolve every literal of the window.  A window whose
+// words exceed those bounds (words counting no group) takes a binary
+// search in device memory per literal.  Persistent: the window count is
+// the down bitmap's groups / kWahSkip, known on the device only.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+void wah_and_no1_win_kernel(
+    const uint32_t *__restrict__ up, const uint32_t *__restrict__ up_excl,
+    const uint32_t *__restrict__ up_skip, uint32_t nsu, uint32_t nup,
+    const uint32_t *__restrict__ down, const uint32_t *__restrict__ down_excl,
+    const uint32_t *__restrict__ down_skip, uint32_t nsd, uint32_t ndown,
+    uint32_t *out) {
+  __shared__ WahStage st;
+  const uint32_t wv = threadIdx.x >> 6, lane = lane_id();
+  const uint32_t dtot = down_excl[ndown];   // ndown >= 1
+  const uint32_t utot = nup ? up_excl[nup] : 0u;
+  const uint32_t nwin = (uint32_t)(((uint64_t)dtot + kWahSkip - 1) / kWahSkip);
+  for (uint32_t w = blockIdx.x * 4 + wv; w < nwin; w += gridDim.x * 4) {
+    const uint32_t gb = w * kWahSkip;
+    // (window 0 also takes the words before the first word covering a
+    // group, which count none)
+    const uint32_t a = w ? wah_first_of(down_excl, down_skip, nsd, ndown, w) : 0u;
+    const uint32_t b = (uint64_t)gb + kWahSkip < dtot
+                           ? wah_first_of(down_excl, down_skip, nsd, ndown, w + 1)
+                           : ndown;
+    if (a >= b) continue;   // no word starts in the window
+    if (b - a <= kWahSpan) {
+      // the down words, their loads in flight with the up staging
+      uint32_t dwv[kWahPerLane], gv[kWahPerLane];
+#pragma unroll
+      for (uint32_t k = 0; k < kWahPerLane; ++k) {
+        const uint32_t d = a + lane + 64u * k, at = d < b ? d : a;
+        const uint32_t dw = down[at];
+        gv[k] = down_excl[at];
+        dwv[k] = dw == (kWahFill | kWahOne | 1u) ? kWahRows : dw;
+      }
+      bool fast = true;   // no up group in the window: every literal reads 0
+      if (gb < utot) {
+        uint32_t cnt;
+        const uint32_t c = wah_cover_of(up_excl, up_skip, nsu, nup, w);
+        const uint32_t end =
+            wah_stage(up, up_excl, nup, c, st.ex[wv], st.rows[wv], &cnt);
+        fast = st.ex[wv][0] <= gb && end > min(gb + kWahSkip, utot) - 1;
+        if (fast) {
+          uint32_t own[kWahPerLane];
+          wah_owners(st.ex[wv], cnt, gb, own);
+        }
+      }
+      if (fast) {
+#pragma unroll
+        for (uint32_t k = 0; k < kWahPerLane; ++k) {
+          const uint32_t d = a + lane + 64u * k;
+          const uint32_t dw = dwv[k], g = gv[k];
+          uint32_t o = dw;
+          if (!(dw & kWahFill)) {
+            const uint32_t v =
+                g < utot
+                    ? dw & st.rows[wv][st.ex[wv][min(g - gb, kWahSkip - 1)]]
+                    : 0u;
+            o = v ? v : (kWahFill | 1u);
+          }
+          if (d < b) out[d] = o;
+        }
+        continue;
+      }
+    }
+    for (uint32_t d = a + lane; d < b; d += 64) {
+      const uint32_t dw0 = down[d], g = down_excl[d];
+      const uint32_t dw = dw0 == (kWahFill | kWahOne | 1u) ? kWahRows : dw0;
+      uint32_t o = dw;
+      if (!(dw & kWahFill)) {
+        const uint32_t v =
+            g < utot ? dw & wah_rows(up[wah_find(up_excl, nup, g)]) : 0u;
+        o = v ? v : (kWahFill | 1u);
+      }
+      out[d] = o;
+    }
+  }
+}
+
 // wahDecomp (wahGpuCmprs.cu:37-51): groups of the words, zero past them.
 // A group is stored as soon as a chunk reaches it (the first chunk reaches
 // every group unless some word counts no group).
 __global__ __launch_bounds__(256) void wah_decompress_kernel(
     const uint32_t *__restrict__ wah, const uint32_t *__restrict__ excl,
-    uint32_t n, uint32_t *groups, uint32_t ng) {
+    const uint32_t *__restrict__ skip, uint32_t nskip, uint32_t n,
+    uint32_t *groups, uint32_t ng) {
   __shared__ WahStage st;
   const uint32_t wv = threadIdx.x >> 6, lane = lane_id();
   const uint64_t g0 = ((uint64_t)blockIdx.x * 4 + wv) * kWahSpan;
@@ -569,7 +698,7 @@ __global__ __launch_bounds__(256) void wah_decompress_kernel(
   const uint32_t total = n ? excl[n] : 0u;
   uint32_t w0 = 0, cnt = 0, end = 0, pend = 0;
   if (g0 < total) {
-    w0 = wah_wave_find(excl, n, (uint32_t)g0);
+    w0 = wah_cover_of(excl, skip, nskip, n, (uint32_t)(g0 / kWahSkip));
     end = wah_stage(wah, excl, n, w0, st.ex[wv], st.rows[wv], &cnt);
     const uint32_t glast = (uint32_t)min<uint64_t>(g0 + kWahSpan, total) - 1;
     if (end > glast) {
@@ -770,12 +899,37 @@ static int wah_tiles_of(mbx_ctx *ctx, size_t bytes, uint32_t **out) {
   return MBX_OK;
 }
 
+size_t mbx_wah_skip_len(size_t ngroups) { return ceil_div(ngroups, kWahSkip); }
+
+// A WAH bitmap argument: its arrays on the context's device, the sizes
+// within the format's limits.
+static int wah_ref_check(mbx_ctx *ctx, const mbx_wah &w, const char *what) {
+  if (!w.excl || (!w.words && w.n) || (!w.skip && w.nskip))
+    return fail(MBX_ERR_INVALID, std::string(what) + ": null");
+  int st = wah_check(ctx, w.excl, what);
+  if (!st && w.n) st = wah_check(ctx, w.words, what);
+  if (!st && w.nskip) st = wah_check(ctx, w.skip, what);
+  if (st) return st;
+  if (w.n > kWahCnt || w.nskip > kWahCnt)
+    return fail(MBX_ERR_CAPACITY, std::string(what) + ": too many words");
+  return MBX_OK;
+}
+
 int mbx_wah_scan(mbx_ctx *ctx, const uint32_t *wah, size_t n, uint32_t *excl) {
-  if ((!wah && n) || !excl) return fail(MBX_ERR_INVALID, "wah_scan: null");
+  return mbx_wah_scan_skip(ctx, wah, n, excl, nullptr, 0);
+}
+
+int mbx_wah_scan_skip(mbx_ctx *ctx, const uint32_t *wah, size_t n,
+                      uint32_t *excl, uint32_t *skip, size_t nskip) {
+  if ((!wah && n) || !excl || (!skip && nskip))
+    return fail(MBX_ERR_INVALID, "wah_scan: null");
   int st = wah_check(ctx, excl, "wah_scan");
   if (!st && n) st = wah_check(ctx, wah, "wah_scan");
+  if (!st && nskip) st = wah_check(ctx, skip, "wah_scan");
   if (st) return st;
-  if (n > kWahCnt) return fail(MBX_ERR_CAPACITY, "wah_scan: too many words");
+  if (n > kWahCnt || nskip > kWahCnt)
+    return fail(MBX_ERR_CAPACITY, "wah_scan: too many words");
+  if (!nskip) skip = nullptr;
   if (n == 0) return launch_fill_u32(ctx->stream, excl, 0, 1);
   const uint32_t ntiles = (uint32_t)ceil_div(n, kWahTile);
   uint32_t *tiles = nullptr;
@@ -785,12 +939,14 @@ int mbx_wah_scan(mbx_ctx *ctx, const uint32_t *wah, size_t n, uint32_t *excl) {
                      ctx->stream, wah, (uint32_t)n, tiles);
   if (ntiles <= kWahSelfPrefixTiles) {
     hipLaunchKernelGGL(wah_count_scan_kernel<true>, dim3(ntiles), dim3(256), 0,
-                       ctx->stream, wah, (uint32_t)n, tiles, excl);
+                       ctx->stream, wah, (uint32_t)n, tiles, excl, skip,
+                       (uint32_t)nskip);
   } else {
     hipLaunchKernelGGL(wah_tiles_scan_kernel, dim3(1), dim3(256), 0,
                        ctx->stream, tiles, ntiles);
     hipLaunchKernelGGL(wah_count_scan_kernel<false>, dim3(ntiles), dim3(256),
-                       0, ctx->stream, wah, (uint32_t)n, tiles, excl);
+                       0, ctx->stream, wah, (uint32_t)n, tiles, excl, skip,
+                       (uint32_t)nskip);
   }
   MBX_HIP(hipGetLastError());
   return MBX_OK;
@@ -818,21 +974,48 @@ int mbx_wah_and_no1(mbx_ctx *ctx, const uint32_t *up, const uint32_t *up_excl,
   return MBX_OK;
 }
 
+int mbx_wah_and_no1_skip(mbx_ctx *ctx, const mbx_wah *up, const mbx_wah *down,
+                         uint32_t *out) {
+  if (!up || !down || !out) return fail(MBX_ERR_INVALID, "wah_and_no1: null");
+  int st = wah_ref_check(ctx, *up, "wah_and_no1");
+  if (!st) st = wah_ref_check(ctx, *down, "wah_and_no1");
+  if (!st) st = wah_check(ctx, out, "wah_and_no1");
+  if (st) return st;
+  if (down->n == 0) return MBX_OK;
+  if (!down->skip || !down->nskip)   // no down windows: the word-mapped kernel
+    return mbx_wah_and_no1(ctx, up->words, up->excl, up->n, down->words,
+                           down->excl, down->n, out);
+  hipLaunchKernelGGL(wah_and_no1_win_kernel, dim3((unsigned)ctx->num_cus * 4),
+                     dim3(256), 0, ctx->stream, up->words, up->excl,
+                     up->nskip ? up->skip : nullptr, (uint32_t)up->nskip,
+                     (uint32_t)up->n, down->words, down->excl, down->skip,
+                     (uint32_t)down->nskip, (uint32_t)down->n, out);
+  MBX_HIP(hipGetLastError());
+  return MBX_OK;
+}
+
 int mbx_wah_decompress(mbx_ctx *ctx, const uint32_t *wah, const uint32_t *excl,
                        size_t n, uint32_t *groups, size_t ngroups) {
-  if (!excl || !groups || (!wah && n))
-    return fail(MBX_ERR_INVALID, "wah_decompress: null");
-  int st = wah_check(ctx, groups, "wah_decompress");
-  if (!st) st = wah_check(ctx, excl, "wah_decompress");
-  if (!st && n) st = wah_check(ctx, wah, "wah_decompress");
+  const mbx_wah w{wah, excl, nullptr, n, 0};
+  return mbx_wah_decompress_skip(ctx, &w, groups, ngroups);
+}
+
+int mbx_wah_decompress_skip(mbx_ctx *ctx, const mbx_wah *w, uint32_t *groups,
+                            size_t ngroups) {
+  if (!w) return fail(MBX_ERR_INVALID, "wah_decompress: null");
+  int st = wah_ref_check(ctx, *w, "wah_decompress");
   if (st) return st;
-  if (n > kWahCnt || ngroups > 0xffffffffull)
+  if (!groups) return fail(MBX_ERR_INVALID, "wah_decompress: null");
+  st = wah_check(ctx, groups, "wah_decompress");
+  if (st) return st;
+  if (ngroups > 0xffffffffull)
     return fail(MBX_ERR_CAPACITY, "wah_decompress: too many words or groups");
   if (ngroups == 0) return MBX_OK;
   hipLaunchKernelGGL(wah_decompress_kernel,
                      dim3(grid_of(ceil_div(ngroups, kWahSpan), 4)), dim3(256), 0,
-                     ctx->stream, wah, excl, (uint32_t)n, groups,
-                     (uint32_t)ngroups);
+                     ctx->stream, w->words, w->excl,
+                     w->nskip ? w->skip : nullptr, (uint32_t)w->nskip,
+                     (uint32_t)w->n, groups, (uint32_t)ngroups);
   MBX_HIP(hipGetLastError());
   return MBX_OK;
 }

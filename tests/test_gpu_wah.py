@@ -164,6 +164,153 @@ def test_wah_and_no1(dev, seed):
             x.free()
 
 
+def _scan_skip(dev, wah_dev, n, nskip):
+    ex = dev.alloc(4 * (n + 1), np.uint32, n + 1)
+    sk = dev.alloc(4 * max(nskip, 1), np.uint32, max(nskip, 1))
+    check(dev.lib.mbx_wah_scan_skip(dev.ctx, wah_dev.ptr if n else None, n,
+                                    ex.ptr, sk.ptr, nskip))
+    return ex, sk
+
+
+def _wah_ref(w_dev, ex, sk, n, nskip):
+    r = abi.Wah()
+    r.words = w_dev.ptr if n else None
+    r.excl = ex.ptr
+    r.skip = sk.ptr if (sk is not None and nskip) else None
+    r.n = n
+    r.nskip = nskip if sk is not None else 0
+    return r
+
+
+def _with_empty_words(rng, w, k):
+    """w with k words that count no group (0-fills of 0) at random places,
+    some of them in one run longer than a skip window's 1,024 words."""
+    at = np.sort(rng.integers(0, len(w) + 1, k))
+    out = np.insert(w, at, np.uint32(O))
+    i = int(rng.integers(0, len(out) + 1))
+    return np.insert(out, i, np.full(1500, O, np.uint32))
+
+
+def _skip_expected(excl, n, nskip):
+    """Entry i: the last word starting at or before group 1,024 i (bit 31:
+    it starts there), for i below nskip and the bitmap's groups / 1,024."""
+    total = int(excl[n])
+    m = min(nskip, -(-total // 1024))
+    g = 1024 * np.arange(m, dtype=np.int64)
+    u = np.searchsorted(excl[:n].astype(np.int64), g, side="right") - 1
+    return (u.astype(np.uint32)
+            | np.where(excl[u] == g, np.uint32(O), np.uint32(0))), m
+
+
+def _skip_cases(rng):
+    cases = [np.array([O | 1024, 0x1234, O | 3000, 0x55], np.uint32),
+             np.array([0x77] * 1024 + [O | 0x40000000 | 2048, 0x9], np.uint32)]
+    for ng in (1, 1023, 1024, 1025, 70_001, 2_000_003):
+        x = _runs(rng, ng, p_lit=float(rng.choice([0.02, 0.3, 0.9])),
+                  max_run=int(rng.choice([40, 5000])))
+        cases.append(cpuref.wah_compress(x))
+    return cases
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_wah_scan_skip(dev, seed):
+    """The skip table: entry i = the last word starting at or before group
+    1,024 i, bit 31 when it starts there (searchsorted on the oracle's
+    scan), full, short and empty tables, words counting no group among the
+    others; the scan beside it == oracle."""
+    rng = np.random.default_rng(700 + seed)
+    for w in _skip_cases(rng):
+        if seed == 2:
+            w = _with_empty_words(rng, w, 50)
+        n = len(w)
+        dw = dev.upload(w) if n else None
+        want_ex = cpuref.wah_scan(w)
+        full = -(-int(want_ex[n]) // 1024)
+        assert dev.lib.mbx_wah_skip_len(int(want_ex[n])) == full
+        for nskip in (full, full // 2, full + 3, 0):
+            ex, sk = _scan_skip(dev, dw, n, nskip)
+            np.testing.assert_array_equal(ex.host(), want_ex)
+            want, m = _skip_expected(want_ex, n, nskip)
+            np.testing.assert_array_equal(sk.host()[:m], want)
+            ex.free()
+            sk.free()
+        if dw is not None:
+            dw.free()
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_wah_decompress_skip(dev, seed):
+    """Decompression through the skip table == oracle with full, short and
+    no tables, words counting no group among the others (seeds 2, 3)."""
+    rng = np.random.default_rng(800 + seed)
+    for w in _skip_cases(rng):
+        if len(w) == 0:
+            continue
+        if seed >= 2:
+            w = _with_empty_words(rng, w, 50)
+        n = len(w)
+        dw = dev.upload(w)
+        total = int(cpuref.wah_scan(w)[n])
+        full = -(-total // 1024)
+        for nskip in (full, full // 3, 0):
+            ex, sk = _scan_skip(dev, dw, n, nskip)
+            r = _wah_ref(dw, ex, sk if nskip else None, n, nskip)
+            for cap in (total, total + 2000, max(1, total // 2)):
+                dec = dev.alloc(4 * cap, np.uint32, cap)
+                check(dev.lib.mbx_wah_decompress_skip(dev.ctx, C.byref(r),
+                                                      dec.ptr, cap))
+                np.testing.assert_array_equal(dec.host(),
+                                              cpuref.wah_decompress(w, cap))
+                dec.free()
+            ex.free()
+            sk.free()
+        dw.free()
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_wah_and_no1_skip(dev, seed):
+    """The AND by skip windows == oracle: both tables, a short table on
+    either side, no up table, no down table (the word-mapped kernel), down
+    1-fills of one group, down words past the end of up and words counting
+    no group on either side (seeds 2, 3: runs of 1,500 of them push a
+    window past its one-staging bounds)."""
+    rng = np.random.default_rng(900 + seed)
+    cases = [(np.array([O | 5, 0x7ffffffe], np.uint32),
+              np.array([O | 2, 0x12345, O | 0x40000000 | 1, 0x44, O | 3000, 0x66],
+                       np.uint32))]
+    for _ in range(4):
+        ng = int(rng.choice([50, 5000, 300_000, 2_000_003]))
+        a = _runs(rng, ng, p_lit=float(rng.choice([0.05, 0.5, 0.95])),
+                  max_run=int(rng.choice([40, 5000])))
+        b = _runs(rng, int(ng * rng.choice([0.5, 1.0, 1.5])),
+                  p_lit=float(rng.choice([0.05, 0.5, 0.95])), p_one=0.0,
+                  max_run=int(rng.choice([40, 5000])))
+        cases.append((cpuref.wah_compress(a), cpuref.wah_compress(b)))
+    for up, down in cases:
+        if len(down) == 0 or len(up) == 0:
+            continue
+        if seed >= 2:
+            up = _with_empty_words(rng, up, 30)
+            down = _with_empty_words(rng, down, 30)
+        want = cpuref.wah_and_no1(up, down)
+        du, dd = dev.upload(up), dev.upload(down)
+        tu = -(-int(cpuref.wah_scan(up)[-1]) // 1024)
+        td = -(-int(cpuref.wah_scan(down)[-1]) // 1024)
+        for nu, nd in ((tu, td), (tu // 2, td), (tu, td // 2), (0, td), (tu, 0)):
+            exu, sku = _scan_skip(dev, du, len(up), nu)
+            exd, skd = _scan_skip(dev, dd, len(down), nd)
+            ru = _wah_ref(du, exu, sku if nu else None, len(up), nu)
+            rd = _wah_ref(dd, exd, skd if nd else None, len(down), nd)
+            out = dev.alloc(4 * len(down), np.uint32, len(down))
+            check(dev.lib.mbx_wah_and_no1_skip(dev.ctx, C.byref(ru), C.byref(rd),
+                                               out.ptr))
+            np.testing.assert_array_equal(out.host(), want, err_msg=str((nu, nd)))
+            for x in (exu, sku, exd, skd, out):
+                x.free()
+        du.free()
+        dd.free()
+
+
 def test_wah_longest_1fill(dev):
     """Consecutive one fills add up; a 0x7fffffff literal counts as one."""
     rng = np.random.default_rng(7)
