@@ -565,11 +565,13 @@ double now_ms() {
       .count();
 }
 
-// One predicate as a WAH bitmap (device words and their count scan).
+// One predicate as a WAH bitmap (device words, their count scan and skip
+// table).
 struct Wah {
-  uint32_t *w = nullptr, *ex = nullptr;
-  size_t n = 0;
+  uint32_t *w = nullptr, *ex = nullptr, *skip = nullptr;
+  size_t n = 0, nskip = 0;
   uint64_t longest1 = 0;
+  mbx_wah arg() const { return mbx_wah{w, ex, skip, n, nskip}; }
 };
 
 // The predicate's groups (mbx_wah_select, the analogue of dbjoinFlatWah)
@@ -587,10 +589,13 @@ bool wah_of(mbx_ctx *ctx, const mbx_colref &src, uint32_t nrows, uint64_t lo,
     return false;
   }
   out.n = k;
+  out.nskip = mbx_wah_skip_len(ng);
   if (mbx_alloc(ctx, 4 * (k ? k : 1), (void **)&out.w) != MBX_OK ||
       mbx_alloc(ctx, 4 * (k + 1), (void **)&out.ex) != MBX_OK ||
+      mbx_alloc(ctx, 4 * (out.nskip ? out.nskip : 1), (void **)&out.skip) !=
+          MBX_OK ||
       (k && mbx_memcpy_d2d(ctx, out.w, tmp, 4 * k) != MBX_OK) ||
-      mbx_wah_scan(ctx, out.w, k, out.ex) != MBX_OK ||
+      mbx_wah_scan_skip(ctx, out.w, k, out.ex, out.skip, out.nskip) != MBX_OK ||
       mbx_wah_longest_1fill(ctx, out.w, k, &out.longest1) != MBX_OK) {
     std::fprintf(stderr, "wah: %s\n", mbx_last_error());
     return false;
@@ -603,6 +608,7 @@ bool wah_of(mbx_ctx *ctx, const mbx_colref &src, uint32_t nrows, uint64_t lo,
 void wah_free(mbx_ctx *ctx, Wah &w) {
   if (w.w) mbx_free(ctx, w.w);
   if (w.ex) mbx_free(ctx, w.ex);
+  if (w.skip) mbx_free(ctx, w.skip);
   w = Wah();
 }
 
@@ -664,16 +670,20 @@ int main(int argc, char **argv) {
   // ssb/wah-demo.cu:24-292: each flight's predicates (its exact ranges, as
   // the Perfect layout below) as WAH bitmaps, built and scanned untimed;
   // timed per rep, index only: the predicates ANDed left to right
-  // (mbx_wah_and_no1; the down side, which must hold no long 1-fill, is
-  // the operand with more words, and a flight where neither operand can
-  // be down prints nan), the result scanned and decompressed.
+  // (mbx_wah_and_no1_skip; the down side, which must hold no long 1-fill,
+  // is the operand with more words, and a flight where neither operand
+  // can be down prints nan), the result scanned and decompressed (every
+  // scan with its skip table).
   // RTScan (OptiX ray tracing) has no counterpart here: nan.
   const size_t ngroups = mbx_wah_groups_of(t.nrows);
-  uint32_t *wgroups = nullptr, *wres[2] = {nullptr, nullptr}, *wex = nullptr;
+  const size_t nskip = mbx_wah_skip_len(ngroups);
+  uint32_t *wgroups = nullptr, *wres[2] = {nullptr, nullptr}, *wex = nullptr,
+           *wsk = nullptr;
   if (mbx_alloc(ctx, 4 * ngroups, (void **)&wgroups) != MBX_OK ||
       mbx_alloc(ctx, 4 * ngroups, (void **)&wres[0]) != MBX_OK ||
       mbx_alloc(ctx, 4 * ngroups, (void **)&wres[1]) != MBX_OK ||
-      mbx_alloc(ctx, 4 * (ngroups + 1), (void **)&wex) != MBX_OK) {
+      mbx_alloc(ctx, 4 * (ngroups + 1), (void **)&wex) != MBX_OK ||
+      mbx_alloc(ctx, 4 * (nskip ? nskip : 1), (void **)&wsk) != MBX_OK) {
     std::fprintf(stderr, "alloc: %s\n", mbx_last_error());
     return 2;
   }
@@ -711,27 +721,23 @@ int main(int argc, char **argv) {
     float ms = NAN;
     if (ok) {
       ms = timed([&] {
-        const uint32_t *cur = w[0].w, *cur_ex = w[0].ex;
-        size_t cur_n = w[0].n;
+        mbx_wah cur = w[0].arg();
         int buf = 0;
         for (const auto &st : steps) {
-          const Wah &o = w[st.first];
-          const uint32_t *up = st.second ? cur : o.w;
-          const uint32_t *up_ex = st.second ? cur_ex : o.ex;
-          const size_t nup = st.second ? cur_n : o.n;
-          const uint32_t *dn = st.second ? o.w : cur;
-          const uint32_t *dn_ex = st.second ? o.ex : cur_ex;
-          const size_t ndn = st.second ? o.n : cur_n;
-          int rc = mbx_wah_and_no1(ctx, up, up_ex, nup, dn, dn_ex, ndn, wres[buf]);
+          const mbx_wah o = w[st.first].arg();
+          const mbx_wah &up = st.second ? cur : o;
+          const mbx_wah dn = st.second ? o : cur;
+          int rc = mbx_wah_and_no1_skip(ctx, &up, &dn, wres[buf]);
           if (rc != MBX_OK) return rc;
-          cur = wres[buf];
-          cur_ex = dn_ex;   // the result keeps down's word structure
-          cur_n = ndn;
+          // the result keeps down's word structure: its scan and table
+          cur = dn;
+          cur.words = wres[buf];
           buf ^= 1;
         }
-        int rc = mbx_wah_scan(ctx, cur, cur_n, wex);
+        int rc = mbx_wah_scan_skip(ctx, cur.words, cur.n, wex, wsk, nskip);
         if (rc != MBX_OK) return rc;
-        return mbx_wah_decompress(ctx, cur, wex, cur_n, wgroups, ngroups);
+        const mbx_wah res{cur.words, wex, wsk, cur.n, nskip};
+        return mbx_wah_decompress_skip(ctx, &res, wgroups, ngroups);
       });
     }
     wah_ms.push_back(ms);
