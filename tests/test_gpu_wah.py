@@ -375,6 +375,33 @@ def test_wah_errors(dev):
     assert dev.lib.mbx_wah_scan(dev.ctx, g.ptr, 1 << 30, g.ptr) == abi.ERR_CAPACITY
     assert dev.lib.mbx_wah_and_no1(dev.ctx, None, g.ptr, 5, g.ptr, g.ptr, 5,
                                    g.ptr) == abi.ERR_INVALID
+    # the skip-table entry points: a table of entries without a pointer,
+    # misaligned tables, sizes past a fill's count, null arguments
+    assert dev.lib.mbx_wah_scan_skip(dev.ctx, g.ptr, 10, g.ptr, None, 4) \
+        == abi.ERR_INVALID
+    assert dev.lib.mbx_wah_scan_skip(dev.ctx, g.ptr, 10, g.ptr, g.ptr + 2, 4) \
+        == abi.ERR_ALIGN
+    assert dev.lib.mbx_wah_scan_skip(dev.ctx, g.ptr, 10, g.ptr, g.ptr,
+                                     1 << 30) == abi.ERR_CAPACITY
+    ok = abi.Wah()
+    ok.words, ok.excl, ok.skip, ok.n, ok.nskip = g.ptr, g.ptr, g.ptr, 5, 1
+    bad = abi.Wah()
+    bad.words, bad.excl, bad.skip, bad.n, bad.nskip = g.ptr, g.ptr, None, 5, 1
+    mis = abi.Wah()
+    mis.words, mis.excl, mis.skip, mis.n, mis.nskip = g.ptr, g.ptr + 2, g.ptr, 5, 1
+    for up, dn in ((bad, ok), (ok, bad)):
+        assert dev.lib.mbx_wah_and_no1_skip(dev.ctx, C.byref(up), C.byref(dn),
+                                            g.ptr) == abi.ERR_INVALID
+    assert dev.lib.mbx_wah_and_no1_skip(dev.ctx, C.byref(ok), C.byref(mis),
+                                        g.ptr) == abi.ERR_ALIGN
+    assert dev.lib.mbx_wah_and_no1_skip(dev.ctx, C.byref(ok), None, g.ptr) \
+        == abi.ERR_INVALID
+    assert dev.lib.mbx_wah_decompress_skip(dev.ctx, C.byref(bad), g.ptr, 10) \
+        == abi.ERR_INVALID
+    assert dev.lib.mbx_wah_decompress_skip(dev.ctx, C.byref(ok), None, 10) \
+        == abi.ERR_INVALID
+    assert dev.lib.mbx_wah_decompress_skip(dev.ctx, C.byref(mis), g.ptr, 10) \
+        == abi.ERR_ALIGN
     g.free()
 
 
