@@ -359,7 +359,21 @@ def test_wah_full_size_chain(dev):
     check(dev.lib.mbx_wah_decompress(dev.ctx, r.ptr, sr.ptr, ka, dec.ptr, ng))
     want = cpuref.wah_select(a, 0, 10) & cpuref.wah_select(b, 100, 400)
     np.testing.assert_array_equal(dec.host(), want)
-    for x in (da, db, ga, gb, wa, wb, sa, sb, r, sr, dec):
+    # the same chain through skip tables: the same words and groups
+    nsk = int(dev.lib.mbx_wah_skip_len(ng))
+    ea, ka_ = _scan_skip(dev, wa, ka, nsk)
+    eb, kb_ = _scan_skip(dev, wb, kb, nsk)
+    np.testing.assert_array_equal(ea.host(), sa.host())
+    r2 = dev.alloc(4 * ka, np.uint32, ka)
+    check(dev.lib.mbx_wah_and_no1_skip(dev.ctx, C.byref(_wah_ref(wb, eb, kb_, kb, nsk)),
+                                       C.byref(_wah_ref(wa, ea, ka_, ka, nsk)), r2.ptr))
+    np.testing.assert_array_equal(r2.host(), r.host())
+    er, kr = _scan_skip(dev, r2, ka, nsk)
+    check(dev.lib.mbx_fill_u32(dev.ctx, dec.ptr, 0xdeadbeef, ng))
+    check(dev.lib.mbx_wah_decompress_skip(dev.ctx, C.byref(_wah_ref(r2, er, kr, ka, nsk)),
+                                          dec.ptr, ng))
+    np.testing.assert_array_equal(dec.host(), want)
+    for x in (da, db, ga, gb, wa, wb, sa, sb, r, sr, dec, ea, ka_, eb, kb_, r2, er, kr):
         x.free()
 
 
