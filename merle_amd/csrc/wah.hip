@@ -361,9 +361,11 @@ __device__ __forceinline__ uint32_t wah_find(const uint32_t *__restrict__ excl,
 // every load and store is coalesced.  The words of the other operand that
 // cover the wave's groups are staged in LDS, kWahSpan words at a time
 // (their group offsets and their 31 rows), from the word covering the
-// wave's first group on; an item then costs a 10-step binary search in
-// LDS.  The items' groups increase with the item, so an item the staged
-// words do not reach is reached by a later chunk of words.
+// wave's first group on.  When one staging covers every item's group, the
+// owner scan (wah_owners) names each group's word at once; otherwise an
+// item costs a 10-step binary search in LDS, and as the items' groups
+// increase with the item, an item the staged words do not reach is
+// reached by a later chunk of words.
 constexpr uint32_t kWahSpan = kWahSkip;
 constexpr uint32_t kWahPerLane = kWahSpan / 64;
 
