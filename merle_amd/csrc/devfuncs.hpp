@@ -43,6 +43,15 @@ __device__ __forceinline__ bool fetch_src(const DevSrc &s, size_t row,
 // wave64 helpers ---------------------------------------------------------
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
 
+// Lane src's v for a wave-uniform src: two scalar lane reads, no LDS trip.
+__device__ __forceinline__ uint64_t wave_read64(uint64_t v, uint32_t src) {
+  const uint32_t lo =
+      (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, (int)src);
+  const uint32_t hi =
+      (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), (int)src);
+  return ((uint64_t)hi << 32) | lo;
+}
+
 // Sum of x over the 64 lanes of a wave (every lane gets it).
 __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long x) {
 #pragma unroll

@@ -655,106 +655,6 @@ __device__ __forceinline__ void keyval_of(const DevOp &O,
   ok = vok && k >= 0 && k < (int64_t)nr_grp;
 }
 
-// Index of a source for fact row r: r itself, or its foreign key.  Every
-// lane loads (the branch is on the source, which is wave-uniform).
-__device__ __forceinline__ uint32_t fetch_index(const DevSrc &s, uint32_t r) {
-  return s.fk ? s.fk[r] : r;
-}
-
-// Value of a source at index ix; ok = false past the dimension, where the
-// lane reads entry 0 instead (dropped), so the load is never conditional on
-// the lane.  A source with an empty dimension reads nothing.
-__device__ __forceinline__ uint64_t fetch_value(const DevSrc &s, uint32_t ix,
-                                                bool &ok) {
-  if (s.fk && s.dimsz == 0) {
-    ok = false;
-    return 0;
-  }
-  ok = s.fk == nullptr || ix < s.dimsz;
-  return ld_elem(s.data, s.eb, ok ? ix : 0u, false);
-}
-
-// The row op of E three-valued entries in one or two rounds of loads
-// whatever the number of sources -- the foreign keys of every source, then
-// every value; one round when no source goes through a foreign key: the
-// predicate values an uncertain row's re-check reads are fetched for every
-// entry together with its key parts and measures, where process_entries
-// re-checks each uncertain row inside a lane-dependent branch (its loads
-// waited for there) before the key fetch: up to E + 1 rounds.  An invalid
-// entry reads row 0 and is dropped; a certain row reads its predicate
-// values for nothing.  The columns in P.known are not read.  Used by the
-// word-by-word row op of dense ranges (an edge bin over a skewed column's
-// head: most rows uncertain), where those extra loads are coalesced.
-template <int E>
-__device__ __forceinline__ void rows_fetch_chk(const DevPrg &P, const DevOp &O,
-                                               const uint32_t *ent,
-                                               const bool *valid,
-                                               uint32_t nr_grp, uint32_t *key,
-                                               int64_t *val, bool *ok) {
-  constexpr int NS = MBX_MAXKEYS + 2;   // key parts, then measure a, b
-  const bool has_a = O.measure != MBX_MEAS_COUNT;
-  const bool has_b = O.measure == MBX_MEAS_MUL || O.measure == MBX_MEAS_SUB;
-  const uint32_t need = ~P.known & ((1u << O.npreds) - 1u);
-  uint32_t row[E];
-  bool unc[E];
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-    row[e] = valid[e] ? (ent[e] & 0x7fffffffu) : 0u;
-    unc[e] = valid[e] && (ent[e] >> 31) != 0;
-  }
-  uint32_t pix[E][MBX_MAXCOLS], kix[E][NS];
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-#pragma unroll
-    for (int c = 0; c < MBX_MAXCOLS; ++c)
-      if ((need >> c) & 1u) pix[e][c] = fetch_index(O.pred[c], row[e]);
-#pragma unroll
-    for (int i = 0; i < MBX_MAXKEYS; ++i)
-      if (i < (int)O.nkeys) kix[e][i] = fetch_index(O.keys[i].src, row[e]);
-    if (has_a) kix[e][3] = fetch_index(O.ma, row[e]);
-    if (has_b) kix[e][4] = fetch_index(O.mb, row[e]);
-  }
-  uint64_t pv[E][MBX_MAXCOLS], kv[E][NS];
-  bool pok[E][MBX_MAXCOLS], vok[E];
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-#pragma unroll
-    for (int c = 0; c < MBX_MAXCOLS; ++c)
-      if ((need >> c) & 1u) pv[e][c] = fetch_value(O.pred[c], pix[e][c], pok[e][c]);
-    vok[e] = valid[e];
-#pragma unroll
-    for (int i = 0; i < MBX_MAXKEYS; ++i)
-      if (i < (int)O.nkeys) {
-        bool o;
-        kv[e][i] = fetch_value(O.keys[i].src, kix[e][i], o);
-        vok[e] = vok[e] && o;
-      }
-    kv[e][3] = 1;
-    kv[e][4] = 0;
-    if (has_a) {
-      bool o;
-      kv[e][3] = fetch_value(O.ma, kix[e][3], o);
-      vok[e] = vok[e] && o;
-    }
-    if (has_b) {
-      bool o;
-      kv[e][4] = fetch_value(O.mb, kix[e][4], o);
-      vok[e] = vok[e] && o;
-    }
-  }
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-    uint32_t m = P.known_val;
-#pragma unroll
-    for (int c = 0; c < MBX_MAXCOLS; ++c)
-      if ((need >> c) & 1u)
-        m |= (pok[e][c] && pv[e][c] >= O.lo[c] && pv[e][c] < O.hi[c]) ? 1u << c
-                                                                     : 0u;
-    const bool pass = !unc[e] || ((P.truth >> m) & 1u) != 0;
-    keyval_of(O, kv[e], vok[e] && pass, nr_grp, key[e], val[e], ok[e]);
-  }
-}
-
 // {value, group} of E selected rows at once (primitive.cuh:10-15); ok[e] is
 // false for a dropped row (fk out of range or key outside [0, nr_grp)).
 template <bool ACCT, int E>
@@ -969,9 +869,7 @@ __device__ __forceinline__ void sink_wave(const GroupSink &sink, uint32_t key,
 
 // List entries: row id, MSB = uncertain (primitive.cuh:406-407).  E entries
 // per lane share their memory round trips.
-// MERGED (three-valued, timed path only): rows_fetch_chk's single set of
-// loads for re-checks and keys.
-template <bool CHK, bool ACCT, int E, bool MERGED = false>
+template <bool CHK, bool ACCT, int E>
 __device__ __forceinline__ void process_entries(const DevPrg &P,
                                                 const DevOp &O,
                                                 uint32_t nr_grp,
@@ -982,20 +880,16 @@ __device__ __forceinline__ void process_entries(const DevPrg &P,
   uint32_t key[E];
   int64_t val[E];
   bool ok[E];
-  if constexpr (MERGED && CHK && !ACCT) {
-    rows_fetch_chk<E>(P, O, ent, valid, nr_grp, key, val, ok);
-  } else {
-    uint32_t row[E];
-    bool live[E];
+  uint32_t row[E];
+  bool live[E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      row[e] = ent[e] & 0x7fffffffu;
-      live[e] = valid[e];
-      if (CHK && live[e] && (ent[e] >> 31))
-        live[e] = row_passes<ACCT>(P, O, row[e], acct);
-    }
-    rows_keyval<ACCT, E>(O, row, live, nr_grp, key, val, ok, acct);
+  for (int e = 0; e < E; ++e) {
+    row[e] = ent[e] & 0x7fffffffu;
+    live[e] = valid[e];
+    if (CHK && live[e] && (ent[e] >> 31))
+      live[e] = row_passes<ACCT>(P, O, row[e], acct);
   }
+  rows_keyval<ACCT, E>(O, row, live, nr_grp, key, val, ok, acct);
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     if (ACCT) {
@@ -1118,8 +1012,52 @@ constexpr uint32_t kWaveList = kListCap / 4;
 // uncertain (an edge bin over a skewed column's head) that serial fill cost
 // more than the row op itself.
 constexpr uint32_t kDenseRowsPerWord = 16;
-// Words per round of the dense path's three-valued row op (rows_fetch_chk).
-constexpr int kDenseChkE = 2;
+// Words per round of the dense re-check (recheck_words).
+constexpr int kDenseChkE = 1;
+
+// The three-valued words of a dense range (an edge bin over a skewed
+// column's head: most rows uncertain, few passing) made exact: each word's
+// uncertain rows are re-checked, kDenseChkE words a round, lane b taking
+// bit b of a word, so a predicate column's loads for a word are one
+// coalesced 256-B access; p[k] becomes the word's passing rows and u[k] 0.
+// The row op then runs on the passing rows only, through the list path.
+// Kept apart from the row op so that the loop holds the predicates'
+// descriptors only.
+template <int WPL>
+__device__ __forceinline__ void recheck_words(const DevPrg &P, const DevOp &O,
+                                              uint32_t w0, uint32_t ws,
+                                              uint64_t *p, uint64_t *u) {
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (int k = 0; k < WPL; ++k) {
+    const uint32_t wk = w0 + ws * k;   // lane s holds word wk + s
+    uint64_t mine = 0;                 // lane s: word wk + s's passing rows
+    for (uint32_t s = 0; s < 64u; s += kDenseChkE) {
+      uint32_t row[kDenseChkE];
+      bool cand[kDenseChkE], ok[kDenseChkE];
+#pragma unroll
+      for (int e = 0; e < kDenseChkE; ++e) {
+        const uint32_t src = s + (uint32_t)e;
+        const uint64_t pw = wave_read64(p[k], src);
+        const uint64_t uw = wave_read64(u[k], src);
+        cand[e] = (pw >> lane) & 1ull;
+        ok[e] = cand[e] && ((uw >> lane) & 1ull);   // to re-check
+        row[e] = (wk + src) * 64u + lane;
+      }
+      bool chk[kDenseChkE];
+#pragma unroll
+      for (int e = 0; e < kDenseChkE; ++e) chk[e] = ok[e];
+      rows_pass<false, true, kDenseChkE>(P, O, row, ok, nullptr);
+#pragma unroll
+      for (int e = 0; e < kDenseChkE; ++e) {
+        const uint64_t m = __ballot(cand[e] && (!chk[e] || ok[e]));
+        if (lane == s + (uint32_t)e) mine = m;
+      }
+    }
+    p[k] = mine;
+    u[k] = 0;
+  }
+}
 
 // SCAN: the words come from the rows' exact predicates (scan_words), not
 // from bitmaps.
@@ -1167,8 +1105,20 @@ __device__ __forceinline__ void run_wave_range(const DevPrg &P, const DevOp &O,
   uint32_t cnt = 0;
 #pragma unroll
   for (int k = 0; k < WPL; ++k) cnt += __popcll(p[k]);
-  const uint32_t incl = wave_incl_scan(cnt);
-  const uint32_t total = __shfl(incl, 63, 64);
+  uint32_t incl = wave_incl_scan(cnt);
+  uint32_t total = __shfl(incl, 63, 64);
+  if constexpr (CHK && !ACCT) {
+    if (total >= kDenseRowsPerWord * 64u * WPL) {
+      // dense three-valued: exact words first, then the row op on the
+      // passing rows (the list path, or the dense path if they are dense)
+      recheck_words<WPL>(P, O, w0, ws, p, u);
+      cnt = 0;
+#pragma unroll
+      for (int k = 0; k < WPL; ++k) cnt += __popcll(p[k]);
+      incl = wave_incl_scan(cnt);
+      total = __shfl(incl, 63, 64);
+    }
+  }
   const uint32_t off = incl - cnt;
   if (total >= kDenseRowsPerWord * 64u * WPL) {
     // Dense: word by word (E at a time), lane b takes bit b, so a column's
@@ -1186,10 +1136,7 @@ __device__ __forceinline__ void run_wave_range(const DevPrg &P, const DevOp &O,
       }
       wave_lds_sync();
     }
-    // three-valued: kDenseChkE words at a time, their re-check and key
-    // loads in one set (rows_fetch_chk)
-    constexpr bool kMerge = CHK && !ACCT;
-    constexpr int ED = kMerge ? kDenseChkE : E;
+    constexpr int ED = E;
 #pragma unroll
     for (int k = 0; k < WPL; ++k) {
       const uint32_t wk = w0 + ws * k;   // lane s holds word wk + s
@@ -1212,8 +1159,7 @@ __device__ __forceinline__ void run_wave_range(const DevPrg &P, const DevOp &O,
           anyv = anyv || valid[e];
         }
         if (__ballot(anyv) == 0) continue;
-        process_entries<CHK, ACCT, ED, kMerge>(P, O, nr_grp, ent, valid, sink,
-                                               acct);
+        process_entries<CHK, ACCT, ED>(P, O, nr_grp, ent, valid, sink, acct);
       }
     }
     if (kStage) wave_lds_sync();   // the list space is rewritten next range

@@ -34,6 +34,7 @@ inline unsigned long long __umul64hi(unsigned long long a, unsigned long long b)
 }
 inline unsigned __shfl_up(unsigned v, int, int) { return v; }
 inline unsigned __shfl(unsigned v, int, int) { return v; }
+#define __builtin_amdgcn_readlane(v, l) (v)
 inline void __syncthreads() {}
 #define __builtin_amdgcn_fence(a, b)
 #define __builtin_amdgcn_wave_barrier()
