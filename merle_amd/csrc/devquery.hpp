@@ -1012,14 +1012,12 @@ constexpr uint32_t kWaveList = kListCap / 4;
 // uncertain (an edge bin over a skewed column's head) that serial fill cost
 // more than the row op itself.
 constexpr uint32_t kDenseRowsPerWord = 16;
-// Words per round of the dense re-check (recheck_words).
-constexpr int kDenseChkE = 1;
 
 // The three-valued words of a dense range (an edge bin over a skewed
-// column's head: most rows uncertain, few passing) made exact: each word's
-// uncertain rows are re-checked, kDenseChkE words a round, lane b taking
-// bit b of a word, so a predicate column's loads for a word are one
-// coalesced 256-B access; p[k] becomes the word's passing rows and u[k] 0.
+// column's head: most rows uncertain, few passing) made exact: a word with
+// uncertain rows is re-checked, lane b taking bit b, so a predicate
+// column's loads for the word are one coalesced 256-B access; p[k] becomes
+// the word's passing rows and u[k] 0.
 // The row op then runs on the passing rows only, through the list path.
 // Kept apart from the row op so that the loop holds the predicates'
 // descriptors only.
@@ -1028,31 +1026,38 @@ __device__ __forceinline__ void recheck_words(const DevPrg &P, const DevOp &O,
                                               uint32_t w0, uint32_t ws,
                                               uint64_t *p, uint64_t *u) {
   const uint32_t lane = lane_id();
+  const uint32_t need = ~P.known & ((1u << O.npreds) - 1u);
 #pragma unroll
   for (int k = 0; k < WPL; ++k) {
     const uint32_t wk = w0 + ws * k;   // lane s holds word wk + s
     uint64_t mine = 0;                 // lane s: word wk + s's passing rows
-    for (uint32_t s = 0; s < 64u; s += kDenseChkE) {
-      uint32_t row[kDenseChkE];
-      bool cand[kDenseChkE], ok[kDenseChkE];
+    for (uint32_t s = 0; s < 64u; ++s) {
+      const uint64_t pw = wave_read64(p[k], s);
+      const uint64_t uw = wave_read64(u[k], s);
+      const bool cand = (pw >> lane) & 1ull;
+      uint64_t m;
+      if ((pw & uw) == 0) {
+        m = pw;   // no uncertain row in the word (wave-uniform)
+      } else {
+        // every lane reads its row's predicate columns, so the loads are
+        // branch-free (a row past the table reads the last row; it is no
+        // candidate)
+        const uint32_t rc = min((wk + s) * 64u + lane, P.nrows - 1u);
+        uint32_t bits = P.known_val;
 #pragma unroll
-      for (int e = 0; e < kDenseChkE; ++e) {
-        const uint32_t src = s + (uint32_t)e;
-        const uint64_t pw = wave_read64(p[k], src);
-        const uint64_t uw = wave_read64(u[k], src);
-        cand[e] = (pw >> lane) & 1ull;
-        ok[e] = cand[e] && ((uw >> lane) & 1ull);   // to re-check
-        row[e] = (wk + src) * 64u + lane;
+        for (int c = 0; c < MBX_MAXCOLS; ++c) {
+          if (!((need >> c) & 1u)) continue;
+          const DevSrc &sc = O.pred[c];
+          if (sc.fk && sc.dimsz == 0) continue;   // no value: false
+          const uint32_t ix = sc.fk ? sc.fk[rc] : rc;
+          const bool okv = sc.fk == nullptr || ix < sc.dimsz;
+          const uint64_t v = ld_elem(sc.data, sc.eb, okv ? ix : 0u, false);
+          bits |= (okv && v >= O.lo[c] && v < O.hi[c]) ? 1u << c : 0u;
+        }
+        const bool unc = (uw >> lane) & 1ull;
+        m = __ballot(cand && (!unc || ((P.truth >> bits) & 1u)));
       }
-      bool chk[kDenseChkE];
-#pragma unroll
-      for (int e = 0; e < kDenseChkE; ++e) chk[e] = ok[e];
-      rows_pass<false, true, kDenseChkE>(P, O, row, ok, nullptr);
-#pragma unroll
-      for (int e = 0; e < kDenseChkE; ++e) {
-        const uint64_t m = __ballot(cand[e] && (!chk[e] || ok[e]));
-        if (lane == s + (uint32_t)e) mine = m;
-      }
+      if (lane == s) mine = m;
     }
     p[k] = mine;
     u[k] = 0;
