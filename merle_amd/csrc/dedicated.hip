@@ -49,6 +49,7 @@ namespace mbx {
 template <uint32_t CODE>
 struct StaticShape {
   static constexpr bool kNT = true;   // bitmap words: streaming loads
+  static constexpr uint32_t kRecheck = 4;   // words a round of the dense re-check
   static constexpr uint32_t clause(uint32_t k) { return (CODE >> (6 * k)) & 63u; }
   static constexpr uint32_t entries(uint32_t k) {
     return (clause(k) & 7u) + ((clause(k) >> 3) & 1u) + ((clause(k) >> 4) & 1u);

@@ -218,6 +218,8 @@ __device__ __forceinline__ void apply_ins(uint32_t op, uint32_t d, uint32_t s,
 // argument).
 struct RuntimeShape {
   static constexpr bool kNT = true;   // bitmap words: streaming loads
+  // words a round of the dense re-check (recheck_words)
+  static constexpr uint32_t kRecheck = 4;
   static __device__ __forceinline__ uint32_t nfl(const DevPrg &P) { return P.nfl; }
   static __device__ __forceinline__ bool edge(const DevPrg &P, uint32_t e) {
     return P.fledge[e] != 0;
@@ -229,9 +231,12 @@ struct RuntimeShape {
     return P.flneg[e] != 0;
   }
 };
-// The same, bitmap words through cached loads (the batched launch).
+// The same, bitmap words through cached loads (the batched launch), and a
+// word a round of the dense re-check (its conjunction kernel is held to 80
+// VGPRs: two words spill).
 struct RuntimeShapeCached : RuntimeShape {
   static constexpr bool kNT = false;
+  static constexpr uint32_t kRecheck = 1;
 };
 
 // EAGER 3: a conjunction of clauses (P.conj).  in[k] / keep[k]: word k is
@@ -1021,7 +1026,8 @@ constexpr uint32_t kDenseRowsPerWord = 16;
 // The row op then runs on the passing rows only, through the list path.
 // Kept apart from the row op so that the loop holds the predicates'
 // descriptors only.
-template <int WPL>
+// R words a round: their loads are all in flight before the first compare.
+template <int WPL, uint32_t R>
 __device__ __forceinline__ void recheck_words(const DevPrg &P, const DevOp &O,
                                               uint32_t w0, uint32_t ws,
                                               uint64_t *p, uint64_t *u) {
@@ -1031,33 +1037,55 @@ __device__ __forceinline__ void recheck_words(const DevPrg &P, const DevOp &O,
   for (int k = 0; k < WPL; ++k) {
     const uint32_t wk = w0 + ws * k;   // lane s holds word wk + s
     uint64_t mine = 0;                 // lane s: word wk + s's passing rows
-    for (uint32_t s = 0; s < 64u; ++s) {
-      const uint64_t pw = wave_read64(p[k], s);
-      const uint64_t uw = wave_read64(u[k], s);
-      const bool cand = (pw >> lane) & 1ull;
-      uint64_t m;
-      if ((pw & uw) == 0) {
-        m = pw;   // no uncertain row in the word (wave-uniform)
-      } else {
-        // every lane reads its row's predicate columns, so the loads are
+    for (uint32_t s = 0; s < 64u; s += R) {
+      uint64_t pw[R], uw[R];
+      bool anyu = false;
+#pragma unroll
+      for (uint32_t e = 0; e < R; ++e) {
+        pw[e] = wave_read64(p[k], s + e);
+        uw[e] = wave_read64(u[k], s + e);
+        anyu = anyu || (pw[e] & uw[e]) != 0;   // wave-uniform
+      }
+      uint64_t m[R];
+#pragma unroll
+      for (uint32_t e = 0; e < R; ++e) m[e] = pw[e];
+      if (anyu) {
+        // every lane reads its rows' predicate columns, so the loads are
         // branch-free (a row past the table reads the last row; it is no
         // candidate)
-        const uint32_t rc = min((wk + s) * 64u + lane, P.nrows - 1u);
-        uint32_t bits = P.known_val;
+        uint32_t rc[R];
+#pragma unroll
+        for (uint32_t e = 0; e < R; ++e)
+          rc[e] = min((wk + s + e) * 64u + lane, P.nrows - 1u);
+        uint32_t bits[R];
+#pragma unroll
+        for (uint32_t e = 0; e < R; ++e) bits[e] = P.known_val;
 #pragma unroll
         for (int c = 0; c < MBX_MAXCOLS; ++c) {
           if (!((need >> c) & 1u)) continue;
           const DevSrc &sc = O.pred[c];
-          if (sc.fk && sc.dimsz == 0) continue;   // no value: false
-          const uint32_t ix = sc.fk ? sc.fk[rc] : rc;
-          const bool okv = sc.fk == nullptr || ix < sc.dimsz;
-          const uint64_t v = ld_elem(sc.data, sc.eb, okv ? ix : 0u, false);
-          bits |= (okv && v >= O.lo[c] && v < O.hi[c]) ? 1u << c : 0u;
+          uint64_t v[R];
+          bool okv[R];
+#pragma unroll
+          for (uint32_t e = 0; e < R; ++e) {
+            const uint32_t ix = sc.fk ? sc.fk[rc[e]] : rc[e];
+            okv[e] = !(sc.fk && (sc.dimsz == 0 || ix >= sc.dimsz));
+            v[e] = ld_elem(sc.data, sc.eb, okv[e] ? ix : 0u, false);
+          }
+#pragma unroll
+          for (uint32_t e = 0; e < R; ++e)
+            bits[e] |= (okv[e] && v[e] >= O.lo[c] && v[e] < O.hi[c]) ? 1u << c : 0u;
         }
-        const bool unc = (uw >> lane) & 1ull;
-        m = __ballot(cand && (!unc || ((P.truth >> bits) & 1u)));
+#pragma unroll
+        for (uint32_t e = 0; e < R; ++e) {
+          const bool cand = (pw[e] >> lane) & 1ull;
+          const bool unc = (uw[e] >> lane) & 1ull;
+          m[e] = __ballot(cand && (!unc || ((P.truth >> bits[e]) & 1u)));
+        }
       }
-      if (lane == s) mine = m;
+#pragma unroll
+      for (uint32_t e = 0; e < R; ++e)
+        if (lane == s + e) mine = m[e];
     }
     p[k] = mine;
     u[k] = 0;
@@ -1116,7 +1144,7 @@ __device__ __forceinline__ void run_wave_range(const DevPrg &P, const DevOp &O,
     if (total >= kDenseRowsPerWord * 64u * WPL) {
       // dense three-valued: exact words first, then the row op on the
       // passing rows (the list path, or the dense path if they are dense)
-      recheck_words<WPL>(P, O, w0, ws, p, u);
+      recheck_words<WPL, F::kRecheck>(P, O, w0, ws, p, u);
       cnt = 0;
 #pragma unroll
       for (int k = 0; k < WPL; ++k) cnt += __popcll(p[k]);
