@@ -295,6 +295,57 @@ def learned_queries(dev, sch, bins, bounds, qs, specs, train_specs):
     return out, note, kinds
 
 
+def run_serving_alone(dev, nreq=1000, reps=3):
+    """The config-5 requests one by one, each alone on the device under its
+    static heuristic plan (best of `reps` launches between an event pair):
+    the service times the serving replays queue behind, and the slowest
+    requests -- those whose edge bin covers a Zipf head -- with their
+    algorithmic bytes."""
+    from merle_amd import abi, configs
+    from merle_amd.host import Timer, check
+    sch = configs.config5_device(dev)
+    bins, slabs = configs.build_bins(dev, sch, configs.C5_BOUNDS)
+    specs = configs.config5_queries(nreq)
+    hq = configs.heuristic_queries(
+        dev, configs.query_array(sch, specs, bins, configs.C5_BOUNDS))
+    out = dev.alloc(8 * 10_000, np.int64, 10_000)
+    t = Timer(dev)
+    ms = np.zeros(nreq)
+    for i in range(nreq):
+        q = hq[i]
+
+        def run():
+            check(dev.lib.mbx_prog_grpby(dev.ctx, C.byref(q.prg), C.byref(q.op),
+                                         q.nr_grp, out.ptr,
+                                         abi.PLAN_FUSED | q.flags))
+        run()
+        best = np.inf
+        for _ in range(reps):
+            t.start()
+            run()
+            best = min(best, t.stop())
+        ms[i] = best
+    slow = []
+    for i in np.argsort(-ms)[:5]:
+        q = hq[int(i)]
+        b, k = C.c_uint64(), C.c_uint64()
+        check(dev.lib.mbx_alg_bytes(dev.ctx, C.byref(q.prg), C.byref(q.op),
+                                    q.nr_grp, C.byref(b), C.byref(k)))
+        slow.append({"request": int(i), "ms": round(float(ms[i]), 3),
+                     "alg_GB": round(b.value / 1e9, 3), "selected": k.value,
+                     "alg_TB_per_s": round(b.value / ms[i] / 1e9, 2)})
+    t.close()
+    out.free()
+    for a in slabs + [c.dev for c in sch.cols.values()]:
+        a.free()
+    return {"workload": f"{nreq} config-5 requests over {sch.nrows:,} rows, each "
+                        "alone under its heuristic plan",
+            "p50_ms": round(float(np.percentile(ms, 50)), 3),
+            "p99_ms": round(float(np.percentile(ms, 99)), 3),
+            "max_ms": round(float(ms.max()), 3),
+            "mean_ms": round(float(ms.mean()), 4), "slowest": slow}
+
+
 def run_serving(dev, nreq=1000, rates=(2000.0, 20000.0), budget=256 << 20,
                 tight=256 << 10, train_requests=600):
     """Config 5: 600M rows x 10 Zipf columns + 640 bucket bitmaps resident;
@@ -626,13 +677,18 @@ def main():
                     help="run only the config-3 batched section (profiling)")
     ap.add_argument("--wah-only", action="store_true",
                     help="config 2 and the WAH section alone (profiling)")
+    ap.add_argument("--serving-alone", action="store_true",
+                    help="only the config-5 requests timed one by one (profiling)")
     args = ap.parse_args()
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration, max(1, args.steps))
-    if args.batched_only:
+    if args.batched_only or args.serving_alone:
         from merle_amd.host import Device
         dev = Device(0)
-        print(json.dumps({"batched": run_batched(dev)}), flush=True)
+        if args.batched_only:
+            print(json.dumps({"batched": run_batched(dev)}), flush=True)
+        else:
+            print(json.dumps({"serving_alone": run_serving_alone(dev)}), flush=True)
         dev.close()
         return 0
     if args.gpus != 1:
