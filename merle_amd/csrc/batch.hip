@@ -38,7 +38,14 @@ struct BatchArgs {
   uint32_t *counter;    // nx counters, kXcdCounterStride apart
 };
 
-template <bool CHK, int EAGER>
+// Bitmap words through cached loads, R words a round of the dense re-check
+// (the conjunction kernel, held to 80 VGPRs, takes one).
+template <uint32_t R>
+struct BatchItemShape : RuntimeShapeCached {
+  static constexpr uint32_t kRecheck = R;
+};
+
+template <bool CHK, int EAGER, uint32_t R = 1>
 __device__ __forceinline__ void batch_item(const BatchQ &q, uint32_t w_begin,
                                            uint32_t w_end, const GroupSink &sink,
                                            uint32_t *s_list) {
@@ -47,7 +54,7 @@ __device__ __forceinline__ void batch_item(const BatchQ &q, uint32_t w_begin,
   constexpr int W = kBWPL;
   const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   for (uint32_t w0 = w_begin + wv * 64; w0 < w_end; w0 += 4 * 64 * W)
-    run_wave_range<CHK, false, W, EAGER, 2, false, false, RuntimeShapeCached>(
+    run_wave_range<CHK, false, W, EAGER, 2, false, false, BatchItemShape<R>>(
         q.prg, q.op, q.nr_grp, w0, 4 * 64, w_end, sink, s_list, nullptr);
 }
 
@@ -92,9 +99,9 @@ __attribute__((amdgpu_waves_per_eu(CONJ ? 6 : 1, 8))) void batch_kernel(BatchArg
       else batch_item<false, 3>(q, w_begin, w_end, sink, s_list[wv]);
     } else if (q.chk) {
       if (q.eager && q.prg.conj)
-        batch_item<true, 3>(q, w_begin, w_end, sink, s_list[wv]);
+        batch_item<true, 3, 4>(q, w_begin, w_end, sink, s_list[wv]);
       else
-        batch_item<true, 0>(q, w_begin, w_end, sink, s_list[wv]);
+        batch_item<true, 0, 4>(q, w_begin, w_end, sink, s_list[wv]);
     } else if (q.eager && q.prg.conj) {
       batch_item<false, 3>(q, w_begin, w_end, sink, s_list[wv]);
     } else if (q.eager && q.prg.nfl <= kDirectOps) {
