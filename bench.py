@@ -352,9 +352,9 @@ def run_serving(dev, nreq=1000, rates=(2000.0, 20000.0), budget=256 << 20,
     Poisson arrivals replayed against the server at a moderate and at a
     saturating rate, once with continuous batching (4 streams; up to 4 small
     requests per dispatch, large and scan requests alone as launches of
-    their own) and once one request at a time, under the plans a cost model trained on other
-    requests picks (the scan for requests whose edge bins leave most rows to
-    re-check) and under the static heuristic plans; then once more at the
+    their own) and once one request at a time, under the plans a cost model
+    trained on other requests picks and under the static heuristic plans;
+    then once more at the
     saturating rate with a budget of the server's fixed memory plus `tight`
     bytes, so admission binds.  Every request's result must be identical in
     every replay."""

@@ -10,7 +10,8 @@ runs the cheapest; the static heuristic (sparsest column first, fused) is
 the fallback and the yardstick: every plan returns identical sums.  A share
 of the programs put a misaligned range over the head of a skewed column, so
 that an edge bin leaves most rows to re-check -- the case where the
-heuristic's fused plan loses to the scan.
+heuristic's fused plan lost to the scan before the fused kernels re-checked
+dense ranges into exact words ahead of the row op; now the two are close.
 """
 import ctypes as C
 import math
